@@ -1,0 +1,89 @@
+/*
+ * libxstream-amd extensions: entry points with no one-to-one counterpart in the reference.
+ *
+ * They expose what the reference keeps behind environment variables, OpenCL event profiling or
+ * LIBXS: explicit kernel configurations (what samples/smm/tune_multiply.py sets through
+ * OPENCL_LIBSMM_SMM_* variables, tune_multiply.py:278-298), the per-launch work/duration records of
+ * LIBXSTREAM_PROFILE (src/libxstream.c:2629-2781), the tuned-parameter CSV
+ * (samples/smm/smm_params.c:51-270) and native HIP handles.
+ */
+#ifndef LIBXSTREAM_AMD_H
+#define LIBXSTREAM_AMD_H
+
+#include "acc_libsmm.h"
+#include <stdio.h>
+
+/**
+ * Stack-kernel configuration for one (datatype, m, n, k). MI355X-native parameters: the reference's
+ * (BS, BM, BN, ...) tuple of samples/smm/smm_acc_opencl.h:60-66 describes OpenCL work-items and is
+ * not meaningful for these kernels.
+ *   variant: 0 = vector-FMA kernel, operands staged through LDS
+ *   tm, tn:  C microtile (rows x columns) owned by one thread
+ *   nt:      threads per work-group (64, 128, 256 or 512); the threads left after covering one
+ *            C block with microtiles form further groups that share the run's stack entries
+ *   bs:      nominal stack entries per work-group; a work-group owns the C-runs whose first entry
+ *            falls in its range and finishes them, so each C block is written by one work-group
+ *   kt:      k-slice staged in LDS at once (0 = all of k when it fits)
+ *   flush:   0 = automatic, 1 = plain read-modify-write (stack grouped by C), 2 = atomic add
+ */
+typedef struct libsmm_acc_smm_config_t {
+  int variant, tm, tn, nt, bs, kt, flush;
+  double gflops; /* measured (autotuner), predicted (learned model) or 0 */
+} libsmm_acc_smm_config_t;
+
+/** where a configuration came from */
+typedef enum libsmm_acc_config_source_t {
+  libsmm_acc_config_default = 0, /* built-in heuristic */
+  libsmm_acc_config_tuned = 1,   /* registry: CSV file, libsmm_acc_config_set or autotuner */
+  libsmm_acc_config_env = 2,     /* OPENCL_LIBSMM_SMM_* environment overrides */
+  libsmm_acc_config_learned = 3  /* learned predictor */
+} libsmm_acc_config_source_t;
+
+/** configuration libsmm_acc_process would use for this shape; *source may be NULL */
+LIBXSTREAM_API int libsmm_acc_config_get(libsmm_acc_data_t datatype, int m, int n, int k, libsmm_acc_smm_config_t* config,
+  int* source);
+/** register a configuration for this shape (replaces any existing one) */
+LIBXSTREAM_API int libsmm_acc_config_set(libsmm_acc_data_t datatype, int m, int n, int k,
+  const libsmm_acc_smm_config_t* config);
+/** returns EXIT_SUCCESS if config can be launched for this shape (compiled variant, limits) */
+LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, int n, int k,
+  const libsmm_acc_smm_config_t* config);
+/** libsmm_acc_process with an explicit configuration (the autotuner's entry point) */
+LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* config, const int* dev_param_stack,
+  int stack_size, libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m,
+  int n, int k, void* stream);
+/** compiled kernel variants for datatype: returns their number and fills up to cap (tm, tn, nt) triplets */
+LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* tm_tn_nt, int cap);
+/** load tuned parameters from a CSV file (reference or MI355X header); returns #rows loaded or <0 */
+LIBXSTREAM_API int libsmm_acc_params_load(const char* filename);
+/** write the registry as an MI355X CSV (header line + one row per shape); returns #rows or <0 */
+LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* device);
+
+/**
+ * Parse one CSV row of the reference's tuned-parameter format (samples/smm/smm_params.c:89-270):
+ * [DEVICE;]TYPEID;M;N;K[;S;GFLOPS];BS;BM;BN;BK;WS;WG;LU;NZ;AL;TB;TC;AP;AA;AB;AC[;FLAGS].
+ * key[0..3] = type,m,n,k; params[0..15] = bs,bm,bn,bk,ws,wg,lu,nz,al,tb,tc,ap,aa,ab,ac,flags;
+ * s_gflops[0..1] = S, GFLOPS. with_device/with_perf select the optional columns.
+ * Returns EXIT_SUCCESS when every column parsed.
+ */
+LIBXSTREAM_API int libsmm_acc_read_ref_params(const char* line, int with_device, int with_perf, char* device,
+  size_t device_size, int key[4], int params[16], double s_gflops[2]);
+
+/** profiling: per-kernel launch records timed by HIP events on the launch stream */
+LIBXSTREAM_API int libxstream_profile_enable(int enable);
+LIBXSTREAM_API int libxstream_profile_reset(void);
+/** totals for one kernel name ("smm", "trans", ...); waits for outstanding records */
+LIBXSTREAM_API int libxstream_profile_get(const char* kernel, long* count, double* total_ms, double* total_flops,
+  double* total_bytes);
+LIBXSTREAM_API int libxstream_profile_print(FILE* stream);
+
+/** last error on the calling thread: returns its code (0 = none) and clears it */
+LIBXSTREAM_API int libxstream_error_consume(char* message, size_t size);
+/** native handles */
+LIBXSTREAM_API int libxstream_stream_native(libxstream_stream_t* stream, void** hip_stream);
+/** milliseconds between two recorded events (both must have completed) */
+LIBXSTREAM_API int libxstream_event_elapsed(libxstream_event_t* begin, libxstream_event_t* end, double* ms);
+/** device properties of the active device: number of CUs, name (gcnArchName) */
+LIBXSTREAM_API int libxstream_device_info(int* ncus, char* name, size_t name_size);
+
+#endif /*LIBXSTREAM_AMD_H*/

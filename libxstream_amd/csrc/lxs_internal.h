@@ -1,0 +1,56 @@
+/*
+ * Internal declarations shared by the C host layer (runtime, DBCSR shim, LIBSMM host code).
+ */
+#ifndef LXS_INTERNAL_H
+#define LXS_INTERNAL_H
+
+#include <hip/hip_runtime_api.h>
+#include <pthread.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include "../../include/libxstream_amd.h"
+#include "lxs_kernels.h"
+
+struct libxstream_stream_t {
+  hipStream_t hip;
+  int device;
+  int flags; /* libxstream_stream_flags_t as requested */
+  char name[32];
+};
+
+struct libxstream_event_t {
+  hipEvent_t hip;
+};
+
+typedef struct lxs_config_t {
+  int ninit;     /* reference count of libxstream_init */
+  int ndevices;  /* 0 when no HIP device is visible */
+  int device;    /* active device, -1 if none */
+  int verbosity;
+  int profile;   /* LIBXSTREAM_PROFILE */
+  int ncus;      /* compute units of the active device */
+  char arch[256]; /* gcnArchName of the active device */
+  pthread_mutex_t lock;
+} lxs_config_t;
+
+extern lxs_config_t lxs_cfg;
+
+/** record code/message as the calling thread's last error; prints at verbosity >= 1; returns code */
+int lxs_error(int code, const char* what);
+/** map a hipError_t to the library's return convention (0 or the positive HIP code) */
+int lxs_hip(hipError_t code, const char* what);
+/** bind the calling thread to the active device (HIP keeps the current device per thread) */
+int lxs_ensure_device(void);
+/** HIP stream behind a library stream; NULL selects the calling thread's default stream */
+hipStream_t lxs_stream_hip(const libxstream_stream_t* stream);
+
+/** profiling around one launch: token is NULL when profiling is off */
+void* lxs_profile_begin(hipStream_t stream);
+void lxs_profile_end(void* token, hipStream_t stream, const char* kernel, double flops, double bytes);
+void lxs_profile_finalize(void);
+
+/** LIBSMM registry lifetime (smm.c) */
+int lxs_smm_init(void);
+void lxs_smm_finalize(void);
+
+#endif /*LXS_INTERNAL_H*/

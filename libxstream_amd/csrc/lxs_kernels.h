@@ -1,0 +1,49 @@
+/*
+ * Launch interface between the C host layer and the gfx950 HIP kernels (C ABI, plain pointers).
+ * Every launcher returns 0 on success, a positive hipError_t on a HIP failure, or a negative
+ * LXS_E* code when the request cannot be served by a compiled kernel.
+ */
+#ifndef LXS_KERNELS_H
+#define LXS_KERNELS_H
+
+#include <stddef.h>
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+#define LXS_ENOVARIANT (-2) /* no compiled kernel for (dtype, tm, tn, nt) */
+#define LXS_ELIMIT (-3)     /* configuration exceeds a kernel limit (LDS, threads) */
+
+/** data types use the DBCSR codes: 1 = real_4 (float), 3 = real_8 (double) */
+typedef struct lxs_smm_args_t {
+  void* c;
+  const void* a;
+  const void* b;
+  const int* stack;  /* (a,b,c) triplets, 1-based element offsets */
+  int stack_size;
+  int m, n, k;
+  int dtype;
+  int tm, tn, nt, bs, kt;
+  int atomic;        /* 1: flush C-runs with atomic adds; 0: plain read-modify-write */
+} lxs_smm_args_t;
+
+/** number of LDS bytes the stack kernel needs for this configuration (0 if invalid) */
+size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args);
+/** 1 if a kernel is compiled for (dtype, tm, tn, nt), 2 if also specialised for (m, n, k), else 0 */
+int lxs_smm_available(int dtype, int m, int n, int k, int tm, int tn, int nt);
+/** enumerate compiled (tm, tn, nt) variants for dtype: returns count, fills up to cap triplets */
+int lxs_smm_variants(int dtype, int* tm_tn_nt, int cap);
+/** largest LDS allocation one work-group may use */
+size_t lxs_lds_limit(void);
+int lxs_smm_launch(const lxs_smm_args_t* args, void* hip_stream);
+
+int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* data, int dtype, int m, int n,
+  void* hip_stream);
+int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream);
+
+#if defined(__cplusplus)
+}
+#endif
+
+#endif /*LXS_KERNELS_H*/

@@ -1,0 +1,187 @@
+/*
+ * Launchers for the gfx950 kernels: stack-kernel dispatch over the per-type kernel tables,
+ * in-place block transpose (samples/smm/kernels/transpose.cl:15-84 semantics) and a zero fill
+ * (used for libxstream_mem_zero so that callers under stream capture get no memset nodes).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "lxs_kernels.h"
+
+namespace {
+
+struct smm_entry_t {
+  int tm, tn, nt, m, n, k, kt;
+  const void* fn;
+};
+
+const size_t lds_limit = 65536;
+
+/* In-place transpose of listed blocks: column-major m x n -> row-major m x n. One work-group
+ * handles `bs` consecutive blocks of the transpose stack, staging each block in LDS. */
+template <typename T>
+__global__ void __launch_bounds__(256) trans_stack(const int* __restrict__ trs, int offset, int stack_size,
+  T* __restrict__ data, int m, int n, int bs) {
+  extern __shared__ double lxs_tbuf[];
+  T* const buf = reinterpret_cast<T*>(lxs_tbuf);
+  const int mn = m * n;
+  const int first = blockIdx.x * bs;
+  const int last = min(first + bs, stack_size);
+  for (int i = first; i < last; ++i) {
+    T* const mat = data + trs[offset + i];
+    for (int x = threadIdx.x; x < mn; x += blockDim.x) buf[x] = mat[x];
+    __syncthreads();
+    for (int y = threadIdx.x; y < mn; y += blockDim.x) {
+      const int r = y / n, c = y - r * n; /* row-major position (r, c) */
+      mat[y] = buf[c * m + r];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(256) zero_fill(unsigned char* __restrict__ p, size_t nbytes) {
+  const size_t head = (16 - ((uintptr_t)p & 15)) & 15;
+  const size_t pre = (head < nbytes ? head : nbytes);
+  const size_t nvec = (nbytes - pre) / 16;
+  const size_t tail = pre + nvec * 16;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint4* const v = reinterpret_cast<uint4*>(p + pre);
+  for (size_t i = tid; i < nvec; i += stride) v[i] = make_uint4(0, 0, 0, 0);
+  if (tid < pre) p[tid] = 0;
+  for (size_t i = tail + tid; i < nbytes; i += stride) p[i] = 0;
+}
+
+} /* namespace */
+
+extern "C" const void* lxs_smm_table_f64(int* count);
+extern "C" const void* lxs_smm_table_f32(int* count);
+
+static const smm_entry_t* smm_table(int dtype, int* count) {
+  const void* t = (3 == dtype ? lxs_smm_table_f64(count) : (1 == dtype ? lxs_smm_table_f32(count) : NULL));
+  if (NULL == t) *count = 0;
+  return (const smm_entry_t*)t;
+}
+
+/* best entry for (dtype, tm, tn, nt): a shape-specialised kernel if compiled, else the runtime-shape one */
+static const smm_entry_t* smm_find(int dtype, int m, int n, int k, int tm, int tn, int nt) {
+  int count = 0, i;
+  const smm_entry_t* const table = smm_table(dtype, &count);
+  const smm_entry_t* generic = NULL;
+  for (i = 0; i < count; ++i) {
+    const smm_entry_t* const e = table + i;
+    if (e->tm != tm || e->tn != tn || e->nt != nt) continue;
+    if (e->m == m && e->n == n && e->k == k) return e;
+    if (0 == e->m && NULL == generic) generic = e;
+  }
+  return generic;
+}
+
+extern "C" size_t lxs_lds_limit(void) { return lds_limit; }
+
+static size_t smm_lds(const lxs_smm_args_t* a, int kt) {
+  const int tiles_m = (a->m + a->tm - 1) / a->tm, tiles_n = (a->n + a->tn - 1) / a->tn;
+  const int ntiles = tiles_m * tiles_n;
+  const size_t ts = (3 == a->dtype ? 8 : 4);
+  int groups;
+  size_t slab, red;
+  if (0 >= ntiles || ntiles > a->nt || 0 >= kt) return 0;
+  groups = a->nt / ntiles;
+  slab = (size_t)kt * (size_t)(tiles_m * a->tm + tiles_n * a->tn);
+  red = (1 < groups ? (size_t)(tiles_m * a->tm) * (size_t)(tiles_n * a->tn) : 0);
+  return ts * (size_t)groups * (slab > red ? slab : red);
+}
+
+extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
+  const smm_entry_t* const e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
+  int kt;
+  if (NULL == e) return 0;
+  kt = (0 < e->kt ? e->kt : (0 < args->kt ? (args->kt < args->k ? args->kt : args->k) : args->k));
+  return smm_lds(args, kt);
+}
+
+extern "C" int lxs_smm_available(int dtype, int m, int n, int k, int tm, int tn, int nt) {
+  const smm_entry_t* const e = smm_find(dtype, m, n, k, tm, tn, nt);
+  return (NULL == e ? 0 : (0 != e->m ? 2 : 1));
+}
+
+extern "C" int lxs_smm_variants(int dtype, int* tm_tn_nt, int cap) {
+  int count = 0, i, n = 0;
+  const smm_entry_t* const table = smm_table(dtype, &count);
+  for (i = 0; i < count; ++i) {
+    if (0 != table[i].m) continue;
+    if (n < cap && NULL != tm_tn_nt) {
+      tm_tn_nt[3 * n + 0] = table[i].tm;
+      tm_tn_nt[3 * n + 1] = table[i].tn;
+      tm_tn_nt[3 * n + 2] = table[i].nt;
+    }
+    ++n;
+  }
+  return n;
+}
+
+extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
+  const smm_entry_t* const e = smm_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt);
+  int kt, bs, atomic, stack_size = a->stack_size, m = a->m, n = a->n, k = a->k;
+  size_t lds;
+  unsigned int nblocks;
+  void* c;
+  const void *pa, *pb;
+  const int* stack;
+  void* args[11];
+  if (NULL == e) return LXS_ENOVARIANT;
+  if (0 >= stack_size) return 0;
+  kt = (0 < e->kt ? e->kt : (0 < a->kt ? (a->kt < a->k ? a->kt : a->k) : a->k));
+  lds = smm_lds(a, kt);
+  if (0 == lds || lds > lds_limit) return LXS_ELIMIT;
+  bs = (0 < a->bs ? a->bs : 32);
+  nblocks = (unsigned int)((stack_size + bs - 1) / bs);
+  atomic = a->atomic;
+  c = a->c;
+  pa = a->a;
+  pb = a->b;
+  stack = a->stack;
+  args[0] = &c;
+  args[1] = &pa;
+  args[2] = &pb;
+  args[3] = &stack;
+  args[4] = &stack_size;
+  args[5] = &bs;
+  args[6] = &m;
+  args[7] = &n;
+  args[8] = &k;
+  args[9] = &kt;
+  args[10] = &atomic;
+  return (int)hipLaunchKernel(e->fn, dim3(nblocks), dim3((unsigned int)a->nt), args, lds, (hipStream_t)hip_stream);
+}
+
+extern "C" int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* data, int dtype, int m, int n,
+  void* hip_stream) {
+  const int bs = 8;
+  const size_t ts = (3 == dtype ? 8 : 4);
+  const size_t lds = ts * (size_t)m * (size_t)n;
+  const unsigned int nblocks = (unsigned int)((stack_size + bs - 1) / bs);
+  if (0 >= stack_size) return 0;
+  if (lds > lds_limit) return LXS_ELIMIT;
+  if (3 == dtype) {
+    trans_stack<double><<<nblocks, 256, lds, (hipStream_t)hip_stream>>>(trs_stack, offset, stack_size, (double*)data,
+      m, n, bs);
+  }
+  else if (1 == dtype) {
+    trans_stack<float><<<nblocks, 256, lds, (hipStream_t)hip_stream>>>(trs_stack, offset, stack_size, (float*)data,
+      m, n, bs);
+  }
+  else {
+    return LXS_ENOVARIANT;
+  }
+  return (int)hipGetLastError();
+}
+
+extern "C" int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream) {
+  size_t nblocks;
+  if (0 == nbytes) return 0;
+  nblocks = (nbytes / 16 + 255) / 256;
+  if (nblocks < 1) nblocks = 1;
+  if (nblocks > 4096) nblocks = 4096;
+  zero_fill<<<(unsigned int)nblocks, 256, 0, (hipStream_t)hip_stream>>>((unsigned char*)ptr, nbytes);
+  return (int)hipGetLastError();
+}

@@ -1,0 +1,589 @@
+/*
+ * ACC LIBSMM host layer: libsmm_acc_init/finalize/is_thread_safe/process/transpose.
+ *
+ * Follows the reference's host contract (samples/smm/smm_acc.c:43-604, smm_kernel.c:82-528,
+ * smm_trans.c:48-228): suitability gate with the -1 host-fallback result, per-shape configuration
+ * registry, tuned parameters from a CSV file (OPENCL_LIBSMM_SMM_PARAMS), environment overrides,
+ * asynchronous launch on the caller's stream with flops/bytes stated per launch. The kernels are
+ * compiled ahead of time for gfx950, so there is no JIT step; a configuration selects one of the
+ * compiled kernel variants.
+ */
+#if !defined(_GNU_SOURCE)
+#  define _GNU_SOURCE /* dladdr */
+#endif
+#include "lxs_internal.h"
+#include <dlfcn.h>
+#include <string.h>
+#include <strings.h>
+
+#define LXS_SMM_MAXDIM_DEFAULT 80
+
+typedef struct lxs_smm_slot_t {
+  int type, m, n, k, used, source;
+  libsmm_acc_smm_config_t config;
+} lxs_smm_slot_t;
+
+static pthread_mutex_t lxs_smm_lock = PTHREAD_MUTEX_INITIALIZER;
+static lxs_smm_slot_t* lxs_smm_slots = NULL;
+static int lxs_smm_cap = 0, lxs_smm_count = 0;
+static int lxs_smm_ninit = 0;
+
+static int lxs_typesize(int datatype) {
+  return (dbcsr_type_real_8 == datatype ? 8 : (dbcsr_type_real_4 == datatype ? 4 : 0));
+}
+
+/* registry ------------------------------------------------------------------------------------ */
+
+static unsigned int lxs_smm_hash(int type, int m, int n, int k) {
+  unsigned int h = 2166136261u;
+  const int v[4] = {type, m, n, k};
+  int i;
+  for (i = 0; i < 4; ++i) {
+    h ^= (unsigned int)v[i];
+    h *= 16777619u;
+  }
+  return h;
+}
+
+static lxs_smm_slot_t* lxs_smm_lookup(int type, int m, int n, int k, int insert) { /* lock held */
+  unsigned int i, mask;
+  if (0 == lxs_smm_cap || (0 != insert && 2 * (lxs_smm_count + 1) > lxs_smm_cap)) {
+    const int cap = (0 < lxs_smm_cap ? 2 * lxs_smm_cap : 256);
+    lxs_smm_slot_t* const slots = (lxs_smm_slot_t*)calloc((size_t)cap, sizeof(lxs_smm_slot_t));
+    lxs_smm_slot_t* const old = lxs_smm_slots;
+    const int oldcap = lxs_smm_cap;
+    int j;
+    if (NULL == slots) return NULL;
+    lxs_smm_slots = slots;
+    lxs_smm_cap = cap;
+    lxs_smm_count = 0;
+    for (j = 0; j < oldcap; ++j) {
+      if (0 != old[j].used) {
+        lxs_smm_slot_t* const s = lxs_smm_lookup(old[j].type, old[j].m, old[j].n, old[j].k, 1);
+        *s = old[j];
+      }
+    }
+    free(old);
+  }
+  mask = (unsigned int)lxs_smm_cap - 1;
+  for (i = lxs_smm_hash(type, m, n, k) & mask;; i = (i + 1) & mask) {
+    lxs_smm_slot_t* const s = lxs_smm_slots + i;
+    if (0 == s->used) {
+      if (0 == insert) return NULL;
+      memset(s, 0, sizeof(*s));
+      s->used = 1;
+      s->type = type;
+      s->m = m;
+      s->n = n;
+      s->k = k;
+      ++lxs_smm_count;
+      return s;
+    }
+    if (s->type == type && s->m == m && s->n == n && s->k == k) return s;
+  }
+}
+
+/* configurations ------------------------------------------------------------------------------ */
+
+/* LDS/thread feasibility of a configuration for a shape */
+LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, int n, int k,
+  const libsmm_acc_smm_config_t* config) {
+  lxs_smm_args_t args;
+  size_t lds;
+  if (NULL == config || 0 >= m || 0 >= n || 0 >= k || 0 == lxs_typesize(datatype)) return EXIT_FAILURE;
+  if (0 != config->variant || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt || 0 > config->bs) {
+    return EXIT_FAILURE;
+  }
+  if (0 == lxs_smm_available((int)datatype, m, n, k, config->tm, config->tn, config->nt)) return EXIT_FAILURE;
+  memset(&args, 0, sizeof(args));
+  args.dtype = (int)datatype;
+  args.m = m;
+  args.n = n;
+  args.k = k;
+  args.tm = config->tm;
+  args.tn = config->tn;
+  args.nt = config->nt;
+  args.kt = config->kt;
+  lds = lxs_smm_lds_bytes(&args);
+  return (0 < lds && lds <= lxs_lds_limit()) ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
+/* largest k-slice that keeps the staged blocks of all groups within the LDS limit */
+static int lxs_smm_fit_kt(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config) {
+  int kt = (0 < config->kt && config->kt < k) ? config->kt : k;
+  for (; 0 < kt; kt = (kt > 8 ? kt - 8 : kt - 1)) {
+    config->kt = kt;
+    if (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)datatype, m, n, k, config)) {
+      if (kt == k) config->kt = 0;
+      return EXIT_SUCCESS;
+    }
+  }
+  return EXIT_FAILURE;
+}
+
+/**
+ * Built-in choice when nothing is tuned: the largest square microtile (up to 5x5) such that one C
+ * block needs at most 64 threads (one wavefront); shapes that need more use up to 256 threads
+ * (512 for 4x4 tiles), then the smallest compiled thread count that covers the C block.
+ */
+static int lxs_smm_default(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config) {
+  static const int nts[] = {64, 128, 256, 512};
+  const int limits[] = {64, 256, 512};
+  int pass, t, i;
+  memset(config, 0, sizeof(*config));
+  config->bs = 32;
+  for (pass = 0; pass < 3; ++pass) {
+    for (t = 1; t <= 5; ++t) {
+      const int ntiles = ((m + t - 1) / t) * ((n + t - 1) / t);
+      if (ntiles > limits[pass]) continue;
+      for (i = 0; i < 4; ++i) {
+        if (nts[i] < ntiles) continue;
+        config->tm = config->tn = t;
+        config->nt = nts[i];
+        config->kt = 0;
+        if (EXIT_SUCCESS == lxs_smm_fit_kt(datatype, m, n, k, config)) return EXIT_SUCCESS;
+      }
+    }
+  }
+  memset(config, 0, sizeof(*config));
+  return EXIT_FAILURE;
+}
+
+/* 2-character keys are suppressed when OPENCL_LIBSMM_SMM_PARAMS=0 (smm_acc.c:494-524) */
+static const char* lxs_smm_getenv(const char* key) {
+  char name[64];
+  const char* const params = getenv("OPENCL_LIBSMM_SMM_PARAMS");
+  if (2 == strlen(key) && NULL != params && '0' == *params) return NULL;
+  snprintf(name, sizeof(name), "OPENCL_LIBSMM_SMM_%s", key);
+  {
+    const char* const value = getenv(name);
+    return (NULL != value && '\0' != *value) ? value : NULL;
+  }
+}
+
+static int lxs_smm_env(libsmm_acc_smm_config_t* config) {
+  static const char* const keys[] = {"BS", "TM", "TN", "NT", "KT", "FL"};
+  int i, any = 0;
+  for (i = 0; i < 6; ++i) {
+    const char* const value = lxs_smm_getenv(keys[i]);
+    if (NULL != value) {
+      const int v = atoi(value);
+      switch (i) {
+        case 0: config->bs = v; break;
+        case 1: config->tm = v; break;
+        case 2: config->tn = v; break;
+        case 3: config->nt = v; break;
+        case 4: config->kt = v; break;
+        default: config->flush = v;
+      }
+      any = 1;
+    }
+  }
+  return any;
+}
+
+static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config, int* source) {
+  const lxs_smm_slot_t* slot;
+  int src = libsmm_acc_config_default;
+  pthread_mutex_lock(&lxs_smm_lock);
+  slot = lxs_smm_lookup(datatype, m, n, k, 0);
+  if (NULL != slot) {
+    *config = slot->config;
+    src = slot->source;
+  }
+  pthread_mutex_unlock(&lxs_smm_lock);
+  if (NULL == slot) lxs_smm_default(datatype, m, n, k, config);
+  {
+    libsmm_acc_smm_config_t env = *config;
+    if (0 != lxs_smm_env(&env)) {
+      if (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)datatype, m, n, k, &env)) {
+        *config = env;
+        src = libsmm_acc_config_env;
+      }
+      else if (0 != lxs_cfg.verbosity) {
+        fprintf(stderr, "WARN LIBSMM: OPENCL_LIBSMM_SMM_* settings are not valid for %ix%ix%i (ignored)\n", m, n, k);
+      }
+    }
+  }
+  if (NULL != source) *source = src;
+}
+
+LIBXSTREAM_API int libsmm_acc_config_get(libsmm_acc_data_t datatype, int m, int n, int k, libsmm_acc_smm_config_t* config,
+  int* source) {
+  if (NULL == config || 0 == lxs_typesize(datatype) || 0 >= m || 0 >= n || 0 >= k) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_config_get: invalid argument");
+  }
+  lxs_smm_resolve((int)datatype, m, n, k, config, source);
+  return (0 < config->nt ? EXIT_SUCCESS : EXIT_FAILURE);
+}
+
+static int lxs_smm_register(int datatype, int m, int n, int k, const libsmm_acc_smm_config_t* config, int source,
+  int keep_best) {
+  lxs_smm_slot_t* slot;
+  int result = EXIT_SUCCESS;
+  pthread_mutex_lock(&lxs_smm_lock);
+  slot = lxs_smm_lookup(datatype, m, n, k, 1);
+  if (NULL != slot) {
+    /* a duplicate row replaces the entry only when it is faster (smm_acc.c:117-119) */
+    if (0 == keep_best || 0 == slot->config.nt || slot->config.gflops < config->gflops) {
+      slot->config = *config;
+      slot->source = source;
+    }
+  }
+  else {
+    result = EXIT_FAILURE;
+  }
+  pthread_mutex_unlock(&lxs_smm_lock);
+  return result;
+}
+
+LIBXSTREAM_API int libsmm_acc_config_set(libsmm_acc_data_t datatype, int m, int n, int k,
+  const libsmm_acc_smm_config_t* config) {
+  if (EXIT_SUCCESS != libsmm_acc_config_valid(datatype, m, n, k, config)) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_config_set: configuration not valid for this shape");
+  }
+  return lxs_smm_register((int)datatype, m, n, k, config, libsmm_acc_config_tuned, 0);
+}
+
+LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* tm_tn_nt, int cap) {
+  return lxs_smm_variants((int)datatype, tm_tn_nt, cap);
+}
+
+/* tuned-parameter CSV -------------------------------------------------------------------------- */
+
+static const char lxs_delims[] = ";,\t \r\n";
+
+LIBXSTREAM_API int libsmm_acc_read_ref_params(const char* line, int with_device, int with_perf, char* device,
+  size_t device_size, int key[4], int params[16], double s_gflops[2]) {
+  /* positional columns as in samples/smm/smm_params.c:99-243 */
+  char buffer[1024], *save = NULL, *tok;
+  int column = 0, nkey = 0, nparams = 0, nperf = 0;
+  if (NULL == line || NULL == key || NULL == params) return EXIT_FAILURE;
+  snprintf(buffer, sizeof(buffer), "%s", line);
+  memset(params, 0, sizeof(int) * 16);
+  if (NULL != s_gflops) s_gflops[0] = s_gflops[1] = 0;
+  for (tok = strtok_r(buffer, lxs_delims, &save); NULL != tok; tok = strtok_r(NULL, lxs_delims, &save)) {
+    char* end = NULL;
+    if (0 == column && 0 != with_device) {
+      if (NULL != device && 0 < device_size) snprintf(device, device_size, "%s", tok);
+      ++column;
+      continue;
+    }
+    if (4 > nkey) {
+      const long v = strtol(tok, &end, 10);
+      if ('\0' != *end || (0 < nkey && 0 >= v)) return EXIT_FAILURE;
+      key[nkey++] = (int)v;
+    }
+    else if (0 != with_perf && 2 > nperf) {
+      const double v = strtod(tok, &end);
+      if ('\0' != *end || 0 > v) return EXIT_FAILURE;
+      if (NULL != s_gflops) s_gflops[nperf] = v;
+      ++nperf;
+    }
+    else if (16 > nparams) {
+      const long v = strtol(tok, &end, 10);
+      if ('\0' != *end) return EXIT_FAILURE;
+      params[nparams++] = (int)v;
+    }
+    else {
+      return EXIT_FAILURE; /* too many columns */
+    }
+    ++column;
+  }
+  if (4 != nkey || (0 != with_perf && 2 != nperf) || 15 > nparams) return EXIT_FAILURE;
+  return (dbcsr_type_real_8 == key[0] || dbcsr_type_real_4 == key[0]) ? EXIT_SUCCESS : EXIT_FAILURE;
+}
+
+static int lxs_column(char names[][16], int ncolumns, const char* name) {
+  int i;
+  for (i = 0; i < ncolumns; ++i) {
+    if (0 == strcasecmp(names[i], name)) return i;
+  }
+  return -1;
+}
+
+LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
+  char line[1024], names[32][16];
+  int ncolumns = 0, nrows = 0, native, with_device, with_perf;
+  FILE* const file = (NULL != filename ? fopen(filename, "r") : NULL);
+  if (NULL == file) return -1;
+  if (NULL == fgets(line, sizeof(line), file)) {
+    fclose(file);
+    return -1;
+  }
+  { /* header decides the format: MI355X columns (TM, NT, ...) or the reference's (BM, BN, ...) */
+    char *save = NULL, *tok;
+    for (tok = strtok_r(line, lxs_delims, &save); NULL != tok && ncolumns < 32; tok = strtok_r(NULL, lxs_delims, &save)) {
+      snprintf(names[ncolumns++], sizeof(names[0]), "%s", tok);
+    }
+  }
+  native = (0 <= lxs_column(names, ncolumns, "TM") && 0 <= lxs_column(names, ncolumns, "NT"));
+  with_device = (0 <= lxs_column(names, ncolumns, "DEVICE"));
+  with_perf = (0 <= lxs_column(names, ncolumns, "GFLOPS"));
+  if (0 > lxs_column(names, ncolumns, "TYPEID")) {
+    fclose(file);
+    return -1;
+  }
+  while (NULL != fgets(line, sizeof(line), file)) {
+    libsmm_acc_smm_config_t config;
+    int key[4], ok = 0;
+    if ('\0' == line[strspn(line, lxs_delims)]) continue; /* blank line */
+    memset(&config, 0, sizeof(config));
+    if (0 != native) {
+      char *save = NULL, *tok, values[32][64];
+      int i = 0;
+      for (tok = strtok_r(line, lxs_delims, &save); NULL != tok && i < 32; tok = strtok_r(NULL, lxs_delims, &save)) {
+        snprintf(values[i++], sizeof(values[0]), "%s", tok);
+      }
+      if (i == ncolumns) {
+#define LXS_GETI(NAME) (0 <= lxs_column(names, ncolumns, NAME) ? atoi(values[lxs_column(names, ncolumns, NAME)]) : 0)
+        key[0] = LXS_GETI("TYPEID");
+        key[1] = LXS_GETI("M");
+        key[2] = LXS_GETI("N");
+        key[3] = LXS_GETI("K");
+        config.variant = LXS_GETI("VARIANT");
+        config.tm = LXS_GETI("TM");
+        config.tn = LXS_GETI("TN");
+        config.nt = LXS_GETI("NT");
+        config.bs = LXS_GETI("BS");
+        config.kt = LXS_GETI("KT");
+        config.flush = LXS_GETI("FLUSH");
+#undef LXS_GETI
+        if (0 != with_perf) config.gflops = atof(values[lxs_column(names, ncolumns, "GFLOPS")]);
+        ok = (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)key[0], key[1], key[2], key[3], &config));
+      }
+    }
+    else {
+      int params[16];
+      double perf[2];
+      if (EXIT_SUCCESS == libsmm_acc_read_ref_params(line, with_device, with_perf, NULL, 0, key, params, perf)) {
+        /* only BS (stack entries per work-group) carries over to these kernels */
+        ok = (EXIT_SUCCESS == lxs_smm_default(key[0], key[1], key[2], key[3], &config));
+        config.bs = (0 < params[0] ? params[0] : config.bs);
+        config.gflops = perf[1];
+      }
+    }
+    if (0 != ok) {
+      if (EXIT_SUCCESS != lxs_smm_register(key[0], key[1], key[2], key[3], &config, libsmm_acc_config_tuned, 1)) break;
+      ++nrows;
+    }
+    else if (0 != lxs_cfg.verbosity) {
+      fprintf(stderr, "WARN LIBSMM: skipped tuned-parameter row: %s", line);
+    }
+  }
+  fclose(file);
+  return nrows;
+}
+
+LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* device) {
+  FILE* const file = (NULL != filename ? fopen(filename, "w") : NULL);
+  int i, nrows = 0;
+  if (NULL == file) return -1;
+  fprintf(file, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH\n");
+  pthread_mutex_lock(&lxs_smm_lock);
+  for (i = 0; i < lxs_smm_cap; ++i) {
+    const lxs_smm_slot_t* const s = lxs_smm_slots + i;
+    if (0 == s->used) continue;
+    fprintf(file, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i\n", NULL != device ? device : "MI355X", s->type, s->m,
+      s->n, s->k, 0, s->config.gflops, s->config.variant, s->config.tm, s->config.tn, s->config.nt, s->config.bs,
+      s->config.kt, s->config.flush);
+    ++nrows;
+  }
+  pthread_mutex_unlock(&lxs_smm_lock);
+  fclose(file);
+  return nrows;
+}
+
+/* directory of this library (tuned parameters ship next to it) */
+static int lxs_smm_default_params(char* path, size_t size) {
+  Dl_info info;
+  if (0 != dladdr((void*)&lxs_smm_default_params, &info) && NULL != info.dli_fname) {
+    const char* const slash = strrchr(info.dli_fname, '/');
+    const int len = (NULL != slash ? (int)(slash - info.dli_fname) : 0);
+    snprintf(path, size, "%.*s%sparams/tune_multiply_MI355X.csv", len, info.dli_fname, 0 < len ? "/" : "");
+    return EXIT_SUCCESS;
+  }
+  return EXIT_FAILURE;
+}
+
+int lxs_smm_init(void) {
+  const char* const env = getenv("OPENCL_LIBSMM_SMM_PARAMS");
+  char path[4096];
+  if (NULL == env) {
+    if (EXIT_SUCCESS == lxs_smm_default_params(path, sizeof(path))) (void)libsmm_acc_params_load(path);
+  }
+  else if ('0' != *env && '\0' != *env) {
+    if (0 > libsmm_acc_params_load(env)) {
+      /* not a file: "TYPEID M N K VARIANT TM TN NT BS KT FLUSH" as one configuration */
+      libsmm_acc_smm_config_t config;
+      int key[4];
+      memset(&config, 0, sizeof(config));
+      if (11 == sscanf(env, "%i %i %i %i %i %i %i %i %i %i %i", key, key + 1, key + 2, key + 3, &config.variant,
+                   &config.tm, &config.tn, &config.nt, &config.bs, &config.kt, &config.flush)) {
+        if (EXIT_SUCCESS != libsmm_acc_config_set((libsmm_acc_data_t)key[0], key[1], key[2], key[3], &config)) {
+          return EXIT_FAILURE;
+        }
+      }
+      else if (0 != lxs_cfg.verbosity) {
+        fprintf(stderr, "WARN LIBSMM: failed to open parameter file!\n");
+      }
+    }
+  }
+  return EXIT_SUCCESS;
+}
+
+void lxs_smm_finalize(void) {
+  pthread_mutex_lock(&lxs_smm_lock);
+  free(lxs_smm_slots);
+  lxs_smm_slots = NULL;
+  lxs_smm_cap = lxs_smm_count = 0;
+  pthread_mutex_unlock(&lxs_smm_lock);
+}
+
+/* public entry points -------------------------------------------------------------------------- */
+
+LIBXSTREAM_API int libsmm_acc_init(void) {
+  int result = EXIT_SUCCESS;
+  pthread_mutex_lock(&lxs_smm_lock);
+  ++lxs_smm_ninit;
+  pthread_mutex_unlock(&lxs_smm_lock);
+  if (1 == lxs_smm_ninit) {
+    result = libxstream_init();
+    if (EXIT_SUCCESS == result) result = lxs_smm_init();
+  }
+  return result;
+}
+
+LIBXSTREAM_API int libsmm_acc_finalize(void) {
+  int result = EXIT_SUCCESS, last;
+  pthread_mutex_lock(&lxs_smm_lock);
+  last = (0 < lxs_smm_ninit && 0 == --lxs_smm_ninit);
+  pthread_mutex_unlock(&lxs_smm_lock);
+  if (0 != last) {
+    lxs_smm_finalize();
+    result = libxstream_finalize();
+  }
+  return result;
+}
+
+LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_is_thread_safe(void) { return 1; }
+
+LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t def_mnk, libsmm_acc_data_t datatype,
+  int stack_size, int m_max, int n_max, int k_max, int max_kernel_dim) {
+  const long mn = (long)m_max * (long)n_max;
+  c_dbcsr_acc_bool_t result = 0;
+  if (0 < mn && 0 < k_max && 0 < stack_size && 0 != def_mnk && mn <= (long)max_kernel_dim * max_kernel_dim) {
+    result = (dbcsr_type_real_8 == datatype || dbcsr_type_real_4 == datatype);
+  }
+  if (0 == result && 2 <= lxs_cfg.verbosity) {
+    fprintf(stderr, "INFO ACC/LIBSMM: SMM-kernel {%i,%i,%i,%i} ss=%i %s\n", (int)datatype, m_max, n_max, k_max,
+      stack_size, mn <= (long)max_kernel_dim * max_kernel_dim ? (0 != def_mnk ? "is ignored" : "is inhomogeneous")
+                                                              : "is too large");
+  }
+  return result;
+}
+
+static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,
+  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream) {
+  const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
+  lxs_smm_args_t args;
+  void* token;
+  int result = lxs_ensure_device();
+  if (EXIT_SUCCESS != result) return result;
+  memset(&args, 0, sizeof(args));
+  args.c = c;
+  args.a = a;
+  args.b = b;
+  args.stack = dev_param_stack;
+  args.stack_size = stack_size;
+  args.m = m;
+  args.n = n;
+  args.k = k;
+  args.dtype = datatype;
+  args.tm = config->tm;
+  args.tn = config->tn;
+  args.nt = config->nt;
+  args.bs = config->bs;
+  args.kt = config->kt;
+  /* flush: 1 = read-modify-write (caller guarantees each C block is one run), else atomic add,
+     which is exact-once per element when the stack is grouped by C and correct otherwise */
+  args.atomic = (1 == config->flush ? 0 : 1);
+  token = lxs_profile_begin(hs);
+  result = lxs_smm_launch(&args, (void*)hs);
+  lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,
+    (double)stack_size * lxs_typesize(datatype) * ((double)m * k + (double)k * n + 2.0 * m * n));
+  return result;
+}
+
+LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* config, const int* dev_param_stack,
+  int stack_size, libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m,
+  int n, int k, void* stream) {
+  int result;
+  if (0 == stack_size) return EXIT_SUCCESS;
+  if (0 > stack_size || NULL == dev_param_stack || NULL == dev_a_data || NULL == dev_b_data || NULL == dev_c_data) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_process_config: invalid argument");
+  }
+  if (EXIT_SUCCESS != libsmm_acc_config_valid(datatype, m, n, k, config)) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_process_config: configuration not valid for this shape");
+  }
+  result = lxs_smm_run(config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data, m, n, k,
+    stream);
+  return (0 <= result ? lxs_hip((hipError_t)result, "stack kernel launch") : lxs_error(result, "stack kernel launch"));
+}
+
+LIBXSTREAM_API int libsmm_acc_process(const int* host_param_stack, const int* dev_param_stack, int stack_size,
+  libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m_max, int n_max,
+  int k_max, int max_kernel_dim, c_dbcsr_acc_bool_t def_mnk, void* stack_stream, void* c_stream) {
+  libsmm_acc_smm_config_t config;
+  int result;
+  (void)host_param_stack;
+  (void)c_stream;
+  if (0 == libsmm_acc_process_suitable(def_mnk, datatype, stack_size, m_max, n_max, k_max, max_kernel_dim)) {
+    return (0 < stack_size ? -1 : EXIT_SUCCESS); /* smm_kernel.c:524-526 */
+  }
+  if (NULL == dev_param_stack || NULL == dev_a_data || NULL == dev_b_data || NULL == dev_c_data) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_process: NULL buffer");
+  }
+  lxs_smm_resolve((int)datatype, m_max, n_max, k_max, &config, NULL);
+  if (0 >= config.nt) return -1; /* no compiled kernel covers this shape: host fallback */
+  result = lxs_smm_run(&config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data, m_max,
+    n_max, k_max, stack_stream);
+  if (LXS_ENOVARIANT == result || LXS_ELIMIT == result) {
+    if (EXIT_SUCCESS != lxs_smm_default((int)datatype, m_max, n_max, k_max, &config)) return -1;
+    result = lxs_smm_run(&config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data,
+      m_max, n_max, k_max, stack_stream);
+    if (0 > result) return -1;
+  }
+  return lxs_hip((hipError_t)result, "stack kernel launch");
+}
+
+LIBXSTREAM_API int libsmm_acc_transpose(const int* dev_trs_stack, int offset, int stack_size, void* dev_data,
+  libsmm_acc_data_t datatype, int m, int n, int max_kernel_dim, void* stream) {
+  const long mn = (long)m * (long)n;
+  const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
+  void* token;
+  int result;
+  if (0 >= m || 0 >= n || 0 > stack_size || 0 > offset) return lxs_error(EXIT_FAILURE, "libsmm_acc_transpose: invalid argument");
+  /* smm_trans.c:54-68: nothing to do for these cases, which is not an error */
+  if (0 == stack_size || 1 == mn || 0 == lxs_typesize(datatype) || mn > (long)max_kernel_dim * max_kernel_dim) {
+    return EXIT_SUCCESS;
+  }
+  if (NULL == dev_trs_stack || NULL == dev_data) return lxs_error(EXIT_FAILURE, "libsmm_acc_transpose: NULL buffer");
+  result = lxs_ensure_device();
+  if (EXIT_SUCCESS != result) return result;
+  token = lxs_profile_begin(hs);
+  result = lxs_trans_launch(dev_trs_stack, offset, stack_size, dev_data, (int)datatype, m, n, (void*)hs);
+  lxs_profile_end(token, hs, "trans", 0, 2.0 * stack_size * (double)mn * lxs_typesize(datatype));
+  return (0 <= result ? lxs_hip((hipError_t)result, "transpose launch") : lxs_error(result, "transpose launch"));
+}
+
+LIBXSTREAM_API int c_calculate_norms(const double* mat, int nblks, const int* offsets, const int* nelems, float* norms,
+  void* stream_ptr) {
+  (void)mat;
+  (void)nblks;
+  (void)offsets;
+  (void)nelems;
+  (void)norms;
+  (void)stream_ptr;
+  return -1;
+}

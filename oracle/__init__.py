@@ -1,0 +1,122 @@
+"""ORACLE: test infrastructure only; never the product path.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package. It wraps
+liboracle.so (stack_oracle.c, cpu_baseline.c): a double-double CPU restatement of the reference's
+stack multiply (samples/smm/acc_bench.c:357-418 host path, kernels/multiply.cl:12-20 layout), its
+generators (acc_bench.h:30-79) and the timed CPU baseline.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+PLANT_NONE, PLANT_BLAYOUT, PLANT_DROPK, PLANT_BASE0, PLANT_NOACC = 0, 1, 2, 3, 4
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-C", HERE, "-B"], check=True, stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i, l, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double
+        L.oracle_stack_dd.argtypes = [i, i, i, i, i, vp, vp, vp, i, vp, l, vp, vp, i]
+        L.oracle_stack_dd.restype = i
+        L.oracle_error.argtypes = [i, l, vp, vp, vp, vp]
+        L.oracle_error.restype = i
+        L.oracle_transpose.argtypes = [i, i, i, vp, i, i, vp]
+        L.oracle_glibc_rand.argtypes = [ctypes.c_uint, i, vp]
+        L.oracle_init_stack.argtypes = [vp, i, i, vp, i, i, i, i, i, i]
+        L.oracle_init_stack.restype = i
+        L.oracle_init_mat.argtypes = [i, d, vp, i, i, d]
+        L.oracle_cpu_stack.argtypes = [i, i, i, i, i, vp, vp, vp, vp, i, i, i]
+        L.oracle_cpu_stack.restype = d
+        L.oracle_max_threads.restype = i
+        _lib = L
+    return _lib
+
+
+def _tid(dtype):
+    return 3 if np.dtype(dtype) == np.float64 else 1
+
+
+def stack_dd(m, n, k, stack, a, b, c_in=None, b_rowmajor=False, plant=PLANT_NONE):
+    """Double-double C after applying the stack: returns (hi, lo) float64 arrays shaped like c_in.
+
+    a: A pool (column-major blocks), b: B pool (column-major k-by-n unless b_rowmajor),
+    c_in: initial C pool (defaults to zeros of the size the stack addresses).
+    """
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b)
+    assert a.dtype == b.dtype
+    stack = np.ascontiguousarray(stack, dtype=np.int32)
+    if c_in is None:
+        nelems = int(stack[:, 2].max()) - 1 + m * n if len(stack) else m * n
+        c_in = np.zeros(nelems, dtype=a.dtype)
+    c_in = np.ascontiguousarray(c_in, dtype=a.dtype)
+    hi = np.empty(c_in.size, dtype=np.float64)
+    lo = np.empty(c_in.size, dtype=np.float64)
+    rc = lib().oracle_stack_dd(_tid(a.dtype), m, n, k, len(stack), stack.ctypes.data, a.ctypes.data, b.ctypes.data,
+                               1 if b_rowmajor else 0, c_in.ctypes.data, c_in.size, hi.ctypes.data, lo.ctypes.data,
+                               plant)
+    assert rc == 0
+    return hi.reshape(c_in.shape), lo.reshape(c_in.shape)
+
+
+def error(test, hi, lo):
+    """(max normalised error, normwise error, max|ref|, max abs error) of test against hi+lo."""
+    test = np.ascontiguousarray(test)
+    out = np.zeros(4, dtype=np.float64)
+    lib().oracle_error(_tid(test.dtype), test.size, test.ctypes.data, np.ascontiguousarray(hi).ctypes.data,
+                       np.ascontiguousarray(lo).ctypes.data, out.ctypes.data)
+    return tuple(out)
+
+
+def transpose(m, n, trs, offset, data):
+    trs = np.ascontiguousarray(trs, dtype=np.int32)
+    assert data.flags.c_contiguous
+    rc = lib().oracle_transpose(_tid(data.dtype), m, n, trs.ctypes.data, offset, len(trs) - offset, data.ctypes.data)
+    assert rc == 0
+    return data
+
+
+def glibc_rand(seed, count):
+    out = np.empty(count, dtype=np.int32)
+    lib().oracle_glibc_rand(seed, count, out.ctypes.data)
+    return out
+
+
+def init_stack(stack_size, rnd, mn, mk, kn, nc, na, nb):
+    rnd = np.ascontiguousarray(rnd, dtype=np.int32)
+    out = np.empty((stack_size, 3), dtype=np.int32)
+    rc = lib().oracle_init_stack(out.ctypes.data, stack_size, len(rnd), rnd.ctypes.data, mn, mk, kn, nc, na, nb)
+    assert rc == 0
+    return out
+
+
+def init_mat(dtype, seed, m, n, scale):
+    out = np.empty(m * n, dtype=dtype)
+    lib().oracle_init_mat(_tid(dtype), float(seed), out.ctypes.data, m, n, float(scale))
+    return out
+
+
+def cpu_stack(m, n, k, stack, a, b, c, nrepeat=1, warmup=0, nthreads=0):
+    """Reference host path (B column-major k-by-n) applied in place to c; returns timed seconds."""
+    stack = np.ascontiguousarray(stack, dtype=np.int32)
+    assert a.flags.c_contiguous and b.flags.c_contiguous and c.flags.c_contiguous
+    return lib().oracle_cpu_stack(_tid(a.dtype), m, n, k, len(stack), stack.ctypes.data, a.ctypes.data,
+                                  b.ctypes.data, c.ctypes.data, nrepeat, warmup, nthreads)
+
+
+def max_threads():
+    return lib().oracle_max_threads()

@@ -1,0 +1,71 @@
+"""Shared helpers for the GPU tests: run a stack workload through the C ABI, compare with the oracle."""
+import numpy as np
+
+import libxstream_amd as lx
+import oracle
+
+TOL = {np.dtype(np.float64): 1e-13, np.dtype(np.float32): 1e-5}
+
+
+def tol(dtype):
+    return TOL[np.dtype(dtype)]
+
+
+def run_stack(stream, w, config=None, device_transpose=True, stack=None, repeat=1):
+    """Upload w, (transpose B on the device), apply the stack `repeat` times, return C."""
+    tid = lx.typeid(w.dtype)
+    stack = w.stack if stack is None else stack
+    bufs = []
+    try:
+        a = lx.DeviceBuffer.from_array(w.a)
+        bufs.append(a)
+        if device_transpose:
+            b = lx.DeviceBuffer.from_array(w.b)
+            trs = lx.DeviceBuffer.from_array(np.arange(w.nb, dtype=np.int32) * (w.k * w.n))
+            bufs += [b, trs]
+            lx.transpose(trs, 0, w.nb, b, tid, w.k, w.n, stream)
+        else:
+            b = lx.DeviceBuffer.from_array(w.b_device_layout())
+            bufs.append(b)
+        c = lx.DeviceBuffer.from_array(w.c)
+        st = lx.DeviceBuffer.from_array(np.ascontiguousarray(stack, dtype=np.int32))
+        bufs += [c, st]
+        for _ in range(repeat):
+            if config is None:
+                rc = lx.process(st, len(stack), tid, a, b, c, w.m, w.n, w.k, stream)
+                assert rc == 0, "libsmm_acc_process returned %d" % rc
+            else:
+                lx.process_config(config, st, len(stack), tid, a, b, c, w.m, w.n, w.k, stream)
+        stream.sync()
+        out = np.empty_like(w.c)
+        c.download(out)
+        return out
+    finally:
+        for buf in bufs:
+            buf.free()
+
+
+def oracle_error(w, out, stack=None, repeat=1):
+    stack = w.stack if stack is None else stack
+    c_in = w.c.ravel()
+    for _ in range(repeat - 1):
+        hi, lo = oracle.stack_dd(w.m, w.n, w.k, stack, w.a, w.b, c_in=c_in)
+        c_in = (hi + lo).astype(w.dtype)
+    hi, lo = oracle.stack_dd(w.m, w.n, w.k, stack, w.a, w.b, c_in=c_in)
+    return oracle.error(out.ravel(), hi, lo)
+
+
+def checksum(w, stack=None):
+    """sum over all C elements of the stack's update: sum_e (1^T A_e)(B_e 1), in float64."""
+    stack = w.stack if stack is None else stack
+    m, n, k = w.m, w.n, w.k
+    a64 = w.a.astype(np.float64)
+    b64 = w.b.astype(np.float64)
+    colsum_a = a64.reshape(w.na, k, m).sum(axis=2)  # (na, k): column sums of each A block
+    rowsum_b = b64.reshape(w.nb, n, k).sum(axis=1)  # (nb, k): row sums of each column-major B block
+    ai = (stack[:, 0].astype(np.int64) - 1) // (m * k)
+    bi = (stack[:, 1].astype(np.int64) - 1) // (k * n)
+    total = np.einsum("ij,ij->i", colsum_a[ai], rowsum_b[bi])
+    scale = np.einsum("ij,ij->i", np.abs(a64).reshape(w.na, k, m).sum(axis=2)[ai],
+                      np.abs(b64).reshape(w.nb, n, k).sum(axis=1)[bi])
+    return float(np.sum(total)), float(np.sum(scale))

@@ -1,0 +1,255 @@
+"""GPU parity: the HIP stack kernel and transpose, through the C ABI, against the oracle.
+
+Bar: max normalised error <= 1e-13 (FP64) / 1e-5 (FP32) against the double-double oracle
+(samples/smm/acc_bench.c:357-418 host path, checked with an explicit metric in place of
+libxs_matdiff_epsilon); bit-exact for the transpose (pure data movement).
+"""
+import numpy as np
+import pytest
+
+import libxstream_amd as lx
+import oracle
+from libxstream_amd.workload import make_stack_workload
+from tests.helpers import checksum, oracle_error, run_stack, tol
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(5, 5, 5), (13, 13, 13), (23, 23, 23), (32, 32, 32), (64, 64, 64), (4, 9, 22), (45, 45, 13)]
+DTYPES = [np.float64, np.float32]
+
+
+@pytest.fixture(scope="module")
+def stream():
+    lx.init()
+    assert lx.device_count() > 0, "no HIP device visible"
+    s = lx.Stream("test")
+    yield s
+    s.close()
+    lx.finalize()
+
+
+def _seed(*key):
+    return abs(hash(key)) % (2**31)
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", SHAPES, ids=["%dx%dx%d" % s for s in SHAPES])
+def test_process_parity(stream, shape, dtype):
+    m, n, k = shape
+    w = make_stack_workload(m, n, k, 3000, dtype=dtype, seed=_seed(shape, 1), c_init="uniform")
+    out = run_stack(stream, w)
+    err = oracle_error(w, out)
+    assert err[0] <= tol(dtype), "max normalised error %g" % err[0]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+def test_every_variant(stream, dtype):
+    """Every compiled (tm, tn, nt) kernel on shapes that exercise padding, groups and k-slices."""
+    tid = lx.typeid(dtype)
+    variants = lx.kernel_variants(tid)
+    assert len(variants) >= 10
+    checked = 0
+    for shape in [(23, 23, 23), (7, 11, 3), (16, 16, 40), (9, 4, 22)]:
+        m, n, k = shape
+        w = make_stack_workload(m, n, k, 700, dtype=dtype, seed=_seed(shape, 2), c_init="uniform")
+        for tm, tn, nt in variants:
+            for kt in (0, 8):
+                cfg = lx.SmmConfig(variant=0, tm=tm, tn=tn, nt=nt, bs=16, kt=kt, flush=0)
+                if not lx.config_valid(tid, m, n, k, cfg):
+                    continue
+                out = run_stack(stream, w, config=cfg, device_transpose=False)
+                err = oracle_error(w, out)
+                assert err[0] <= tol(dtype), "%s tm=%d tn=%d nt=%d kt=%d: %g" % (shape, tm, tn, nt, kt, err[0])
+                checked += 1
+    assert checked >= 40
+
+
+@pytest.mark.parametrize("bs", [1, 7, 64, 1000])
+def test_chunk_sizes(stream, bs):
+    """Run ownership across work-group chunks: runs start and end anywhere relative to bs."""
+    m = n = k = 13
+    w = make_stack_workload(m, n, k, 4000, dtype=np.float64, seed=_seed("bs", bs), c_init="uniform", spread=12)
+    cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
+    cfg.bs = bs
+    out = run_stack(stream, w, config=cfg)
+    assert oracle_error(w, out)[0] <= 1e-13
+
+
+@pytest.mark.parametrize("flush", [1, 2])
+def test_flush_modes_bit_identical(stream, flush):
+    """Sorted stack: the read-modify-write and atomic flush give the same bits (one add per element)."""
+    m = n = k = 23
+    w = make_stack_workload(m, n, k, 6000, dtype=np.float64, seed=_seed("flush"), c_init="uniform")
+    cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
+    cfg.flush = 1
+    ref = run_stack(stream, w, config=cfg)
+    cfg.flush = flush
+    out = run_stack(stream, w, config=cfg)
+    assert np.array_equal(ref.view(np.uint64), out.view(np.uint64))
+    assert oracle_error(w, out)[0] <= 1e-13
+
+
+def test_deterministic(stream):
+    m = n = k = 32
+    w = make_stack_workload(m, n, k, 5000, dtype=np.float64, seed=_seed("det"), c_init="uniform")
+    a = run_stack(stream, w)
+    b = run_stack(stream, w)
+    assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+def test_unsorted_stack(stream, dtype):
+    """Interleaved C-runs (legal for the API): the default flush stays correct."""
+    m, n, k = 23, 23, 23
+    w = make_stack_workload(m, n, k, 3000, dtype=dtype, seed=_seed("unsorted"), c_init="uniform")
+    perm = np.random.default_rng(3).permutation(w.stack_size)
+    stack = np.ascontiguousarray(w.stack[perm])
+    out = run_stack(stream, w, stack=stack)
+    assert oracle_error(w, out, stack=stack)[0] <= tol(dtype)
+
+
+@pytest.mark.parametrize("case", ["one_run", "all_distinct", "single", "tail_run"])
+def test_run_structure(stream, case):
+    m, n, k = 13, 13, 13
+    if case == "one_run":
+        w = make_stack_workload(m, n, k, 5000, nc=1, dtype=np.float64, seed=11, c_init="uniform")
+    elif case == "all_distinct":
+        w = make_stack_workload(m, n, k, 2000, nc=2000, dtype=np.float64, seed=12, c_init="uniform", spread=0)
+    elif case == "single":
+        w = make_stack_workload(m, n, k, 1, nc=1, dtype=np.float64, seed=13, c_init="uniform")
+    else:
+        w = make_stack_workload(m, n, k, 3001, nc=10, dtype=np.float64, seed=14, c_init="uniform")
+    out = run_stack(stream, w)
+    assert oracle_error(w, out)[0] <= 1e-13
+
+
+def test_long_k_slices(stream):
+    """k beyond max_kernel_dim (allowed: the gate bounds m*n only) runs in k-slices."""
+    m, n, k = 20, 24, 300
+    w = make_stack_workload(m, n, k, 500, dtype=np.float64, seed=21, c_init="uniform")
+    out = run_stack(stream, w)
+    assert oracle_error(w, out)[0] <= 1e-13
+
+
+def test_repeat_accumulates(stream):
+    """beta = 1 semantics: applying the stack twice accumulates twice."""
+    m, n, k = 23, 23, 23
+    w = make_stack_workload(m, n, k, 3000, dtype=np.float64, seed=22, c_init="uniform")
+    out = run_stack(stream, w, repeat=2)
+    assert oracle_error(w, out, repeat=2)[0] <= 1e-13
+
+
+def test_full_size_23_30k_oracle(stream):
+    """The 23^3 FP64 stack at full size: 2000 A, 2000 B, 1000 C blocks, 30000 entries."""
+    w = make_stack_workload(23, 23, 23, 30000, nc=1000, na=2000, nb=2000, dtype=np.float64, seed=25071975)
+    out = run_stack(stream, w)
+    err = oracle_error(w, out)
+    assert err[0] <= 1e-13, err
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("stack_size", [100000, 500000])
+def test_full_size_checksum(stream, dtype, stack_size):
+    """Large stacks: size-independent checks (checksum of the update, bitwise determinism)."""
+    m = n = k = 23
+    w = make_stack_workload(m, n, k, stack_size, dtype=dtype, seed=_seed("big", stack_size))
+    out = run_stack(stream, w)
+    total, scale = checksum(w)
+    got = float(np.sum(out.astype(np.float64)))
+    eps = 1e-12 if dtype == np.float64 else 1e-5
+    assert abs(got - total) <= eps * scale, (got, total, scale)
+    again = run_stack(stream, w)
+    assert np.array_equal(out, again)
+
+
+def test_transpose_parity(stream):
+    for dtype in DTYPES:
+        for m, n in [(23, 23), (4, 9), (9, 4), (45, 13), (80, 80), (1, 7)]:
+            rng = np.random.default_rng(m * 100 + n)
+            nb = 300
+            data = rng.uniform(-1, 1, size=nb * m * n).astype(dtype)
+            trs = (rng.permutation(nb)[:200].astype(np.int32) * (m * n))
+            offset = 17
+            ref = oracle.transpose(m, n, trs, offset, data.copy())
+            tid = lx.typeid(dtype)
+            d = lx.DeviceBuffer.from_array(data)
+            t = lx.DeviceBuffer.from_array(trs)
+            lx.transpose(t, offset, len(trs) - offset, d, tid, m, n, stream)
+            stream.sync()
+            out = np.empty_like(data)
+            d.download(out)
+            d.free()
+            t.free()
+            assert np.array_equal(out.view(np.uint8), ref.view(np.uint8)), (dtype, m, n)
+
+
+def test_transpose_round_trip(stream):
+    """(k,n) then (n,k) is the identity (samples/smm/acc_bench.c:317-322)."""
+    k, n, nb = 22, 9, 500
+    data = np.random.default_rng(5).uniform(-1, 1, size=nb * k * n)
+    d = lx.DeviceBuffer.from_array(data)
+    t = lx.DeviceBuffer.from_array(np.arange(nb, dtype=np.int32) * (k * n))
+    lx.transpose(t, 0, nb, d, lx.REAL_8, k, n, stream)
+    lx.transpose(t, 0, nb, d, lx.REAL_8, n, k, stream)
+    stream.sync()
+    out = np.empty_like(data)
+    d.download(out)
+    d.free()
+    t.free()
+    assert np.array_equal(out, data)
+
+
+def test_fallback_and_noop(stream):
+    """-1 host fallback and successful no-ops (samples/smm/smm_kernel.c:524-526, smm_trans.c:54-68)."""
+    w = make_stack_workload(23, 23, 23, 100, dtype=np.float64, seed=1)
+    a = lx.DeviceBuffer.from_array(w.a)
+    b = lx.DeviceBuffer.from_array(w.b)
+    c = lx.DeviceBuffer.from_array(w.c)
+    st = lx.DeviceBuffer.from_array(w.stack)
+    try:
+        assert lx.process(st, 100, lx.REAL_8, a, b, c, 23, 23, 23, stream, def_mnk=0) == -1
+        assert lx.process(st, 100, lx.REAL_8, a, b, c, 81, 81, 23, stream) == -1
+        assert lx.process(st, 100, lx.COMPLEX_8, a, b, c, 23, 23, 23, stream) == -1
+        assert lx.process(st, 0, lx.REAL_8, a, b, c, 23, 23, 23, stream) == 0
+        before = np.empty_like(w.c)
+        c.download(before)
+        assert lx.transpose(st, 0, 0, b, lx.REAL_8, 23, 23, stream) == 0
+        assert lx.transpose(st, 0, 10, b, lx.COMPLEX_8, 23, 23, stream) == 0
+        stream.sync()
+        after = np.empty_like(w.c)
+        c.download(after)
+        assert np.array_equal(before, after)
+    finally:
+        for buf in (a, b, c, st):
+            buf.free()
+
+
+def test_memory_api(stream):
+    x = np.arange(1000, dtype=np.float64)
+    d = lx.DeviceBuffer.from_array(x)
+    e = lx.DeviceBuffer(x.nbytes)
+    lx.check(lx.lib().libxstream_mem_copy_d2d(d.ptr, e.ptr, x.nbytes, stream.handle))
+    e.zero(stream, offset=80, nbytes=160)  # elements 10..29
+    stream.sync()
+    out = np.empty_like(x)
+    e.download(out)
+    ref = x.copy()
+    ref[10:30] = 0
+    assert np.array_equal(out, ref)
+    # odd byte offset and length exercise the unaligned edges of the zero fill
+    e.upload(x)
+    e.zero(None, offset=3, nbytes=4001)
+    raw = np.empty(x.nbytes, dtype=np.uint8)
+    e.download(raw)
+    ref_raw = x.view(np.uint8).copy()
+    ref_raw[3:4004] = 0
+    assert np.array_equal(raw, ref_raw)
+    d.free()
+    e.free()
+    ev0, ev1 = lx.Event(), lx.Event()
+    ev0.record(stream)
+    ev1.record(stream)
+    ev1.sync()
+    assert ev0.elapsed_ms(ev1) >= 0
+    ev0.close()
+    ev1.close()
