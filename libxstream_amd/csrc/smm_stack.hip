@@ -100,12 +100,19 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
       if (r_end < base + NT || base + NT >= stack_size) break;
     }
 
-    T acc[TM][TN];
+    /* summation order: each entry's k-sum starts from zero (prod); entries are summed in
+       segments of SEG_LEN (seg) that are folded into the run total (acc). The longest rounding
+       chain is then k + SEG_LEN + run/SEG_LEN instead of run*k, which keeps long C-runs accurate;
+       microtiles above 16 elements skip the segment level to save registers. */
+    constexpr bool SEGMENTED = (TM * TN <= 16);
+    constexpr int SEG_LEN = 16;
+    T acc[TM][TN], seg[TM][TN];
 #pragma unroll
     for (int r = 0; r < TM; ++r) {
 #pragma unroll
-      for (int c = 0; c < TN; ++c) acc[r][c] = T(0);
+      for (int c = 0; c < TN; ++c) acc[r][c] = seg[r][c] = T(0);
     }
+    int nsteps = 0;
 
     for (int sb = h; sb < r_end; sb += G) {
       const int e = sb + g;
@@ -115,6 +122,12 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
       if (valid) {
         ablk = adata + (stack[3 * e + 0] - 1);
         bblk = bdata + (stack[3 * e + 1] - 1);
+      }
+      T prod[TM][TN];
+#pragma unroll
+      for (int r = 0; r < TM; ++r) {
+#pragma unroll
+        for (int c = 0; c < TN; ++c) prod[r][c] = T(0);
       }
       for (int kc = 0; kc < K; kc += KT) {
         const int kl = (KEVEN ? KTC : min(KT, K - kc));
@@ -144,11 +157,40 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
 #pragma unroll
             for (int r = 0; r < TM; ++r) {
 #pragma unroll
-              for (int c = 0; c < TN; ++c) acc[r][c] = fma(a[r], b[c], acc[r][c]);
+              for (int c = 0; c < TN; ++c) prod[r][c] = fma(a[r], b[c], prod[r][c]);
             }
           }
         }
         __syncthreads();
+      }
+#pragma unroll
+      for (int r = 0; r < TM; ++r) {
+#pragma unroll
+        for (int c = 0; c < TN; ++c) {
+          if (SEGMENTED) {
+            seg[r][c] += prod[r][c];
+          }
+          else {
+            acc[r][c] += prod[r][c];
+          }
+        }
+      }
+      if (SEGMENTED && 0 == (++nsteps % SEG_LEN)) {
+#pragma unroll
+        for (int r = 0; r < TM; ++r) {
+#pragma unroll
+          for (int c = 0; c < TN; ++c) {
+            acc[r][c] += seg[r][c];
+            seg[r][c] = T(0);
+          }
+        }
+      }
+    }
+    if (SEGMENTED) {
+#pragma unroll
+      for (int r = 0; r < TM; ++r) {
+#pragma unroll
+        for (int c = 0; c < TN; ++c) acc[r][c] += seg[r][c];
       }
     }
 
