@@ -18,12 +18,16 @@
  * (BS, BM, BN, ...) tuple of samples/smm/smm_acc_opencl.h:60-66 describes OpenCL work-items and is
  * not meaningful for these kernels.
  *   variant: 0 = vector-FMA kernel, operands staged through LDS
- *   tm, tn:  C microtile (rows x columns) owned by one thread
- *   nt:      threads per work-group (64, 128, 256 or 512); the threads left after covering one
- *            C block with microtiles form further groups that share the run's stack entries
+ *            1 = matrix-core (MFMA 16x16x4) kernel, operands loaded straight into registers
+ *   tm, tn:  variant 0: C microtile (rows x columns) owned by one thread;
+ *            variant 1: number of 16x16 output tiles along m and n (padded cover of the C block)
+ *   nt:      threads per work-group; variant 0: 64, 128, 256 or 512, the threads left after
+ *            covering one C block with microtiles form further groups that share the run's
+ *            stack entries; variant 1: 64 per wavefront, the wavefronts split the tile rows
  *   bs:      nominal stack entries per work-group; a work-group owns the C-runs whose first entry
  *            falls in its range and finishes them, so each C block is written by one work-group
- *   kt:      k-slice staged in LDS at once (0 = all of k when it fits)
+ *   kt:      variant 0: k-slice staged in LDS at once (0 = all of k when it fits);
+ *            variant 1: k per register stage (a multiple of 4 fixed by the kernel; 0 = any)
  *   flush:   0 = automatic, 1 = plain read-modify-write (stack grouped by C), 2 = atomic add
  */
 typedef struct libsmm_acc_smm_config_t {
@@ -52,8 +56,9 @@ LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, in
 LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* config, const int* dev_param_stack,
   int stack_size, libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m,
   int n, int k, void* stream);
-/** compiled kernel variants for datatype: returns their number and fills up to cap (tm, tn, nt) triplets */
-LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* tm_tn_nt, int cap);
+/** compiled runtime-shape kernels for datatype: returns their number and fills up to cap
+ *  (variant, tm, tn, nt, kt) quintuples; kt is the kernel's fixed k per stage (0: any) */
+LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* quintuples, int cap);
 /** load tuned parameters from a CSV file (reference or MI355X header); returns #rows loaded or <0 */
 LIBXSTREAM_API int libsmm_acc_params_load(const char* filename);
 /** write the registry as an MI355X CSV (header line + one row per shape); returns #rows or <0 */

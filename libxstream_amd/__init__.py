@@ -279,11 +279,11 @@ def config_valid(tid, m, n, k, cfg):
 
 
 def kernel_variants(tid):
-    """Compiled (tm, tn, nt) stack-kernel variants for a datatype."""
+    """Compiled runtime-shape stack kernels for a datatype: (variant, tm, tn, nt, kt) tuples."""
     cap = 256
-    buf = (ctypes.c_int * (3 * cap))()
+    buf = (ctypes.c_int * (5 * cap))()
     count = lib().libsmm_acc_kernel_variants(tid, buf, cap)
-    return [(buf[3 * i], buf[3 * i + 1], buf[3 * i + 2]) for i in range(min(count, cap))]
+    return [tuple(buf[5 * i + j] for j in range(5)) for i in range(min(count, cap))]
 
 
 def profile_enable(on=True):

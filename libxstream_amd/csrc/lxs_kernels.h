@@ -24,16 +24,21 @@ typedef struct lxs_smm_args_t {
   int stack_size;
   int m, n, k;
   int dtype;
+  int variant;       /* 0: vector-FMA kernel (LDS staging), 1: MFMA kernel (registers only) */
   int tm, tn, nt, bs, kt;
   int atomic;        /* 1: flush C-runs with atomic adds; 0: plain read-modify-write */
 } lxs_smm_args_t;
 
 /** number of LDS bytes the stack kernel needs for this configuration (0 if invalid) */
 size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args);
-/** 1 if a kernel is compiled for (dtype, tm, tn, nt), 2 if also specialised for (m, n, k), else 0 */
-int lxs_smm_available(int dtype, int m, int n, int k, int tm, int tn, int nt);
-/** enumerate compiled (tm, tn, nt) variants for dtype: returns count, fills up to cap triplets */
-int lxs_smm_variants(int dtype, int* tm_tn_nt, int cap);
+/** 1 if a kernel is compiled for (dtype, variant, tm, tn, nt, kt) and shape, 2 if specialised for
+ *  (m, n, k), else 0 */
+int lxs_smm_available(int dtype, int variant, int m, int n, int k, int tm, int tn, int nt, int kt);
+/** enumerate compiled runtime-shape kernels for dtype: returns count, fills up to cap quintuples
+ *  (variant, tm, tn, nt, kt) where kt is the compile-time k per stage (0: any) */
+int lxs_smm_variants(int dtype, int* quintuples, int cap);
+/** first compiled kernel for exactly this shape (specialised), else -1; fills a quintuple */
+int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* quintuple);
 /** largest LDS allocation one work-group may use */
 size_t lxs_lds_limit(void);
 int lxs_smm_launch(const lxs_smm_args_t* args, void* hip_stream);

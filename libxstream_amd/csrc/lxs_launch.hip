@@ -14,6 +14,11 @@ struct smm_entry_t {
   const void* fn;
 };
 
+struct mfma_entry_t {
+  int ti, tj, nwi, kb, m, n, k;
+  const void* fn;
+};
+
 const size_t lds_limit = 65536;
 
 /* In-place transpose of listed blocks: column-major m x n -> row-major m x n. One work-group
@@ -55,6 +60,28 @@ __global__ void __launch_bounds__(256) zero_fill(unsigned char* __restrict__ p, 
 
 extern "C" const void* lxs_smm_table_f64(int* count);
 extern "C" const void* lxs_smm_table_f32(int* count);
+extern "C" const void* lxs_mfma_table_f64(int* count);
+extern "C" const void* lxs_mfma_table_f32(int* count);
+
+static const mfma_entry_t* mfma_table(int dtype, int* count) {
+  const void* t = (3 == dtype ? lxs_mfma_table_f64(count) : (1 == dtype ? lxs_mfma_table_f32(count) : NULL));
+  if (NULL == t) *count = 0;
+  return (const mfma_entry_t*)t;
+}
+
+/* MFMA kernel for (dtype, tiles tm x tn, nt threads, kt = k per stage or 0): exact shape first */
+static const mfma_entry_t* mfma_find(int dtype, int m, int n, int k, int tm, int tn, int nt, int kt) {
+  int count = 0, i;
+  const mfma_entry_t* const table = mfma_table(dtype, &count);
+  const mfma_entry_t* generic = NULL;
+  for (i = 0; i < count; ++i) {
+    const mfma_entry_t* const e = table + i;
+    if (e->ti != tm || e->tj != tn || 64 * e->nwi != nt || (0 != kt && 4 * e->kb != kt)) continue;
+    if (e->m == m && e->n == n && e->k == k) return e;
+    if (0 == e->m && NULL == generic && m <= 16 * e->ti && n <= 16 * e->tj) generic = e;
+  }
+  return generic;
+}
 
 static const smm_entry_t* smm_table(int dtype, int* count) {
   const void* t = (3 == dtype ? lxs_smm_table_f64(count) : (1 == dtype ? lxs_smm_table_f32(count) : NULL));
@@ -92,47 +119,113 @@ static size_t smm_lds(const lxs_smm_args_t* a, int kt) {
 }
 
 extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
-  const smm_entry_t* const e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
+  const smm_entry_t* e;
   int kt;
+  if (1 == args->variant) return 0;
+  e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
   if (NULL == e) return 0;
   kt = (0 < e->kt ? e->kt : (0 < args->kt ? (args->kt < args->k ? args->kt : args->k) : args->k));
   return smm_lds(args, kt);
 }
 
-extern "C" int lxs_smm_available(int dtype, int m, int n, int k, int tm, int tn, int nt) {
-  const smm_entry_t* const e = smm_find(dtype, m, n, k, tm, tn, nt);
-  return (NULL == e ? 0 : (0 != e->m ? 2 : 1));
+extern "C" int lxs_smm_available(int dtype, int variant, int m, int n, int k, int tm, int tn, int nt, int kt) {
+  if (1 == variant) {
+    const mfma_entry_t* const e = mfma_find(dtype, m, n, k, tm, tn, nt, kt);
+    return (NULL == e ? 0 : (0 != e->m ? 2 : 1));
+  }
+  if (0 == variant) {
+    const smm_entry_t* const e = smm_find(dtype, m, n, k, tm, tn, nt);
+    return (NULL == e ? 0 : (0 != e->m ? 2 : 1));
+  }
+  return 0;
 }
 
-extern "C" int lxs_smm_variants(int dtype, int* tm_tn_nt, int cap) {
+extern "C" int lxs_smm_variants(int dtype, int* q, int cap) {
   int count = 0, i, n = 0;
   const smm_entry_t* const table = smm_table(dtype, &count);
+  const mfma_entry_t* mtable;
   for (i = 0; i < count; ++i) {
     if (0 != table[i].m) continue;
-    if (n < cap && NULL != tm_tn_nt) {
-      tm_tn_nt[3 * n + 0] = table[i].tm;
-      tm_tn_nt[3 * n + 1] = table[i].tn;
-      tm_tn_nt[3 * n + 2] = table[i].nt;
+    if (n < cap && NULL != q) {
+      q[5 * n + 0] = 0;
+      q[5 * n + 1] = table[i].tm;
+      q[5 * n + 2] = table[i].tn;
+      q[5 * n + 3] = table[i].nt;
+      q[5 * n + 4] = 0;
+    }
+    ++n;
+  }
+  mtable = mfma_table(dtype, &count);
+  for (i = 0; i < count; ++i) {
+    if (0 != mtable[i].m) continue;
+    if (n < cap && NULL != q) {
+      q[5 * n + 0] = 1;
+      q[5 * n + 1] = mtable[i].ti;
+      q[5 * n + 2] = mtable[i].tj;
+      q[5 * n + 3] = 64 * mtable[i].nwi;
+      q[5 * n + 4] = 4 * mtable[i].kb;
     }
     ++n;
   }
   return n;
 }
 
+extern "C" int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* q) {
+  int count = 0, i;
+  if (1 == variant) {
+    const mfma_entry_t* const table = mfma_table(dtype, &count);
+    for (i = 0; i < count; ++i) {
+      if (table[i].m == m && table[i].n == n && table[i].k == k) {
+        q[0] = 1;
+        q[1] = table[i].ti;
+        q[2] = table[i].tj;
+        q[3] = 64 * table[i].nwi;
+        q[4] = 4 * table[i].kb;
+        return i;
+      }
+    }
+  }
+  else {
+    const smm_entry_t* const table = smm_table(dtype, &count);
+    for (i = 0; i < count; ++i) {
+      if (table[i].m == m && table[i].n == n && table[i].k == k) {
+        q[0] = 0;
+        q[1] = table[i].tm;
+        q[2] = table[i].tn;
+        q[3] = table[i].nt;
+        q[4] = table[i].kt;
+        return i;
+      }
+    }
+  }
+  return -1;
+}
+
 extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
-  const smm_entry_t* const e = smm_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt);
   int kt, bs, atomic, stack_size = a->stack_size, m = a->m, n = a->n, k = a->k;
   size_t lds;
   unsigned int nblocks;
+  const void* fn;
   void* c;
   const void *pa, *pb;
   const int* stack;
   void* args[11];
-  if (NULL == e) return LXS_ENOVARIANT;
+  if (1 == a->variant) {
+    const mfma_entry_t* const e = mfma_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt, a->kt);
+    if (NULL == e) return LXS_ENOVARIANT;
+    fn = e->fn;
+    kt = 4 * e->kb;
+    lds = 0;
+  }
+  else {
+    const smm_entry_t* const e = smm_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt);
+    if (NULL == e) return LXS_ENOVARIANT;
+    fn = e->fn;
+    kt = (0 < e->kt ? e->kt : (0 < a->kt ? (a->kt < a->k ? a->kt : a->k) : a->k));
+    lds = smm_lds(a, kt);
+    if (0 == lds || lds > lds_limit) return LXS_ELIMIT;
+  }
   if (0 >= stack_size) return 0;
-  kt = (0 < e->kt ? e->kt : (0 < a->kt ? (a->kt < a->k ? a->kt : a->k) : a->k));
-  lds = smm_lds(a, kt);
-  if (0 == lds || lds > lds_limit) return LXS_ELIMIT;
   bs = (0 < a->bs ? a->bs : 32);
   nblocks = (unsigned int)((stack_size + bs - 1) / bs);
   atomic = a->atomic;
@@ -151,7 +244,7 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
   args[8] = &k;
   args[9] = &kt;
   args[10] = &atomic;
-  return (int)hipLaunchKernel(e->fn, dim3(nblocks), dim3((unsigned int)a->nt), args, lds, (hipStream_t)hip_stream);
+  return (int)hipLaunchKernel(fn, dim3(nblocks), dim3((unsigned int)a->nt), args, lds, (hipStream_t)hip_stream);
 }
 
 extern "C" int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* data, int dtype, int m, int n,

@@ -91,12 +91,18 @@ LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, in
   lxs_smm_args_t args;
   size_t lds;
   if (NULL == config || 0 >= m || 0 >= n || 0 >= k || 0 == lxs_typesize(datatype)) return EXIT_FAILURE;
-  if (0 != config->variant || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt || 0 > config->bs) {
+  if ((0 != config->variant && 1 != config->variant) || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt ||
+      0 > config->bs || 0 > config->kt) {
     return EXIT_FAILURE;
   }
-  if (0 == lxs_smm_available((int)datatype, m, n, k, config->tm, config->tn, config->nt)) return EXIT_FAILURE;
+  if (0 == lxs_smm_available((int)datatype, config->variant, m, n, k, config->tm, config->tn, config->nt,
+             1 == config->variant ? config->kt : 0)) {
+    return EXIT_FAILURE;
+  }
+  if (1 == config->variant) return EXIT_SUCCESS; /* registers only */
   memset(&args, 0, sizeof(args));
   args.dtype = (int)datatype;
+  args.variant = config->variant;
   args.m = m;
   args.n = n;
   args.k = k;
@@ -122,22 +128,49 @@ static int lxs_smm_fit_kt(int datatype, int m, int n, int k, libsmm_acc_smm_conf
 }
 
 /**
- * Built-in choice when nothing is tuned: the largest square microtile (up to 5x5) such that one C
- * block needs at most 64 threads (one wavefront); shapes that need more use up to 256 threads
- * (512 for 4x4 tiles), then the smallest compiled thread count that covers the C block.
+ * Built-in choice when nothing is tuned. First the matrix-core kernel: a kernel compiled for exactly
+ * this shape, else the runtime-shape kernel with the fewest 16x16 tiles that cover the C block.
+ * Otherwise the vector kernel: the largest square microtile (up to 5x5) such that one C block needs
+ * at most 64 threads (one wavefront); shapes that need more use up to 256 threads (512 for 4x4
+ * tiles), with the smallest compiled thread count that covers the C block.
  */
 static int lxs_smm_default(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config) {
   static const int nts[] = {64, 128, 256, 512};
   const int limits[] = {64, 256, 512};
-  int pass, t, i;
+  int pass, t, i, q[5], best = -1, best_tiles = 0, count, cand[5 * 128];
   memset(config, 0, sizeof(*config));
   config->bs = 32;
+  if (0 <= lxs_smm_specialised(datatype, 1, m, n, k, q)) {
+    config->variant = 1;
+    config->tm = q[1];
+    config->tn = q[2];
+    config->nt = q[3];
+    config->kt = q[4];
+    return EXIT_SUCCESS;
+  }
+  count = lxs_smm_variants(datatype, cand, 128);
+  for (i = 0; i < count && i < 128; ++i) {
+    const int* const c = cand + 5 * i;
+    if (1 == c[0] && 16 * c[1] >= m && 16 * c[2] >= n && (0 > best || c[1] * c[2] < best_tiles)) {
+      best = i;
+      best_tiles = c[1] * c[2];
+    }
+  }
+  if (0 <= best) {
+    config->variant = 1;
+    config->tm = cand[5 * best + 1];
+    config->tn = cand[5 * best + 2];
+    config->nt = cand[5 * best + 3];
+    config->kt = cand[5 * best + 4];
+    return EXIT_SUCCESS;
+  }
   for (pass = 0; pass < 3; ++pass) {
     for (t = 1; t <= 5; ++t) {
       const int ntiles = ((m + t - 1) / t) * ((n + t - 1) / t);
       if (ntiles > limits[pass]) continue;
       for (i = 0; i < 4; ++i) {
         if (nts[i] < ntiles) continue;
+        config->variant = 0;
         config->tm = config->tn = t;
         config->nt = nts[i];
         config->kt = 0;
@@ -162,9 +195,9 @@ static const char* lxs_smm_getenv(const char* key) {
 }
 
 static int lxs_smm_env(libsmm_acc_smm_config_t* config) {
-  static const char* const keys[] = {"BS", "TM", "TN", "NT", "KT", "FL"};
+  static const char* const keys[] = {"BS", "TM", "TN", "NT", "KT", "FL", "VA"};
   int i, any = 0;
-  for (i = 0; i < 6; ++i) {
+  for (i = 0; i < 7; ++i) {
     const char* const value = lxs_smm_getenv(keys[i]);
     if (NULL != value) {
       const int v = atoi(value);
@@ -174,7 +207,8 @@ static int lxs_smm_env(libsmm_acc_smm_config_t* config) {
         case 2: config->tn = v; break;
         case 3: config->nt = v; break;
         case 4: config->kt = v; break;
-        default: config->flush = v;
+        case 5: config->flush = v; break;
+        default: config->variant = v;
       }
       any = 1;
     }
@@ -245,13 +279,14 @@ LIBXSTREAM_API int libsmm_acc_config_set(libsmm_acc_data_t datatype, int m, int 
   return lxs_smm_register((int)datatype, m, n, k, config, libsmm_acc_config_tuned, 0);
 }
 
-LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* tm_tn_nt, int cap) {
-  return lxs_smm_variants((int)datatype, tm_tn_nt, cap);
+LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* quintuples, int cap) {
+  return lxs_smm_variants((int)datatype, quintuples, cap);
 }
 
 /* tuned-parameter CSV -------------------------------------------------------------------------- */
 
-static const char lxs_delims[] = ";,\t \r\n";
+/* CSV column separators; spaces are part of a value (device names contain them) */
+static const char lxs_delims[] = ";,\t\r\n";
 
 LIBXSTREAM_API int libsmm_acc_read_ref_params(const char* line, int with_device, int with_perf, char* device,
   size_t device_size, int key[4], int params[16], double s_gflops[2]) {
@@ -264,24 +299,29 @@ LIBXSTREAM_API int libsmm_acc_read_ref_params(const char* line, int with_device,
   if (NULL != s_gflops) s_gflops[0] = s_gflops[1] = 0;
   for (tok = strtok_r(buffer, lxs_delims, &save); NULL != tok; tok = strtok_r(NULL, lxs_delims, &save)) {
     char* end = NULL;
+    while (' ' == *tok) ++tok;
     if (0 == column && 0 != with_device) {
       if (NULL != device && 0 < device_size) snprintf(device, device_size, "%s", tok);
       ++column;
       continue;
     }
+    if ('\0' == *tok) return EXIT_FAILURE; /* empty column */
     if (4 > nkey) {
       const long v = strtol(tok, &end, 10);
+      while (' ' == *end) ++end;
       if ('\0' != *end || (0 < nkey && 0 >= v)) return EXIT_FAILURE;
       key[nkey++] = (int)v;
     }
     else if (0 != with_perf && 2 > nperf) {
       const double v = strtod(tok, &end);
+      while (' ' == *end) ++end;
       if ('\0' != *end || 0 > v) return EXIT_FAILURE;
       if (NULL != s_gflops) s_gflops[nperf] = v;
       ++nperf;
     }
     else if (16 > nparams) {
       const long v = strtol(tok, &end, 10);
+      while (' ' == *end) ++end;
       if ('\0' != *end) return EXIT_FAILURE;
       params[nparams++] = (int)v;
     }
@@ -500,6 +540,7 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   args.n = n;
   args.k = k;
   args.dtype = datatype;
+  args.variant = config->variant;
   args.tm = config->tm;
   args.tn = config->tn;
   args.nt = config->nt;

@@ -104,15 +104,18 @@ int oracle_stack_dd(int type, int m, int n, int k, int stack_size, const int* st
 int oracle_error(int type, long nelems, const void* test, const double* ref_hi, const double* ref_lo, double* out) {
   double maxref = 0, maxerr = 0, sumerr = 0, sumref = 0;
   long i;
+  int nan = 0;
   for (i = 0; i < nelems; ++i) {
     const double t = load(test, type, i);
     const double d = (t - ref_hi[i]) - ref_lo[i];
     const double r = ref_hi[i] + ref_lo[i];
     if (fabs(r) > maxref) maxref = fabs(r);
-    if (!(fabs(d) <= maxerr)) maxerr = fabs(d); /* NaN propagates as an error */
+    if (isnan(d) || isinf(d)) nan = 1; /* a non-finite result is an error of its own */
+    else if (fabs(d) > maxerr) maxerr = fabs(d);
     sumerr += d * d;
     sumref += r * r;
   }
+  if (0 != nan) maxerr = NAN;
   out[0] = maxerr / (maxref > 1e-300 ? maxref : 1e-300);
   out[1] = (sumref > 0 ? sqrt(sumerr / sumref) : sqrt(sumerr));
   out[2] = maxref;

@@ -52,14 +52,15 @@ def test_every_variant(stream, dtype):
     for shape in [(23, 23, 23), (7, 11, 3), (16, 16, 40), (9, 4, 22)]:
         m, n, k = shape
         w = make_stack_workload(m, n, k, 700, dtype=dtype, seed=_seed(shape, 2), c_init="uniform")
-        for tm, tn, nt in variants:
-            for kt in (0, 8):
-                cfg = lx.SmmConfig(variant=0, tm=tm, tn=tn, nt=nt, bs=16, kt=kt, flush=0)
+        for variant, tm, tn, nt, vkt in variants:
+            for kt in ((0, 8) if variant == 0 else (vkt,)):
+                cfg = lx.SmmConfig(variant=variant, tm=tm, tn=tn, nt=nt, bs=16, kt=kt, flush=0)
                 if not lx.config_valid(tid, m, n, k, cfg):
                     continue
                 out = run_stack(stream, w, config=cfg, device_transpose=False)
                 err = oracle_error(w, out)
-                assert err[0] <= tol(dtype), "%s tm=%d tn=%d nt=%d kt=%d: %g" % (shape, tm, tn, nt, kt, err[0])
+                assert err[0] <= tol(dtype), "%s v=%d tm=%d tn=%d nt=%d kt=%d: %g" % (
+                    shape, variant, tm, tn, nt, kt, err[0])
                 checked += 1
     assert checked >= 40
 
