@@ -15,8 +15,10 @@
  *     so one entry's loads are in flight while the previous entry is multiplied;
  *   - stack indices are wave-uniform scalar loads, fetched one entry ahead;
  *   - C-run ownership as in smm_stack.hip: a work-group owns the runs whose first entry lies in
- *     [w*bs, (w+1)*bs), sums the run in registers (per-entry products, segments of 16 entries)
- *     and flushes it once.
+ *     [w*bs, (w+1)*bs), sums the run in registers (segments of 16 entries in the matrix-core
+ *     accumulator, added to a run total) and flushes it once;
+ *   - fragment loads carry no branches and no selects on loaded values (padding lanes pick their
+ *     address instead), so the compiler can count outstanding loads exactly.
  *
  * Output layout of one 16x16 tile, lane l (0..63), register r (0..3), column j = l % 16:
  *   FP64 row = l/16 + 4r, FP32 row = 4(l/16) + r (checked on gfx950 with a layout probe).
@@ -61,17 +63,39 @@ struct mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+/* zero operands for fragment lanes past k (read-only) */
+__device__ const double lxs_zeros_f64[2] = {0.0, 0.0};
+__device__ const float lxs_zeros_f32[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+template <typename T>
+__device__ __forceinline__ const T* lxs_zeros();
+template <>
+__device__ __forceinline__ const double* lxs_zeros<double>() {
+  return lxs_zeros_f64;
+}
+template <>
+__device__ __forceinline__ const float* lxs_zeros<float>() {
+  return lxs_zeros_f32;
+}
+
+/* load through the global address space: the fragment pointers may point at the operand blocks or
+   at the zero array, and a generic (flat) load would force the compiler to wait for all loads */
+template <typename T>
+__device__ __forceinline__ T gload(const T* p) {
+  return *(const __attribute__((address_space(1))) T*)p;
+}
+
 template <typename T, int TI, int TJ, int NWI, int KB, int MC, int NC, int KC>
 __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
   int kt_unused, int atomic) {
   typedef typename mfma<T>::acc_t acc_t;
-  constexpr int RI = TI / NWI;               /* tile rows per wavefront */
-  constexpr bool SEG = (RI * TJ <= 4);       /* segment level of the run sum (registers permitting) */
+  constexpr int RI = TI / NWI; /* tile rows per wavefront */
+  /* summation order: the matrix core accumulates up to SEG_LEN entries in prod, which is then
+     added to the run total acc; the longest rounding chain is SEG_LEN*k/4 + run/SEG_LEN */
   constexpr int SEG_LEN = 16;
   const int M = (0 < MC ? MC : m_rt), N = (0 < NC ? NC : n_rt), K = (0 < KC ? KC : k_rt);
-  const int KS = (K + 3) >> 2;               /* k-steps of 4 */
-  const int NS = (KS + KB - 1) / KB;         /* k-stages per entry */
+  const int KS = (K + 3) >> 2;       /* k-steps of 4 */
+  const int NS = (KS + KB - 1) / KB; /* k-stages per entry */
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
@@ -94,128 +118,144 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
   }
   if (h >= end) return;
 
-  acc_t acc[RI][TJ], seg[RI][TJ], prod[RI][TJ];
+  /* per-lane parts of the fragment addresses. Lanes past m (A) or past n (B) only feed output
+     rows/columns that are never stored, so they load a clamped, valid element of the block.
+     Lanes past k must contribute zero to valid outputs: they load from a zeroed device array.
+     The choice is made on the address, so no select waits on a loaded value. */
+  int aoff[RI], boff[TJ];
+#pragma unroll
+  for (int ri = 0; ri < RI; ++ri) aoff[ri] = min(16 * (ti0 + ri) + li, M - 1);
+#pragma unroll
+  for (int tj = 0; tj < TJ; ++tj) boff[tj] = min(16 * tj + li, N - 1);
+  const T* const zeros = lxs_zeros<T>();
+
+  acc_t acc[RI][TJ], prod[RI][TJ];
   T fa[KB][RI], fb[KB][TJ];
   const acc_t zero = {T(0), T(0), T(0), T(0)};
 #pragma unroll
   for (int ri = 0; ri < RI; ++ri) {
 #pragma unroll
-    for (int tj = 0; tj < TJ; ++tj) acc[ri][tj] = seg[ri][tj] = prod[ri][tj] = zero;
+    for (int tj = 0; tj < TJ; ++tj) acc[ri][tj] = prod[ri][tj] = zero;
   }
 
-  /* fragments of k-step ks of the entry at offsets (a0, b0) into slot kb */
 #define LXS_LOAD_KSTEP(KBI, A0, B0, KSTEP) \
   do { \
     const int k_ = 4 * (KSTEP) + lk; \
+    const bool kv_ = (k_ < K); \
     _Pragma("unroll") for (int ri = 0; ri < RI; ++ri) { \
-      const int i_ = 16 * (ti0 + ri) + li; \
-      fa[KBI][ri] = (k_ < K && i_ < M) ? adata[(A0) + k_ * M + i_] : T(0); \
+      fa[KBI][ri] = gload(kv_ ? (adata + ((A0) + k_ * M + aoff[ri])) : zeros); \
     } \
     _Pragma("unroll") for (int tj = 0; tj < TJ; ++tj) { \
-      const int j_ = 16 * tj + li; \
-      fb[KBI][tj] = (k_ < K && j_ < N) ? bdata[(B0) + k_ * N + j_] : T(0); \
+      fb[KBI][tj] = gload(kv_ ? (bdata + ((B0) + k_ * N + boff[tj])) : zeros); \
     } \
   } while (0)
 
+  /* stack indices: current entry e, next entry (1) and the one after (2), as scalar loads */
   int e = h;
   int a0 = stack[3 * e + 0] - 1, b0 = stack[3 * e + 1] - 1, c0 = stack[3 * e + 2];
-  bool have1 = (e + 1 < stack_size);
-  int a1 = 0, b1 = 0, c1 = -1;
-  if (have1) {
-    a1 = stack[3 * e + 3] - 1;
-    b1 = stack[3 * e + 4] - 1;
-    c1 = stack[3 * e + 5];
-  }
-  int s = 0, nent = 0;
+  bool have1 = (e + 1 < stack_size), have2 = (e + 2 < stack_size);
+  int a1 = (have1 ? stack[3 * e + 3] - 1 : a0), b1 = (have1 ? stack[3 * e + 4] - 1 : b0);
+  int c1 = (have1 ? stack[3 * e + 5] : -1);
+  int a2 = (have2 ? stack[3 * e + 6] - 1 : a0), b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
+  int c2 = (have2 ? stack[3 * e + 8] : -1);
+  int s = 0;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
     if (kb < KS) LXS_LOAD_KSTEP(kb, a0, b0, kb);
   }
 
-  for (;;) {
-    const bool same_entry = (s + 1 < NS);
-    const bool run_ends = !same_entry && (!have1 || c1 != c0);
-    const bool own_next = have1 && (c1 == c0 || e + 1 < end);
-    const bool has_next = same_entry || own_next;
-    const int na = (same_entry ? a0 : a1), nb = (same_entry ? b0 : b1), ns = (same_entry ? s + 1 : 0);
+  for (;;) { /* one C-run per iteration */
+    int nent = 0;
+    bool run_ends, own_next;
+    for (;;) { /* one unit (entry, k-stage) per iteration; no memory traffic but the fragment loads */
+      const bool same_entry = (s + 1 < NS);
+      run_ends = !same_entry && (!have1 || c1 != c0);
+      own_next = have1 && (c1 == c0 || e + 1 < end);
+      /* the next unit of work; without one, reload the current entry (in bounds, never used) */
+      const int na = ((same_entry || !own_next) ? a0 : a1), nb = ((same_entry || !own_next) ? b0 : b1);
+      const int ns = (same_entry ? s + 1 : 0);
 #pragma unroll
-    for (int kb = 0; kb < KB; ++kb) {
-      const int ks = s * KB + kb;
-      if (ks < KS) {
+      for (int kb = 0; kb < KB; ++kb) {
+        const int ks = s * KB + kb;
+        if (ks < KS) {
+#pragma unroll
+          for (int ri = 0; ri < RI; ++ri) {
+#pragma unroll
+            for (int tj = 0; tj < TJ; ++tj) prod[ri][tj] = mfma<T>::mma(fa[kb][ri], fb[kb][tj], prod[ri][tj]);
+          }
+        }
+        /* refill this slot with the next unit's fragments while the matrix core works */
+        if (ns * KB + kb < KS) LXS_LOAD_KSTEP(kb, na, nb, ns * KB + kb);
+      }
+      if (same_entry) {
+        ++s;
+        continue;
+      }
+      if (SEG_LEN == ++nent && !run_ends) { /* fold the segment into the run total */
 #pragma unroll
         for (int ri = 0; ri < RI; ++ri) {
 #pragma unroll
-          for (int tj = 0; tj < TJ; ++tj) prod[ri][tj] = mfma<T>::mma(fa[kb][ri], fb[kb][tj], prod[ri][tj]);
+          for (int tj = 0; tj < TJ; ++tj) {
+            acc[ri][tj] += prod[ri][tj];
+            prod[ri][tj] = zero;
+          }
         }
+        nent = 0;
       }
-      /* refill this slot with the next unit's fragments while the matrix core works */
-      if (has_next && ns * KB + kb < KS) LXS_LOAD_KSTEP(kb, na, nb, ns * KB + kb);
+      if (run_ends || !own_next) break;
+      ++e; /* next entry of the same run */
+      a0 = a1;
+      b0 = b1;
+      a1 = a2;
+      b1 = b2;
+      c1 = c2;
+      have1 = have2;
+      s = 0;
+      have2 = (e + 2 < stack_size);
+      a2 = (have2 ? stack[3 * e + 6] - 1 : a0);
+      b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
+      c2 = (have2 ? stack[3 * e + 8] : -1);
     }
-    if (same_entry) {
-      ++s;
-      continue;
-    }
-    /* entry done: fold its product into the run sum */
-#pragma unroll
-    for (int ri = 0; ri < RI; ++ri) {
-#pragma unroll
-      for (int tj = 0; tj < TJ; ++tj) {
-        if (SEG) {
-          seg[ri][tj] += prod[ri][tj];
-        }
-        else {
-          acc[ri][tj] += prod[ri][tj];
-        }
-        prod[ri][tj] = zero;
-      }
-    }
-    if (SEG && (0 == (++nent % SEG_LEN) || run_ends)) {
-#pragma unroll
-      for (int ri = 0; ri < RI; ++ri) {
-#pragma unroll
-        for (int tj = 0; tj < TJ; ++tj) {
-          acc[ri][tj] += seg[ri][tj];
-          seg[ri][tj] = zero;
-        }
-      }
-    }
-    if (run_ends) {
+    /* run done: fold and flush C once (read-modify-write, or atomic add for unsorted stacks) */
+    {
       T* const cblk = cdata + (c0 - 1);
 #pragma unroll
       for (int ri = 0; ri < RI; ++ri) {
 #pragma unroll
         for (int tj = 0; tj < TJ; ++tj) {
           const int j = 16 * tj + li;
+          const acc_t sum = acc[ri][tj] + prod[ri][tj];
+          prod[ri][tj] = acc[ri][tj] = zero;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             const int i = 16 * (ti0 + ri) + mfma<T>::row(lane, r);
             if (i < M && j < N) {
               T* const p = cblk + j * M + i;
               if (atomic) {
-                unsafeAtomicAdd(p, acc[ri][tj][r]);
+                unsafeAtomicAdd(p, sum[r]);
               }
               else {
-                *p += acc[ri][tj][r];
+                *p += sum[r];
               }
             }
           }
-          acc[ri][tj] = zero;
         }
       }
-      nent = 0;
     }
     if (!own_next) break;
-    ++e;
+    ++e; /* first entry of the next run */
     a0 = a1;
     b0 = b1;
     c0 = c1;
+    a1 = a2;
+    b1 = b2;
+    c1 = c2;
+    have1 = have2;
     s = 0;
-    have1 = (e + 1 < stack_size);
-    if (have1) {
-      a1 = stack[3 * e + 3] - 1;
-      b1 = stack[3 * e + 4] - 1;
-      c1 = stack[3 * e + 5];
-    }
+    have2 = (e + 2 < stack_size);
+    a2 = (have2 ? stack[3 * e + 6] - 1 : a0);
+    b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
+    c2 = (have2 ? stack[3 * e + 8] : -1);
   }
 #undef LXS_LOAD_KSTEP
 }
