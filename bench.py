@@ -7,14 +7,21 @@ like one repetition of samples/smm/acc_bench.c:338-342. Prints one JSON line.
   value         whole-job GFLOP/s = 2*m*n*k*S / wall time per step (host clock, stream-synchronised)
   roofline      the stack kernel's algorithmic bytes S*8*(mk+kn+2mn) (samples/smm/smm_kernel.c:504-506)
                 per launch / its average duration from HIP events recorded on the launch stream,
-                against 8 TB/s HBM
+                against 8 TB/s HBM; `traffic` is the HBM bytes per launch read by rocprofv3 PMC
+                counters (FETCH_SIZE, WRITE_SIZE) in two child runs, each scaled by what the same
+                counter reads for a device-to-device copy of known size in the same run
   cpu_baseline  the reference host path (OpenMP, whole C-runs per thread) on the host's cores,
                 timed on a bounded sample of the same stack
 """
 import argparse
+import csv
+import glob
 import json
 import os
+import shutil
+import subprocess
 import sys
+import tempfile
 import time
 
 import numpy as np
@@ -24,6 +31,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+COPY_BYTES = 256 << 20  # calibration copy for the PMC byte counters
 
 
 def parse():
@@ -38,7 +46,131 @@ def parse():
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline time (0 = skip)")
     p.add_argument("--no-cpu", action="store_true")
+    p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC child runs")
+    p.add_argument("--no-small", action="store_true", help="skip the 30000-entry stack line")
+    p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
+
+
+def make_workload(args):
+    from libxstream_amd.workload import make_stack_workload
+
+    m = args.m
+    n = args.n or m
+    k = args.k or m
+    dtype = np.float64 if args.dtype == "f64" else np.float32
+    return make_stack_workload(m, n, k, args.stack_size, dtype=dtype, seed=25071975)
+
+
+class Resident:
+    """The workload's pools and stack uploaded once; B transposed on the device (acc_bench.c:325)."""
+
+    def __init__(self, lx, w, stream):
+        self.lx, self.w = lx, w
+        self.tid = lx.typeid(w.dtype)
+        self.a = lx.DeviceBuffer.from_array(w.a)
+        self.b = lx.DeviceBuffer.from_array(w.b)
+        self.trs = lx.DeviceBuffer.from_array(np.arange(w.nb, dtype=np.int32) * (w.k * w.n))
+        self.c = lx.DeviceBuffer.from_array(w.c)
+        self.st = lx.DeviceBuffer.from_array(w.stack)
+        lx.transpose(self.trs, 0, w.nb, self.b, self.tid, w.k, w.n, stream)
+
+    def process(self, stream):
+        w = self.w
+        rc = self.lx.process(self.st, w.stack_size, self.tid, self.a, self.b, self.c, w.m, w.n, w.k, stream)
+        assert rc == 0, "libsmm_acc_process returned %d" % rc
+
+    def free(self):
+        for buf in (self.a, self.b, self.trs, self.c, self.st):
+            buf.free()
+
+
+def timed(lx, res, stream, steps, warmup):
+    for _ in range(warmup):
+        res.process(stream)
+    stream.sync()
+    lx.profile_enable(True)
+    lx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        res.process(stream)
+    stream.sync()
+    t1 = time.perf_counter()
+    prof = lx.profile_get("smm")
+    lx.profile_enable(False)
+    return (t1 - t0) * 1e3 / steps, prof["ms"] / max(prof["count"], 1)
+
+
+def traffic_probe(args):
+    """Child of rocprofv3 --pmc: one calibration copy, then three stack launches."""
+    import libxstream_amd as lx
+
+    w = make_workload(args)
+    lx.init()
+    stream = lx.Stream("probe")
+    src = lx.DeviceBuffer(COPY_BYTES)
+    dst = lx.DeviceBuffer(COPY_BYTES)
+    src.zero(stream)
+    stream.sync()
+    lx.check(lx.lib().libxstream_mem_copy_d2d(src.ptr, dst.ptr, COPY_BYTES, stream.handle), "copy_d2d")
+    stream.sync()
+    res = Resident(lx, w, stream)
+    for _ in range(3):
+        res.process(stream)
+    stream.sync()
+    res.free()
+    src.free()
+    dst.free()
+    stream.close()
+    lx.finalize()
+
+
+def pmc_run(args, counter, tmp):
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    out = os.path.join(tmp, counter.lower())
+    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
+           sys.executable, os.path.abspath(__file__), "--traffic-probe", "--m", str(args.m),
+           "--stack-size", str(args.stack_size), "--dtype", args.dtype]
+    if args.n:
+        cmd += ["--n", str(args.n)]
+    if args.k:
+        cmd += ["--k", str(args.k)]
+    subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180, check=False)
+    rows = []
+    for path in glob.glob(os.path.join(out, "**", "*counter_collection*.csv"), recursive=True):
+        with open(path) as f:
+            rows += list(csv.DictReader(f))
+    copy_vals, smm_vals = [], []
+    for r in rows:
+        if r.get("Counter_Name") != counter:
+            continue
+        name = r.get("Kernel_Name", "")
+        val = float(r.get("Counter_Value", 0))
+        if "smm_" in name:
+            smm_vals.append(val)
+        elif "opy" in name:  # the runtime's device-to-device copy kernel
+            copy_vals.append(val)
+    if not copy_vals or not smm_vals:
+        return None
+    # counter unit per byte, from the copy of known size (reads and writes COPY_BYTES each)
+    factor = max(copy_vals) / COPY_BYTES
+    return {"per_launch_bytes": float(np.median(smm_vals)) / factor, "copy_counter": max(copy_vals),
+            "factor_per_byte": factor}
+
+
+def traffic(args):
+    try:
+        with tempfile.TemporaryDirectory(prefix="lxs_pmc_") as tmp:
+            fetch = pmc_run(args, "FETCH_SIZE", tmp)
+            write = pmc_run(args, "WRITE_SIZE", tmp)
+    except (OSError, subprocess.SubprocessError):
+        return None, None
+    if fetch is None or write is None:
+        return None, None
+    return fetch["per_launch_bytes"] + write["per_launch_bytes"], {
+        "fetch_bytes": round(fetch["per_launch_bytes"]), "write_bytes": round(write["per_launch_bytes"]),
+        "fetch_counter_per_byte": fetch["factor_per_byte"], "write_counter_per_byte": write["factor_per_byte"],
+        "calibration": "device-to-device copy of %d bytes in the same run" % COPY_BYTES}
 
 
 def cpu_baseline(w, seconds):
@@ -46,11 +178,10 @@ def cpu_baseline(w, seconds):
     import oracle
 
     nthreads = max(1, min(16, os.cpu_count() or 1))
-    # bounded sample: the first min(S, 100000) entries of the same stack, repeated to ~`seconds`
-    sample = np.ascontiguousarray(w.stack[: min(w.stack_size, 100000)])
+    sample = np.ascontiguousarray(w.stack)  # the same stack; repeated until ~`seconds` of CPU work
     c = np.zeros_like(w.c)
     t1 = oracle.cpu_stack(w.m, w.n, w.k, sample, w.a, w.b, c, nrepeat=1, warmup=1, nthreads=nthreads)
-    nrep = int(max(1, min(200, seconds / max(t1, 1e-6))))
+    nrep = int(max(1, min(1000, seconds / max(t1, 1e-6))))
     t = oracle.cpu_stack(w.m, w.n, w.k, sample, w.a, w.b, c, nrepeat=nrep, warmup=0, nthreads=nthreads)
     flops = 2.0 * w.m * w.n * w.k * len(sample) * nrep
     return {
@@ -58,67 +189,54 @@ def cpu_baseline(w, seconds):
         "unit": "GFLOP/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": "%d-entry prefix of the same stack x %d passes (%.1f s), B column-major as the reference host path"
+        "sample": "the full %d-entry stack x %d passes (%.1f s), B column-major as the reference host path"
         % (len(sample), nrep, t),
     }
 
 
 def main():
     args = parse()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the stack path shards by C-run "
-              "ranges with no exchange, see DESIGN.md)" % args.gpus, file=sys.stderr)
+              "ranges with no exchange between GPUs)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    # PMC child runs first, before this process initialises the GPU
+    tr, tr_detail = (None, None) if args.no_traffic else traffic(args)
+
     import libxstream_amd as lx
     from libxstream_amd.workload import make_stack_workload
 
-    m = args.m
-    n = args.n or m
-    k = args.k or m
-    dtype = np.float64 if args.dtype == "f64" else np.float32
-    tid = lx.typeid(dtype)
-    w = make_stack_workload(m, n, k, args.stack_size, dtype=dtype, seed=25071975)
-
+    w = make_workload(args)
+    m, n, k = w.m, w.n, w.k
     lx.init()
     stream = lx.Stream("bench")
-    a = lx.DeviceBuffer.from_array(w.a)
-    b = lx.DeviceBuffer.from_array(w.b)
-    trs = lx.DeviceBuffer.from_array(np.arange(w.nb, dtype=np.int32) * (k * n))
-    c = lx.DeviceBuffer.from_array(w.c)
-    st = lx.DeviceBuffer.from_array(w.stack)
-    lx.transpose(trs, 0, w.nb, b, tid, k, n, stream)  # B to the kernel's row-major layout (acc_bench.c:325)
-    cfg, src = lx.config_get(tid, m, n, k)
-    for _ in range(args.warmup):
-        assert lx.process(st, w.stack_size, tid, a, b, c, m, n, k, stream) == 0
-    stream.sync()
+    res = Resident(lx, w, stream)
+    cfg, src = lx.config_get(res.tid, m, n, k)
+    ms_per_step, kernel_ms = timed(lx, res, stream, args.steps, args.warmup)
+    res.free()
+    small = None
+    if not args.no_small:  # the 30000-entry stack over 2000/2000/1000 blocks (the reference's stack size)
+        ws = make_stack_workload(m, n, k, 30000, nc=1000, na=2000, nb=2000, dtype=w.dtype, seed=25071975)
+        rs = Resident(lx, ws, stream)
+        s_step, s_kernel = timed(lx, rs, stream, max(args.steps, 20), args.warmup)
+        rs.free()
+        small = {"stack_size": 30000, "blocks_a_b_c": [2000, 2000, 1000], "ms_per_step": round(s_step, 5),
+                 "gflops": round(ws.flops() / (s_step * 1e-3) * 1e-9, 1), "kernel_ms": round(s_kernel, 5),
+                 "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
+    ncus, arch = lx.device_info()
+    stream.close()
+    lx.finalize()
 
-    lx.profile_enable(True)
-    lx.profile_reset()
-    stream.sync()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        lx.process(st, w.stack_size, tid, a, b, c, m, n, k, stream)
-    stream.sync()
-    t1 = time.perf_counter()
-    prof = lx.profile_get("smm")
-    lx.profile_enable(False)
-
-    ms_per_step = (t1 - t0) * 1e3 / args.steps
     flops = w.flops()
     value = flops / (ms_per_step * 1e-3) * 1e-9
-    kernel_ms = prof["ms"] / max(prof["count"], 1)
     bytes_alg = w.bytes_alg()
     achieved = bytes_alg / (kernel_ms * 1e-3) * 1e-9
-    ncus, arch = lx.device_info()
-
     cpu = None
     if not args.no_cpu and args.cpu_seconds > 0:
         cpu = cpu_baseline(w, args.cpu_seconds)
-
-    for buf in (a, b, trs, c, st):
-        buf.free()
-    stream.close()
-    lx.finalize()
 
     line = {
         "metric": "SMM stack GFLOP/s (FP64 23^3)" if (m, n, k, args.dtype) == (23, 23, 23, "f64")
@@ -139,7 +257,7 @@ def main():
             "1-based offsets sorted by C" % (m, n, k, args.dtype, w.stack_size, w.nc, w.na, w.nb),
             "m": m, "n": n, "k": k, "stack_size": w.stack_size,
             "kernel_config": {kk: v for kk, v in cfg.as_dict().items() if kk != "gflops"},
-            "config_source": src,
+            "config_source": ["default", "tuned", "env", "learned"][src],
             "device": "%s (%d CUs)" % (arch, ncus),
         },
         "roofline": {
@@ -148,8 +266,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
-            "kernel": "smm_stack",
+            "traffic": (round(tr) if tr is not None else None),
+            "traffic_detail": tr_detail,
+            "kernel": "smm_mfma (libsmm_acc_process)",
             "kernel_ms": round(kernel_ms, 5),
             "bytes_alg_per_launch": bytes_alg,
             "flops_per_launch": flops,
@@ -157,7 +276,8 @@ def main():
             "fp64_frac": round(flops / (kernel_ms * 1e-3) * 1e-12 / FP64_PEAK_TFLOPS, 4),
         },
         "cpu_baseline": cpu,
-        "reference_published_gflops_23cubed_fp64": {"V100": 2459.7, "P100": 1657.8},
+        "stack_30000": small,
+        "reference_published_gflops_23cubed_fp64_s30000": {"V100": 2459.7, "P100": 1657.8},
     }
     print(json.dumps(line))
 

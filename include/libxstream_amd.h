@@ -59,6 +59,10 @@ LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* conf
 /** compiled runtime-shape kernels for datatype: returns their number and fills up to cap
  *  (variant, tm, tn, nt, kt) quintuples; kt is the kernel's fixed k per stage (0: any) */
 LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* quintuples, int cap);
+/** every compiled kernel usable for (m, n, k) as (variant, tm, tn, nt, kt) quintuples (kt: the
+ *  kernel's fixed k per stage, 0 = any); returns their number, fills up to cap */
+LIBXSTREAM_API int libsmm_acc_kernel_candidates(libsmm_acc_data_t datatype, int m, int n, int k, int* quintuples,
+  int cap);
 /** load tuned parameters from a CSV file (reference or MI355X header); returns #rows loaded or <0 */
 LIBXSTREAM_API int libsmm_acc_params_load(const char* filename);
 /** write the registry as an MI355X CSV (header line + one row per shape); returns #rows or <0 */

@@ -37,6 +37,8 @@ int lxs_smm_available(int dtype, int variant, int m, int n, int k, int tm, int t
 /** enumerate compiled runtime-shape kernels for dtype: returns count, fills up to cap quintuples
  *  (variant, tm, tn, nt, kt) where kt is the compile-time k per stage (0: any) */
 int lxs_smm_variants(int dtype, int* quintuples, int cap);
+/** every compiled kernel usable for (m, n, k) as (variant, tm, tn, nt, kt); returns the count */
+int lxs_smm_candidates(int dtype, int m, int n, int k, int* quintuples, int cap);
 /** first compiled kernel for exactly this shape (specialised), else -1; fills a quintuple */
 int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* quintuple);
 /** largest LDS allocation one work-group may use */

@@ -170,6 +170,54 @@ extern "C" int lxs_smm_variants(int dtype, int* q, int cap) {
   return n;
 }
 
+/* every compiled kernel usable for (m, n, k): runtime-shape kernels and kernels compiled for exactly
+   this shape, as (variant, tm, tn, nt, kt) quintuples without duplicates */
+extern "C" int lxs_smm_candidates(int dtype, int m, int n, int k, int* q, int cap) {
+  int count = 0, i, j, nq = 0;
+  const smm_entry_t* const table = smm_table(dtype, &count);
+  const mfma_entry_t* mtable;
+  for (i = 0; i < count + 1; ++i) { /* vector kernels (runtime shape, or compiled for this shape) */
+    int v[5];
+    if (i == count) break;
+    if (0 != table[i].m && (table[i].m != m || table[i].n != n || table[i].k != k)) continue;
+    v[0] = 0;
+    v[1] = table[i].tm;
+    v[2] = table[i].tn;
+    v[3] = table[i].nt;
+    v[4] = 0;
+    for (j = 0; j < nq && j < cap; ++j) {
+      if (q[5 * j] == v[0] && q[5 * j + 1] == v[1] && q[5 * j + 2] == v[2] && q[5 * j + 3] == v[3]) break;
+    }
+    if (j == nq) {
+      if (nq < cap) {
+        for (j = 0; j < 5; ++j) q[5 * nq + j] = v[j];
+      }
+      ++nq;
+    }
+  }
+  mtable = mfma_table(dtype, &count);
+  for (i = 0; i < count; ++i) {
+    const mfma_entry_t* const e = mtable + i;
+    int v[5];
+    if (0 != e->m ? (e->m != m || e->n != n || e->k != k) : (m > 16 * e->ti || n > 16 * e->tj)) continue;
+    v[0] = 1;
+    v[1] = e->ti;
+    v[2] = e->tj;
+    v[3] = 64 * e->nwi;
+    v[4] = 4 * e->kb;
+    for (j = 0; j < nq && j < cap; ++j) {
+      if (q[5 * j] == 1 && q[5 * j + 1] == v[1] && q[5 * j + 2] == v[2] && q[5 * j + 3] == v[3] && q[5 * j + 4] == v[4]) break;
+    }
+    if (j == nq) {
+      if (nq < cap) {
+        for (j = 0; j < 5; ++j) q[5 * nq + j] = v[j];
+      }
+      ++nq;
+    }
+  }
+  return nq;
+}
+
 extern "C" int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* q) {
   int count = 0, i;
   if (1 == variant) {

@@ -128,56 +128,69 @@ static int lxs_smm_fit_kt(int datatype, int m, int n, int k, libsmm_acc_smm_conf
 }
 
 /**
- * Built-in choice when nothing is tuned. First the matrix-core kernel: a kernel compiled for exactly
- * this shape, else the runtime-shape kernel with the fewest 16x16 tiles that cover the C block.
- * Otherwise the vector kernel: the largest square microtile (up to 5x5) such that one C block needs
- * at most 64 threads (one wavefront); shapes that need more use up to 256 threads (512 for 4x4
- * tiles), with the smallest compiled thread count that covers the C block.
+ * Built-in choice when nothing is tuned. The matrix-core kernel covers the C block with 16x16 tiles;
+ * when at least 30% of the covered area is useful (m*n >= 0.3 * 256 * ti * tj), it takes a kernel
+ * compiled for exactly this shape, else the runtime-shape kernel for exactly that tile grid.
+ * Smaller blocks go to the vector kernel: 256 threads, the square microtile (up to 5x5) that keeps
+ * the most threads busy with at most 16 groups of threads sharing a C-run's entries (runs are tens
+ * of entries long), ties to the smaller tile; 64 threads for blocks too small for that.
+ * Stack entries per work-group: 32 (matrix cores) or 64 (vector kernel), from measurements on
+ * MI355X over the shapes 5^3..64^3.
  */
 static int lxs_smm_default(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config) {
-  static const int nts[] = {64, 128, 256, 512};
-  const int limits[] = {64, 256, 512};
-  int pass, t, i, q[5], best = -1, best_tiles = 0, count, cand[5 * 128];
+  const int ti = (m + 15) / 16, tj = (n + 15) / 16;
+  int q[5], i, t, count, cand[5 * 256], best_t = 0, best_nt = 256;
+  double best_util = 0;
   memset(config, 0, sizeof(*config));
-  config->bs = 32;
-  if (0 <= lxs_smm_specialised(datatype, 1, m, n, k, q)) {
-    config->variant = 1;
-    config->tm = q[1];
-    config->tn = q[2];
-    config->nt = q[3];
-    config->kt = q[4];
-    return EXIT_SUCCESS;
-  }
-  count = lxs_smm_variants(datatype, cand, 128);
-  for (i = 0; i < count && i < 128; ++i) {
-    const int* const c = cand + 5 * i;
-    if (1 == c[0] && 16 * c[1] >= m && 16 * c[2] >= n && (0 > best || c[1] * c[2] < best_tiles)) {
-      best = i;
-      best_tiles = c[1] * c[2];
+  if ((double)m * n >= 0.3 * 256.0 * ti * tj) {
+    int found = (0 <= lxs_smm_specialised(datatype, 1, m, n, k, q));
+    count = lxs_smm_variants(datatype, cand, 256);
+    for (i = 0; i < count && i < 256 && !found; ++i) {
+      const int* const c = cand + 5 * i;
+      if (1 == c[0] && ti == c[1] && tj == c[2]) {
+        memcpy(q, c, sizeof(q));
+        found = 1;
+      }
+    }
+    if (found) {
+      config->variant = 1;
+      config->tm = q[1];
+      config->tn = q[2];
+      config->nt = q[3];
+      config->kt = q[4];
+      config->bs = 32;
+      return EXIT_SUCCESS;
     }
   }
-  if (0 <= best) {
-    config->variant = 1;
-    config->tm = cand[5 * best + 1];
-    config->tn = cand[5 * best + 2];
-    config->nt = cand[5 * best + 3];
-    config->kt = cand[5 * best + 4];
-    return EXIT_SUCCESS;
-  }
-  for (pass = 0; pass < 3; ++pass) {
+  for (i = 0; i < 2 && 0 == best_t; ++i) { /* 256 threads, else 64 threads for tiny blocks */
+    const int nt = (0 == i ? 256 : 64);
     for (t = 1; t <= 5; ++t) {
       const int ntiles = ((m + t - 1) / t) * ((n + t - 1) / t);
-      if (ntiles > limits[pass]) continue;
-      for (i = 0; i < 4; ++i) {
-        if (nts[i] < ntiles) continue;
-        config->variant = 0;
-        config->tm = config->tn = t;
-        config->nt = nts[i];
-        config->kt = 0;
-        if (EXIT_SUCCESS == lxs_smm_fit_kt(datatype, m, n, k, config)) return EXIT_SUCCESS;
+      if (ntiles <= nt && nt / ntiles <= 16) {
+        const double util = (double)((nt / ntiles) * ntiles) / nt;
+        if (util > best_util + 1e-9) {
+          best_util = util;
+          best_t = t;
+          best_nt = nt;
+        }
       }
     }
   }
+  config->variant = 0;
+  config->bs = 64;
+  if (0 == best_t && m * n <= 64) { /* tiny blocks: one element per thread, 64 threads */
+    best_t = 1;
+    best_nt = 64;
+  }
+  if (0 < best_t) {
+    config->tm = config->tn = best_t;
+    config->nt = best_nt;
+    if (EXIT_SUCCESS == lxs_smm_fit_kt(datatype, m, n, k, config)) return EXIT_SUCCESS;
+  }
+  config->tm = config->tn = 4; /* up to 88 x 88 with 512 threads */
+  config->nt = 512;
+  config->kt = 0;
+  if (EXIT_SUCCESS == lxs_smm_fit_kt(datatype, m, n, k, config)) return EXIT_SUCCESS;
   memset(config, 0, sizeof(*config));
   return EXIT_FAILURE;
 }
@@ -281,6 +294,11 @@ LIBXSTREAM_API int libsmm_acc_config_set(libsmm_acc_data_t datatype, int m, int 
 
 LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* quintuples, int cap) {
   return lxs_smm_variants((int)datatype, quintuples, cap);
+}
+
+LIBXSTREAM_API int libsmm_acc_kernel_candidates(libsmm_acc_data_t datatype, int m, int n, int k, int* quintuples,
+  int cap) {
+  return lxs_smm_candidates((int)datatype, m, n, k, quintuples, cap);
 }
 
 /* tuned-parameter CSV -------------------------------------------------------------------------- */

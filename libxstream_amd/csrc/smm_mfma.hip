@@ -264,14 +264,20 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
 #define LXS_MS(M, N, K, TI, TJ, NWI, KB) {TI, TJ, NWI, KB, M, N, K, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, M, N, K>},
 
 const mfma_entry_t mfma_table[] = {
-  /* runtime shape (m <= 16*ti, n <= 16*tj), 4 k-steps (16 of k) per stage */
-  LXS_MG(1, 1, 1, 4) LXS_MG(1, 2, 1, 4) LXS_MG(2, 1, 1, 4) LXS_MG(2, 2, 1, 4) LXS_MG(2, 2, 2, 4)
-  LXS_MG(3, 3, 1, 2) LXS_MG(3, 3, 3, 4) LXS_MG(4, 4, 4, 4) LXS_MG(4, 4, 2, 2) LXS_MG(5, 5, 5, 4)
+  /* runtime shape (m <= 16*ti, n <= 16*tj), 4 k-steps (16 of k) per stage: every tile grid up to
+     5x5 (m, n <= 80) with one wavefront per tile row, plus one wavefront for grids up to 4 tiles */
+  LXS_MG(1, 1, 1, 4) LXS_MG(1, 2, 1, 4) LXS_MG(1, 3, 1, 4) LXS_MG(1, 4, 1, 4) LXS_MG(1, 5, 1, 4)
+  LXS_MG(2, 1, 1, 4) LXS_MG(2, 1, 2, 4) LXS_MG(2, 2, 1, 4) LXS_MG(2, 2, 2, 4) LXS_MG(2, 3, 2, 4)
+  LXS_MG(2, 3, 1, 2) LXS_MG(2, 4, 2, 4) LXS_MG(2, 5, 2, 4) LXS_MG(3, 1, 1, 4) LXS_MG(3, 1, 3, 4)
+  LXS_MG(3, 2, 3, 4) LXS_MG(3, 2, 1, 2) LXS_MG(3, 3, 3, 4) LXS_MG(3, 4, 3, 4) LXS_MG(3, 5, 3, 4)
+  LXS_MG(4, 1, 1, 4) LXS_MG(4, 1, 4, 4) LXS_MG(4, 2, 4, 4) LXS_MG(4, 3, 4, 4) LXS_MG(4, 4, 4, 4)
+  LXS_MG(4, 5, 4, 4) LXS_MG(5, 1, 5, 4) LXS_MG(5, 1, 1, 2) LXS_MG(5, 2, 5, 4) LXS_MG(5, 3, 5, 4)
+  LXS_MG(5, 4, 5, 4) LXS_MG(5, 5, 5, 4)
   /* compile-time shapes of the stack workloads (whole k per stage where registers allow) */
   LXS_MS(5, 5, 5, 1, 1, 1, 2) LXS_MS(13, 13, 13, 1, 1, 1, 4)
   LXS_MS(23, 23, 23, 2, 2, 1, 6) LXS_MS(23, 23, 23, 2, 2, 2, 6) LXS_MS(23, 23, 23, 2, 2, 1, 3)
   LXS_MS(32, 32, 32, 2, 2, 1, 8) LXS_MS(32, 32, 32, 2, 2, 2, 8) LXS_MS(32, 32, 32, 2, 2, 1, 4)
-  LXS_MS(64, 64, 64, 4, 4, 4, 4) LXS_MS(64, 64, 64, 4, 4, 2, 4)
+  LXS_MS(64, 64, 64, 4, 4, 4, 4) LXS_MS(64, 64, 64, 4, 4, 2, 4) LXS_MS(64, 64, 64, 4, 4, 4, 8)
   LXS_MS(4, 9, 22, 1, 1, 1, 6) LXS_MS(45, 45, 13, 3, 3, 1, 4) LXS_MS(45, 45, 13, 3, 3, 3, 4)
 };
 

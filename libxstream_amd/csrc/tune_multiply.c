@@ -1,0 +1,330 @@
+/*
+ * tune_multiply: exhaustive autotuner for the stack kernels (MI355X).
+ *
+ * Behaviour of the reference's tuner (samples/smm/tune_multiply.py:66-716, driven through
+ * acc_bench.x and OPENCL_LIBSMM_SMM_* variables) without its search library: for every
+ * (datatype, m, n, k) it measures every compiled kernel that fits the shape, times each
+ * configuration with HIP events on the launch stream (like acc_bench.c:330-344: warm-up, then
+ * repetitions over one synthetic stack), checks the fastest result against a host multiply
+ * (CHECK=1 of tune_multiply.py:288, :639-646), and writes
+ *   - one JSON line per measured configuration to the sweep file (resumable: shapes already
+ *     marked done there are skipped, tune_multiply.py:656-671), and
+ *   - the best configuration per shape to a CSV in the MI355X format
+ *     DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH (load with OPENCL_LIBSMM_SMM_PARAMS).
+ *
+ * usage: tune_multiply [-t f64|f32] [-s stack] [-r reps] [-o best.csv] [-j sweep.jsonl]
+ *                      [-d device] [-f triplets.txt | m n k [m n k ...]]
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include "../../include/libxstream_amd.h"
+
+typedef struct tune_opts_t {
+  int dtype, stack_size, reps;
+  const char *csv, *jsonl, *device;
+} tune_opts_t;
+
+static uint64_t splitmix64(uint64_t* state) {
+  uint64_t z = (*state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+static double uniform(uint64_t* state) { /* [-1, 1) */
+  return 2.0 * (double)(splitmix64(state) >> 11) * (1.0 / 9007199254740992.0) - 1.0;
+}
+
+/* sorted-by-C stack: runs of 30 +- 4 entries over nc = S/30 C blocks, A/B pools of 2*nc blocks */
+typedef struct tune_work_t {
+  int m, n, k, stack_size, nc, na, nb, ts;
+  int* stack;
+  void *a, *b, *c;
+} tune_work_t;
+
+static int work_init(tune_work_t* w, int dtype, int m, int n, int k, int stack_size, uint64_t seed) {
+  uint64_t st = seed;
+  long i, count;
+  int c = 0, left;
+  memset(w, 0, sizeof(*w));
+  w->m = m;
+  w->n = n;
+  w->k = k;
+  w->stack_size = stack_size;
+  w->ts = (dbcsr_type_real_8 == dtype ? 8 : 4);
+  w->nc = (stack_size / 30 > 0 ? stack_size / 30 : 1);
+  w->na = w->nb = 2 * w->nc;
+  if ((double)w->na * m * k >= 2147483647.0 || (double)w->nc * m * n >= 2147483647.0) return EXIT_FAILURE;
+  w->stack = (int*)malloc(sizeof(int) * 3 * (size_t)stack_size);
+  w->a = malloc((size_t)w->ts * w->na * m * k);
+  w->b = malloc((size_t)w->ts * w->nb * k * n);
+  w->c = calloc((size_t)w->nc * m * n, (size_t)w->ts);
+  if (NULL == w->stack || NULL == w->a || NULL == w->b || NULL == w->c) return EXIT_FAILURE;
+  left = 26 + (int)(splitmix64(&st) % 9);
+  for (i = 0; i < stack_size; ++i) {
+    if (0 == left && c + 1 < w->nc) {
+      ++c;
+      left = 26 + (int)(splitmix64(&st) % 9);
+    }
+    if (0 < left) --left;
+    w->stack[3 * i + 0] = (int)(splitmix64(&st) % (uint64_t)w->na) * m * k + 1;
+    w->stack[3 * i + 1] = (int)(splitmix64(&st) % (uint64_t)w->nb) * k * n + 1;
+    w->stack[3 * i + 2] = c * m * n + 1;
+  }
+  count = (long)w->na * m * k;
+  for (i = 0; i < count; ++i) {
+    if (8 == w->ts) ((double*)w->a)[i] = uniform(&st);
+    else ((float*)w->a)[i] = (float)uniform(&st);
+  }
+  count = (long)w->nb * k * n; /* row-major k-by-n blocks: the layout the kernels read */
+  for (i = 0; i < count; ++i) {
+    if (8 == w->ts) ((double*)w->b)[i] = uniform(&st);
+    else ((float*)w->b)[i] = (float)uniform(&st);
+  }
+  return EXIT_SUCCESS;
+}
+
+static void work_free(tune_work_t* w) {
+  free(w->stack);
+  free(w->a);
+  free(w->b);
+  free(w->c);
+}
+
+static double elem(const void* p, int ts, long i) { return 8 == ts ? ((const double*)p)[i] : ((const float*)p)[i]; }
+
+/* host check of the first `nruns` C blocks: max |C - C_host| / max |C_host| (double accumulation) */
+static double host_check(const tune_work_t* w, const void* cdev_copy, int nblocks) {
+  const int m = w->m, n = w->n, k = w->k, mn = m * n;
+  double* ref = (double*)calloc((size_t)nblocks * mn, sizeof(double));
+  double maxref = 0, maxerr = 0;
+  long s;
+  int i;
+  if (NULL == ref) return INFINITY;
+  for (s = 0; s < w->stack_size; ++s) {
+    const long a0 = w->stack[3 * s] - 1, b0 = w->stack[3 * s + 1] - 1, c0 = w->stack[3 * s + 2] - 1;
+    int r, col, l;
+    if (c0 / mn >= nblocks) continue;
+    for (col = 0; col < n; ++col) {
+      for (r = 0; r < m; ++r) {
+        double sum = 0;
+        for (l = 0; l < k; ++l) sum += elem(w->a, w->ts, a0 + (long)l * m + r) * elem(w->b, w->ts, b0 + (long)l * n + col);
+        ref[c0 + (long)col * m + r] += sum;
+      }
+    }
+  }
+  for (i = 0; i < nblocks * mn; ++i) {
+    const double d = fabs(elem(cdev_copy, w->ts, i) - ref[i]);
+    if (fabs(ref[i]) > maxref) maxref = fabs(ref[i]);
+    if (!(d <= maxerr)) maxerr = d;
+  }
+  free(ref);
+  return maxerr / (maxref > 0 ? maxref : 1);
+}
+
+static int shape_done(const char* jsonl, int dtype, int m, int n, int k, int stack_size) {
+  char line[1024], key[160];
+  FILE* const f = (NULL != jsonl ? fopen(jsonl, "r") : NULL);
+  int done = 0;
+  if (NULL == f) return 0;
+  snprintf(key, sizeof(key), "\"done\": 1, \"dtype\": %i, \"m\": %i, \"n\": %i, \"k\": %i, \"S\": %i,", dtype, m, n, k,
+    stack_size);
+  while (!done && NULL != fgets(line, sizeof(line), f)) done = (NULL != strstr(line, key));
+  fclose(f);
+  return done;
+}
+
+static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
+  const int dtype = o->dtype;
+  tune_work_t w;
+  void *da = NULL, *db = NULL, *dc = NULL, *ds = NULL, *stream = NULL;
+  int cand[5 * 256], ncand, i, j, b, f, result = EXIT_SUCCESS, nmeasured = 0;
+  static const int bss[] = {16, 32, 64, 128};
+  libsmm_acc_smm_config_t best, order[2048];
+  double times[2048], flops;
+  FILE* sweep = NULL;
+  if (shape_done(o->jsonl, dtype, m, n, k, o->stack_size)) {
+    fprintf(stderr, "tune_multiply: %ix%ix%i already tuned (skipped)\n", m, n, k);
+    return EXIT_SUCCESS;
+  }
+  if (EXIT_SUCCESS != work_init(&w, dtype, m, n, k, o->stack_size, 0x2507197525071975ull ^ (uint64_t)(m * 7919 + n * 104729 + k))) {
+    work_free(&w);
+    return EXIT_FAILURE;
+  }
+  flops = 2.0 * m * n * k * (double)w.stack_size;
+  result |= c_dbcsr_acc_stream_create(&stream, "tune", -1);
+  result |= c_dbcsr_acc_dev_mem_allocate(&da, (size_t)w.ts * w.na * m * k);
+  result |= c_dbcsr_acc_dev_mem_allocate(&db, (size_t)w.ts * w.nb * k * n);
+  result |= c_dbcsr_acc_dev_mem_allocate(&dc, (size_t)w.ts * w.nc * m * n);
+  result |= c_dbcsr_acc_dev_mem_allocate(&ds, sizeof(int) * 3 * (size_t)w.stack_size);
+  if (EXIT_SUCCESS == result) {
+    result |= c_dbcsr_acc_memcpy_h2d(w.a, da, (size_t)w.ts * w.na * m * k, stream);
+    result |= c_dbcsr_acc_memcpy_h2d(w.b, db, (size_t)w.ts * w.nb * k * n, stream);
+    result |= c_dbcsr_acc_memcpy_h2d(w.stack, ds, sizeof(int) * 3 * (size_t)w.stack_size, stream);
+    result |= c_dbcsr_acc_memset_zero(dc, 0, (size_t)w.ts * w.nc * m * n, stream);
+    result |= c_dbcsr_acc_stream_sync(stream);
+  }
+  ncand = libsmm_acc_kernel_candidates((libsmm_acc_data_t)dtype, m, n, k, cand, 256);
+  if (NULL != o->jsonl) sweep = fopen(o->jsonl, "a");
+  libxstream_profile_enable(1);
+  for (i = 0; i < ncand && i < 256 && EXIT_SUCCESS == result; ++i) {
+    for (b = 0; b < 4; ++b) {
+      for (f = 1; f <= 2; ++f) {
+        libsmm_acc_smm_config_t cfg;
+        long cnt = 0;
+        double ms = 0;
+        int r;
+        memset(&cfg, 0, sizeof(cfg));
+        cfg.variant = cand[5 * i];
+        cfg.tm = cand[5 * i + 1];
+        cfg.tn = cand[5 * i + 2];
+        cfg.nt = cand[5 * i + 3];
+        cfg.kt = cand[5 * i + 4];
+        cfg.bs = bss[b];
+        cfg.flush = f;
+        if (EXIT_SUCCESS != libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) {
+          if (0 != cfg.variant) continue;
+          for (cfg.kt = (k > 8 ? (k - 1) / 8 * 8 : k - 1); 0 < cfg.kt; cfg.kt -= (cfg.kt > 8 ? 8 : 1)) {
+            if (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) break;
+          }
+          if (0 >= cfg.kt) continue;
+        }
+        if (EXIT_SUCCESS != libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da,
+                              db, dc, m, n, k, stream)) {
+          (void)libxstream_error_consume(NULL, 0);
+          continue;
+        }
+        libxstream_profile_reset();
+        for (r = 0; r < o->reps; ++r) {
+          libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,
+            stream);
+        }
+        c_dbcsr_acc_stream_sync(stream);
+        libxstream_profile_get("smm", &cnt, &ms, NULL, NULL);
+        if (0 >= cnt || 0 >= ms) continue;
+        ms /= (double)cnt;
+        cfg.gflops = flops / (ms * 1e6);
+        if (nmeasured < 2048) {
+          order[nmeasured] = cfg;
+          times[nmeasured++] = ms;
+        }
+        if (NULL != sweep) {
+          fprintf(sweep,
+            "{\"dtype\": %i, \"m\": %i, \"n\": %i, \"k\": %i, \"S\": %i, \"variant\": %i, \"tm\": %i, \"tn\": %i, "
+            "\"nt\": %i, \"bs\": %i, \"kt\": %i, \"flush\": %i, \"ms\": %.6f, \"gflops\": %.2f}\n",
+            dtype, m, n, k, w.stack_size, cfg.variant, cfg.tm, cfg.tn, cfg.nt, cfg.bs, cfg.kt, cfg.flush, ms, cfg.gflops);
+        }
+      }
+    }
+  }
+  libxstream_profile_enable(0);
+  /* fastest first; the first one that passes the host check wins */
+  for (i = 0; i < nmeasured; ++i) {
+    for (j = i + 1; j < nmeasured; ++j) {
+      if (times[j] < times[i]) {
+        const double t = times[i];
+        const libsmm_acc_smm_config_t c = order[i];
+        times[i] = times[j];
+        order[i] = order[j];
+        times[j] = t;
+        order[j] = c;
+      }
+    }
+  }
+  memset(&best, 0, sizeof(best));
+  for (i = 0; i < nmeasured && i < 8 && EXIT_SUCCESS == result; ++i) {
+    const int nblocks = (w.nc < 40 ? w.nc : 40);
+    double err;
+    void* host = malloc((size_t)w.ts * nblocks * m * n);
+    if (NULL == host) break;
+    c_dbcsr_acc_memset_zero(dc, 0, (size_t)w.ts * w.nc * m * n, stream);
+    libsmm_acc_process_config(order + i, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,
+      stream);
+    c_dbcsr_acc_memcpy_d2h(dc, host, (size_t)w.ts * nblocks * m * n, stream);
+    c_dbcsr_acc_stream_sync(stream);
+    err = host_check(&w, host, nblocks);
+    free(host);
+    if (err <= (8 == w.ts ? 1e-13 : 1e-5)) {
+      best = order[i];
+      break;
+    }
+    fprintf(stderr, "tune_multiply: %ix%ix%i configuration %i failed the check (error %g)\n", m, n, k, i, err);
+  }
+  if (0 < best.nt) {
+    libsmm_acc_config_set((libsmm_acc_data_t)dtype, m, n, k, &best);
+    printf("%ix%ix%i %s: %.1f GFLOPS/s (variant=%i tm=%i tn=%i nt=%i bs=%i kt=%i flush=%i, %i configurations)\n", m,
+      n, k, 8 == w.ts ? "f64" : "f32", best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt,
+      best.flush, nmeasured);
+    if (NULL != o->csv) {
+      FILE* const csv = fopen(o->csv, "a+");
+      if (NULL != csv) {
+        fseek(csv, 0, SEEK_END);
+        if (0 == ftell(csv)) fprintf(csv, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH\n");
+        fprintf(csv, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i\n", o->device, dtype, m, n, k, w.stack_size,
+          best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt, best.flush);
+        fclose(csv);
+      }
+    }
+    if (NULL != sweep) {
+      fprintf(sweep, "{\"done\": 1, \"dtype\": %i, \"m\": %i, \"n\": %i, \"k\": %i, \"S\": %i, \"gflops\": %.2f}\n", dtype,
+        m, n, k, w.stack_size, best.gflops);
+    }
+  }
+  else {
+    fprintf(stderr, "tune_multiply: %ix%ix%i: no configuration passed\n", m, n, k);
+    result = EXIT_FAILURE;
+  }
+  if (NULL != sweep) fclose(sweep);
+  c_dbcsr_acc_dev_mem_deallocate(da);
+  c_dbcsr_acc_dev_mem_deallocate(db);
+  c_dbcsr_acc_dev_mem_deallocate(dc);
+  c_dbcsr_acc_dev_mem_deallocate(ds);
+  c_dbcsr_acc_stream_destroy(stream);
+  work_free(&w);
+  return result;
+}
+
+int main(int argc, char* argv[]) {
+  tune_opts_t o;
+  int i, result, ndevices = 0;
+  FILE* triplets = NULL;
+  o.dtype = dbcsr_type_real_8;
+  o.stack_size = 30000;
+  o.reps = 5;
+  o.csv = NULL;
+  o.jsonl = NULL;
+  o.device = "MI355X";
+  for (i = 1; i < argc && '-' == argv[i][0] && '\0' != argv[i][1] && i + 1 < argc; i += 2) {
+    const char* const v = argv[i + 1];
+    switch (argv[i][1]) {
+      case 't': o.dtype = (0 == strcmp(v, "f32") ? dbcsr_type_real_4 : dbcsr_type_real_8); break;
+      case 's': o.stack_size = atoi(v); break;
+      case 'r': o.reps = atoi(v); break;
+      case 'o': o.csv = v; break;
+      case 'j': o.jsonl = v; break;
+      case 'd': o.device = v; break;
+      case 'f': triplets = fopen(v, "r"); break;
+      default: fprintf(stderr, "tune_multiply: unknown option %s\n", argv[i]); return EXIT_FAILURE;
+    }
+  }
+  if (0 >= o.stack_size || 0 >= o.reps) return EXIT_FAILURE;
+  result = libsmm_acc_init();
+  if (EXIT_SUCCESS == result) result = c_dbcsr_acc_get_ndevices(&ndevices);
+  if (EXIT_SUCCESS != result || 0 >= ndevices) {
+    fprintf(stderr, "tune_multiply: no device\n");
+    return EXIT_FAILURE;
+  }
+  if (NULL != triplets) {
+    int m, n, k;
+    while (EXIT_SUCCESS == result && 3 == fscanf(triplets, "%i %i %i", &m, &n, &k)) result = tune_shape(&o, m, n, k);
+    fclose(triplets);
+  }
+  for (; i + 2 < argc + 0 && EXIT_SUCCESS == result; i += 3) {
+    result = tune_shape(&o, atoi(argv[i]), atoi(argv[i + 1]), atoi(argv[i + 2]));
+  }
+  libsmm_acc_finalize();
+  return result;
+}
