@@ -156,8 +156,9 @@ def test_reference_csv_rows():
     assert read_ref("gfx90a;3;0;23;23;30000;0;20;23;1;10;23;-2;-2;0;1;1;1;1;2;2;0;0")[0] != 0  # m = 0
 
 
-def test_params_load_reference_and_roundtrip(tmp_path):
+def test_params_load_reference_and_roundtrip(tmp_path, monkeypatch):
     L = lx.lib()
+    monkeypatch.setenv("OPENCL_LIBSMM_SMM_PARAMS", "0")  # start from an empty registry (no shipped CSV)
     assert L.libsmm_acc_init() == 0
     try:
         n = L.libsmm_acc_params_load(os.path.join(ROOT, "tests", "golden", "tune_multiply_excerpt.csv").encode())
@@ -175,6 +176,7 @@ def test_params_load_reference_and_roundtrip(tmp_path):
         L.libsmm_acc_params_save(str(out).encode(), b"MI355X")
         lx.config_set(lx.REAL_8, 23, 23, 23, lx.SmmConfig(variant=0, tm=4, tn=4, nt=64, bs=8))
         assert L.libsmm_acc_params_load(str(out).encode()) == 5
+        monkeypatch.delenv("OPENCL_LIBSMM_SMM_PARAMS")
         back, src = lx.config_get(lx.REAL_8, 23, 23, 23)
         assert (back.variant, back.tm, back.tn, back.nt, back.bs, back.flush) == (0, 3, 3, 128, 40, 1)
     finally:
@@ -201,6 +203,20 @@ def test_config_validation():
     assert not lx.config_valid(tid, 80, 80, 2000, lx.SmmConfig(variant=0, tm=5, tn=5, nt=256, bs=32, kt=2000))
     variants = lx.kernel_variants(tid)
     assert any(v[0] == 0 for v in variants) and any(v[0] == 1 for v in variants)
+
+
+def test_shipped_params(monkeypatch):
+    """libsmm_acc_init loads the tuned MI355X CSV that ships next to the library."""
+    monkeypatch.delenv("OPENCL_LIBSMM_SMM_PARAMS", raising=False)
+    L = lx.lib()
+    assert L.libsmm_acc_init() == 0
+    try:
+        for tid in (lx.REAL_8, lx.REAL_4):
+            for shape in [(5, 5, 5), (13, 13, 13), (23, 23, 23), (32, 32, 32), (64, 64, 64), (4, 9, 22), (45, 45, 13)]:
+                cfg, src = lx.config_get(tid, *shape)
+                assert src == 1 and cfg.gflops > 0 and lx.config_valid(tid, *shape, cfg), (tid, shape)
+    finally:
+        assert L.libsmm_acc_finalize() == 0
 
 
 def test_env_overrides(monkeypatch):
