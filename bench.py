@@ -48,6 +48,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC child runs")
     p.add_argument("--no-small", action="store_true", help="skip the 30000-entry stack line")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched mixed-shape measurement")
     p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -99,6 +100,58 @@ def timed(lx, res, stream, steps, warmup):
     prof = lx.profile_get("smm")
     lx.profile_enable(False)
     return (t1 - t0) * 1e3 / steps, prof["ms"] / max(prof["count"], 1)
+
+
+def batched(lx, stream, steps, warmup):
+    """24 stacks, (m, n, k) and lengths log-uniform in 4..64 and 1k..200k, FP64: one persistent launch
+    replayed from a hipGraph, against the same stacks as 24 libsmm_acc_process launches."""
+    from libxstream_amd.workload import make_mixed_batch
+
+    works = make_mixed_batch(24, seed=25071975, dtype=np.float64)
+    res = [Resident(lx, w, stream) for w in works]
+    stream.sync()
+    items = [(r.st, r.w.stack_size, r.w.m, r.w.n, r.w.k, r.a, r.b, r.c) for r in res]
+    batch = lx.Batch(lx.REAL_8, items)
+    info = batch.info()
+    for _ in range(warmup):
+        batch.replay(stream)
+    stream.sync()
+    lx.profile_enable(True)
+    lx.profile_reset()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.replay(stream)
+    stream.sync()
+    t1 = time.perf_counter()
+    prof = lx.profile_get("batch")
+    for _ in range(warmup):
+        for r in res:
+            r.process(stream)
+    stream.sync()
+    t2 = time.perf_counter()
+    for _ in range(steps):
+        for r in res:
+            r.process(stream)
+    stream.sync()
+    t3 = time.perf_counter()
+    lx.profile_enable(False)
+    batch.close()
+    for r in res:
+        r.free()
+    graph_ms = (t1 - t0) * 1e3 / steps
+    single_ms = (t3 - t2) * 1e3 / steps
+    return {
+        "workload": "24 FP64 stacks, (m,n,k) log-uniform in 4..64, lengths log-uniform 1k..200k, seed 25071975",
+        "entries": int(sum(w.stack_size for w in works)),
+        "shapes": ["%dx%dx%d:%d" % (w.m, w.n, w.k, w.stack_size) for w in works],
+        "tasks": info["ntasks"],
+        "graph_ms_per_step": round(graph_ms, 4),
+        "graph_gflops": round(info["flops"] / (graph_ms * 1e-3) * 1e-9, 1),
+        "graph_kernel_ms": round(prof["ms"] / max(prof["count"], 1), 4),
+        "alg_gbs": round(info["bytes"] / (prof["ms"] / max(prof["count"], 1) * 1e-3) * 1e-9, 1),
+        "separate_launches_ms_per_step": round(single_ms, 4),
+        "separate_launches_gflops": round(info["flops"] / (single_ms * 1e-3) * 1e-9, 1),
+    }
 
 
 def traffic_probe(args):
@@ -226,6 +279,7 @@ def main():
         small = {"stack_size": 30000, "blocks_a_b_c": [2000, 2000, 1000], "ms_per_step": round(s_step, 5),
                  "gflops": round(ws.flops() / (s_step * 1e-3) * 1e-9, 1), "kernel_ms": round(s_kernel, 5),
                  "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
+    batch = None if args.no_batch else batched(lx, stream, args.steps, args.warmup)
     ncus, arch = lx.device_info()
     stream.close()
     lx.finalize()
@@ -277,6 +331,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "stack_30000": small,
+        "batched_mixed": batch,
         "reference_published_gflops_23cubed_fp64_s30000": {"V100": 2459.7, "P100": 1657.8},
     }
     print(json.dumps(line))

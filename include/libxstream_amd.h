@@ -78,6 +78,31 @@ LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* devi
 LIBXSTREAM_API int libsmm_acc_read_ref_params(const char* line, int with_device, int with_perf, char* device,
   size_t device_size, int key[4], int params[16], double s_gflops[2]);
 
+/**
+ * Batched stack multiply (no counterpart in the reference): stacks of different (m, n, k) and
+ * lengths, each with the semantics of libsmm_acc_process, in one persistent kernel launch; the
+ * launch can be captured once into a hipGraph and replayed. m and n must lie in 1..64 (any k);
+ * all stacks share the batch's datatype. C-runs are flushed with atomic adds, so stacks may share
+ * C pools and need not be sorted.
+ */
+typedef struct libsmm_acc_batch_stack_t {
+  const int* dev_param_stack;
+  int stack_size, m, n, k;
+  const void* dev_a_data;
+  const void* dev_b_data;
+  void* dev_c_data;
+} libsmm_acc_batch_stack_t;
+typedef struct libsmm_acc_batch_t libsmm_acc_batch_t;
+/** plan the batch and upload the plan (allocates device memory; not for use under capture) */
+LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
+  int nstacks, libsmm_acc_batch_t** batch);
+/** enqueue the batch as one kernel launch on stream */
+LIBXSTREAM_API int libsmm_acc_batch_process(libsmm_acc_batch_t* batch, void* stream);
+/** replay the batch from a hipGraph (captured on the first call) on stream */
+LIBXSTREAM_API int libsmm_acc_batch_replay(libsmm_acc_batch_t* batch, void* stream);
+LIBXSTREAM_API int libsmm_acc_batch_info(const libsmm_acc_batch_t* batch, int* ntasks, double* flops, double* bytes);
+LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch);
+
 /** profiling: per-kernel launch records timed by HIP events on the launch stream */
 LIBXSTREAM_API int libxstream_profile_enable(int enable);
 LIBXSTREAM_API int libxstream_profile_reset(void);

@@ -49,6 +49,21 @@ class SmmConfig(ctypes.Structure):
         return "SmmConfig(%s)" % ", ".join("%s=%s" % kv for kv in self.as_dict().items())
 
 
+class BatchStack(ctypes.Structure):
+    """libsmm_acc_batch_stack_t (include/libxstream_amd.h)."""
+
+    _fields_ = [
+        ("dev_param_stack", ctypes.c_void_p),
+        ("stack_size", ctypes.c_int),
+        ("m", ctypes.c_int),
+        ("n", ctypes.c_int),
+        ("k", ctypes.c_int),
+        ("dev_a_data", ctypes.c_void_p),
+        ("dev_b_data", ctypes.c_void_p),
+        ("dev_c_data", ctypes.c_void_p),
+    ]
+
+
 def _declare(L):
     vp, i, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
     pvp, pi, psz = ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)
@@ -93,6 +108,11 @@ def _declare(L):
         "libsmm_acc_params_load": (i, [ctypes.c_char_p]),
         "libsmm_acc_params_save": (i, [ctypes.c_char_p, ctypes.c_char_p]),
         "libsmm_acc_read_ref_params": (i, [ctypes.c_char_p, i, i, ctypes.c_char_p, sz, pi, pi, ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_batch_create": (i, [i, ctypes.POINTER(BatchStack), i, pvp]),
+        "libsmm_acc_batch_process": (i, [vp, vp]),
+        "libsmm_acc_batch_replay": (i, [vp, vp]),
+        "libsmm_acc_batch_info": (i, [vp, pi, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_batch_destroy": (i, [vp]),
         "c_dbcsr_acc_init": (i, []),
         "c_dbcsr_acc_finalize": (i, []),
         "c_dbcsr_acc_get_ndevices": (i, [pi]),
@@ -300,3 +320,34 @@ def profile_get(kernel):
     check(lib().libxstream_profile_get(kernel.encode(), ctypes.byref(cnt), ctypes.byref(ms), ctypes.byref(fl),
                                        ctypes.byref(by)), "profile_get")
     return {"count": cnt.value, "ms": ms.value, "flops": fl.value, "bytes": by.value}
+
+
+class Batch:
+    """Several stacks of different shapes in one persistent launch (libsmm_acc_batch_*)."""
+
+    def __init__(self, tid, stacks):
+        """stacks: list of (stack DeviceBuffer, stack_size, m, n, k, a DeviceBuffer, b DeviceBuffer, c DeviceBuffer)."""
+        arr = (BatchStack * len(stacks))()
+        for x, (st, size, m, n, k, a, b, c) in zip(arr, stacks):
+            x.dev_param_stack = st.ptr.value if st and st.ptr else None
+            x.stack_size, x.m, x.n, x.k = size, m, n, k
+            x.dev_a_data, x.dev_b_data, x.dev_c_data = a.ptr.value, b.ptr.value, c.ptr.value
+        h = ctypes.c_void_p()
+        check(lib().libsmm_acc_batch_create(tid, arr, len(stacks), ctypes.byref(h)), "libsmm_acc_batch_create")
+        self.handle = h
+
+    def process(self, stream=None):
+        check(lib().libsmm_acc_batch_process(self.handle, stream.handle if stream else None), "libsmm_acc_batch_process")
+
+    def replay(self, stream=None):
+        check(lib().libsmm_acc_batch_replay(self.handle, stream.handle if stream else None), "libsmm_acc_batch_replay")
+
+    def info(self):
+        nt, fl, by = ctypes.c_int(0), ctypes.c_double(0), ctypes.c_double(0)
+        check(lib().libsmm_acc_batch_info(self.handle, ctypes.byref(nt), ctypes.byref(fl), ctypes.byref(by)))
+        return {"ntasks": nt.value, "flops": fl.value, "bytes": by.value}
+
+    def close(self):
+        if self.handle:
+            check(lib().libsmm_acc_batch_destroy(self.handle), "libsmm_acc_batch_destroy")
+            self.handle = None

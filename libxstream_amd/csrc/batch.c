@@ -1,0 +1,224 @@
+/*
+ * Batched stack multiply: stacks of different (m, n, k) and lengths in one persistent kernel
+ * launch, optionally replayed from a captured hipGraph. Each stack has the semantics of
+ * libsmm_acc_process (samples/smm/acc_libsmm.h:45-47); the batch has no reference counterpart.
+ *
+ * Plan (built once on the host): every stack is cut into chunks of run-aligned entries, sized so
+ * that a chunk moves about as many operand elements as 32 entries of 23x23x23; blocks that need
+ * more than 4 output tiles of 16x16 get one work-group task per chunk (its 4 wavefronts split the
+ * tiles), smaller blocks are packed 4 chunks per task (one per wavefront). Expensive tasks come
+ * first; the persistent grid strides over the task list.
+ */
+#include "lxs_internal.h"
+#include <string.h>
+
+#define LXS_BATCH_CHUNK_ELEMENTS (32 * 2 * 23 * 23)
+
+struct libsmm_acc_batch_t {
+  int dtype, nstacks, ntasks, nblocks;
+  lxs_batch_stack_t* desc;  /* device */
+  lxs_batch_task_t* tasks;  /* device */
+  hipStream_t capture;
+  hipGraph_t graph;
+  hipGraphExec_t exec;
+  double flops, bytes;
+};
+
+typedef struct lxs_chunk_t {
+  int stack, chunk;
+  double cost;
+} lxs_chunk_t;
+
+static int lxs_chunk_cmp(const void* x, const void* y) {
+  const double a = ((const lxs_chunk_t*)x)->cost, b = ((const lxs_chunk_t*)y)->cost;
+  return (a < b) - (a > b); /* descending */
+}
+
+LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
+  int nstacks, libsmm_acc_batch_t** batch) {
+  const int ts = (dbcsr_type_real_8 == datatype ? 8 : (dbcsr_type_real_4 == datatype ? 4 : 0));
+  lxs_batch_stack_t* desc = NULL;
+  lxs_batch_task_t* tasks = NULL;
+  lxs_chunk_t *big = NULL, *small = NULL;
+  libsmm_acc_batch_t* b = NULL;
+  int i, nbig = 0, nsmall = 0, ntasks = 0, result;
+  if (NULL == batch || NULL == stacks || 0 >= nstacks || 0 == ts) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: invalid argument");
+  }
+  *batch = NULL;
+  result = lxs_ensure_device();
+  if (EXIT_SUCCESS != result) return result;
+  desc = (lxs_batch_stack_t*)calloc((size_t)nstacks, sizeof(*desc));
+  b = (libsmm_acc_batch_t*)calloc(1, sizeof(*b));
+  if (NULL == desc || NULL == b) {
+    free(desc);
+    free(b);
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: out of memory");
+  }
+  for (i = 0; i < nstacks; ++i) { /* validate, size the chunks, count them */
+    const libsmm_acc_batch_stack_t* const s = stacks + i;
+    lxs_batch_stack_t* const d = desc + i;
+    if (0 >= s->m || 64 < s->m || 0 >= s->n || 64 < s->n || 0 >= s->k || 0 > s->stack_size ||
+        (0 < s->stack_size && (NULL == s->dev_param_stack || NULL == s->dev_a_data || NULL == s->dev_b_data ||
+                                 NULL == s->dev_c_data))) {
+      free(desc);
+      free(b);
+      return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: stack with m, n outside 1..64 or NULL buffers");
+    }
+    d->a = s->dev_a_data;
+    d->b = s->dev_b_data;
+    d->c = s->dev_c_data;
+    d->stack = s->dev_param_stack;
+    d->stack_size = s->stack_size;
+    d->m = s->m;
+    d->n = s->n;
+    d->k = s->k;
+    d->ti = (s->m + 15) / 16;
+    d->tj = (s->n + 15) / 16;
+    d->bs = LXS_BATCH_CHUNK_ELEMENTS / (s->m * s->k + s->k * s->n);
+    if (d->bs < 8) d->bs = 8;
+    if (d->bs > 512) d->bs = 512;
+    if (4 < d->ti * d->tj) nbig += (s->stack_size + d->bs - 1) / d->bs;
+    else nsmall += (s->stack_size + d->bs - 1) / d->bs;
+    b->flops += 2.0 * s->m * s->n * s->k * (double)s->stack_size;
+    b->bytes += (double)ts * s->stack_size * ((double)s->m * s->k + (double)s->k * s->n + 2.0 * s->m * s->n);
+  }
+  big = (lxs_chunk_t*)malloc(sizeof(lxs_chunk_t) * (size_t)(nbig + 1));
+  small = (lxs_chunk_t*)malloc(sizeof(lxs_chunk_t) * (size_t)(nsmall + 1));
+  tasks = (lxs_batch_task_t*)malloc(sizeof(lxs_batch_task_t) * (size_t)(nbig + (nsmall + 3) / 4 + 1));
+  if (NULL == big || NULL == small || NULL == tasks) {
+    result = lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: out of memory");
+  }
+  if (EXIT_SUCCESS == result) {
+    int c, j;
+    nbig = nsmall = 0;
+    for (i = 0; i < nstacks; ++i) {
+      const lxs_batch_stack_t* const d = desc + i;
+      const int nchunks = (d->stack_size + d->bs - 1) / d->bs;
+      /* cost of a chunk ~ the operand bytes it moves; a team chunk has 4 wavefronts on it */
+      const double cost = (double)d->bs * (d->m * d->k + d->k * d->n) / (4 < d->ti * d->tj ? 4.0 : 1.0);
+      for (c = 0; c < nchunks; ++c) {
+        lxs_chunk_t* const x = (4 < d->ti * d->tj ? big + nbig++ : small + nsmall++);
+        x->stack = i;
+        x->chunk = c;
+        x->cost = cost;
+      }
+    }
+    qsort(big, (size_t)nbig, sizeof(*big), lxs_chunk_cmp);
+    qsort(small, (size_t)nsmall, sizeof(*small), lxs_chunk_cmp);
+    for (i = 0; i < nbig; ++i) { /* team tasks first: they are the expensive ones */
+      lxs_batch_task_t* const t = tasks + ntasks++;
+      memset(t, 0, sizeof(*t));
+      t->team = 1;
+      t->stack[0] = big[i].stack;
+      t->chunk[0] = big[i].chunk;
+      for (j = 1; j < 4; ++j) t->stack[j] = -1;
+    }
+    for (i = 0; i < nsmall; i += 4) {
+      lxs_batch_task_t* const t = tasks + ntasks++;
+      memset(t, 0, sizeof(*t));
+      for (j = 0; j < 4; ++j) {
+        t->stack[j] = (i + j < nsmall ? small[i + j].stack : -1);
+        t->chunk[j] = (i + j < nsmall ? small[i + j].chunk : 0);
+      }
+    }
+  }
+  if (EXIT_SUCCESS == result) {
+    b->dtype = (int)datatype;
+    b->nstacks = nstacks;
+    b->ntasks = ntasks;
+    b->nblocks = (0 < lxs_cfg.ncus ? 8 * lxs_cfg.ncus : 2048);
+    if (b->nblocks > ntasks) b->nblocks = ntasks;
+    result = lxs_hip(hipMalloc((void**)&b->desc, sizeof(*desc) * (size_t)nstacks), "hipMalloc batch");
+  }
+  if (EXIT_SUCCESS == result && 0 < ntasks) {
+    result = lxs_hip(hipMalloc((void**)&b->tasks, sizeof(*tasks) * (size_t)ntasks), "hipMalloc batch");
+  }
+  if (EXIT_SUCCESS == result) {
+    result = lxs_hip(hipMemcpy(b->desc, desc, sizeof(*desc) * (size_t)nstacks, hipMemcpyHostToDevice), "batch upload");
+  }
+  if (EXIT_SUCCESS == result && 0 < ntasks) {
+    result = lxs_hip(hipMemcpy(b->tasks, tasks, sizeof(*tasks) * (size_t)ntasks, hipMemcpyHostToDevice), "batch upload");
+  }
+  free(big);
+  free(small);
+  free(tasks);
+  free(desc);
+  if (EXIT_SUCCESS == result) {
+    *batch = b;
+  }
+  else {
+    if (NULL != b->desc) (void)hipFree(b->desc);
+    if (NULL != b->tasks) (void)hipFree(b->tasks);
+    free(b);
+  }
+  return result;
+}
+
+static int lxs_batch_enqueue(libsmm_acc_batch_t* b, hipStream_t hs) {
+  const int code = lxs_batch_launch(b->dtype, b->desc, b->tasks, b->ntasks, b->nblocks, (void*)hs);
+  return (0 <= code ? lxs_hip((hipError_t)code, "batch launch") : lxs_error(code, "batch launch"));
+}
+
+LIBXSTREAM_API int libsmm_acc_batch_process(libsmm_acc_batch_t* batch, void* stream) {
+  const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
+  void* token;
+  int result;
+  if (NULL == batch) return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_process: NULL batch");
+  result = lxs_ensure_device();
+  if (EXIT_SUCCESS != result) return result;
+  token = lxs_profile_begin(hs);
+  result = lxs_batch_enqueue(batch, hs);
+  lxs_profile_end(token, hs, "batch", batch->flops, batch->bytes);
+  return result;
+}
+
+LIBXSTREAM_API int libsmm_acc_batch_replay(libsmm_acc_batch_t* batch, void* stream) {
+  const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
+  void* token;
+  int result;
+  if (NULL == batch) return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_replay: NULL batch");
+  result = lxs_ensure_device();
+  if (EXIT_SUCCESS == result && NULL == batch->exec) { /* capture the launch once */
+    if (NULL == batch->capture) {
+      result = lxs_hip(hipStreamCreateWithFlags(&batch->capture, hipStreamNonBlocking), "hipStreamCreate");
+    }
+    if (EXIT_SUCCESS == result) {
+      result = lxs_hip(hipStreamBeginCapture(batch->capture, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    }
+    if (EXIT_SUCCESS == result) {
+      const int launched = lxs_batch_enqueue(batch, batch->capture);
+      const int ended = lxs_hip(hipStreamEndCapture(batch->capture, &batch->graph), "hipStreamEndCapture");
+      result = (EXIT_SUCCESS != launched ? launched : ended);
+    }
+    if (EXIT_SUCCESS == result) {
+      result = lxs_hip(hipGraphInstantiate(&batch->exec, batch->graph, NULL, NULL, 0), "hipGraphInstantiate");
+    }
+  }
+  if (EXIT_SUCCESS != result) return result;
+  token = lxs_profile_begin(hs);
+  result = lxs_hip(hipGraphLaunch(batch->exec, hs), "hipGraphLaunch");
+  lxs_profile_end(token, hs, "batch", batch->flops, batch->bytes);
+  return result;
+}
+
+LIBXSTREAM_API int libsmm_acc_batch_info(const libsmm_acc_batch_t* batch, int* ntasks, double* flops, double* bytes) {
+  if (NULL == batch) return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_info: NULL batch");
+  if (NULL != ntasks) *ntasks = batch->ntasks;
+  if (NULL != flops) *flops = batch->flops;
+  if (NULL != bytes) *bytes = batch->bytes;
+  return EXIT_SUCCESS;
+}
+
+LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch) {
+  int result = EXIT_SUCCESS;
+  if (NULL != batch) {
+    if (NULL != batch->exec) result = lxs_hip(hipGraphExecDestroy(batch->exec), "hipGraphExecDestroy");
+    if (NULL != batch->graph) (void)hipGraphDestroy(batch->graph);
+    if (NULL != batch->capture) (void)hipStreamDestroy(batch->capture);
+    if (NULL != batch->desc) (void)hipFree(batch->desc);
+    if (NULL != batch->tasks) (void)hipFree(batch->tasks);
+    free(batch);
+  }
+  return result;
+}

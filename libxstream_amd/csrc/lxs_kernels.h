@@ -45,6 +45,29 @@ int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* quintu
 size_t lxs_lds_limit(void);
 int lxs_smm_launch(const lxs_smm_args_t* args, void* hip_stream);
 
+/** one stack of a batch (device pointers; m, n <= 64) */
+typedef struct lxs_batch_stack_t {
+  const void* a;
+  const void* b;
+  void* c;
+  const int* stack;
+  int stack_size, m, n, k;
+  int bs;     /* entries per chunk */
+  int ti, tj; /* 16x16 output tiles covering the C block */
+  int pad;
+} lxs_batch_stack_t;
+
+/** up to 4 chunks (one per wavefront), or one chunk split over the 4 wavefronts (team != 0) */
+typedef struct lxs_batch_task_t {
+  int stack[4]; /* stack index, -1 = none */
+  int chunk[4];
+  int team;
+  int pad[3];
+} lxs_batch_task_t;
+
+int lxs_batch_launch(int dtype, const lxs_batch_stack_t* desc, const lxs_batch_task_t* tasks, int ntasks, int nblocks,
+  void* hip_stream);
+
 int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* data, int dtype, int m, int n,
   void* hip_stream);
 int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream);

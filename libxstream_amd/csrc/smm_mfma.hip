@@ -20,11 +20,11 @@
  *   - fragment loads carry no branches and no selects on loaded values (padding lanes pick their
  *     address instead), so the compiler can count outstanding loads exactly.
  *
- * Output layout of one 16x16 tile, lane l (0..63), register r (0..3), column j = l % 16:
- *   FP64 row = l/16 + 4r, FP32 row = 4(l/16) + r (checked on gfx950 with a layout probe).
+ * Fragment and output-tile lane layouts: lxs_mfma.h.
  */
 #include <hip/hip_runtime.h>
 #include "lxs_kernels.h"
+#include "lxs_mfma.h"
 
 #if !defined(LXS_REAL)
 #  define LXS_REAL double
@@ -36,53 +36,13 @@
 namespace {
 
 typedef LXS_REAL real_t;
-typedef double d4_t __attribute__((ext_vector_type(4)));
-typedef float f4_t __attribute__((ext_vector_type(4)));
-
 struct mfma_entry_t {
   int ti, tj, nwi, kb, m, n, k; /* m = n = k = 0: runtime shape with m <= 16*ti, n <= 16*tj */
   const void* fn;
 };
 
-template <typename T>
-struct mfma;
-template <>
-struct mfma<double> {
-  typedef d4_t acc_t;
-  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int r) { return (lane >> 4) + 4 * r; }
-};
-template <>
-struct mfma<float> {
-  typedef f4_t acc_t;
-  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
-};
-
-/* zero operands for fragment lanes past k (read-only) */
-__device__ const double lxs_zeros_f64[2] = {0.0, 0.0};
-__device__ const float lxs_zeros_f32[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-template <typename T>
-__device__ __forceinline__ const T* lxs_zeros();
-template <>
-__device__ __forceinline__ const double* lxs_zeros<double>() {
-  return lxs_zeros_f64;
-}
-template <>
-__device__ __forceinline__ const float* lxs_zeros<float>() {
-  return lxs_zeros_f32;
-}
-
-/* load through the global address space: the fragment pointers may point at the operand blocks or
-   at the zero array, and a generic (flat) load would force the compiler to wait for all loads */
-template <typename T>
-__device__ __forceinline__ T gload(const T* p) {
-  return *(const __attribute__((address_space(1))) T*)p;
-}
+using lxs::gload;
+using lxs::mfma;
 
 template <typename T, int TI, int TJ, int NWI, int KB, int MC, int NC, int KC>
 __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, const T* __restrict__ adata,
@@ -127,7 +87,7 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
   for (int ri = 0; ri < RI; ++ri) aoff[ri] = min(16 * (ti0 + ri) + li, M - 1);
 #pragma unroll
   for (int tj = 0; tj < TJ; ++tj) boff[tj] = min(16 * tj + li, N - 1);
-  const T* const zeros = lxs_zeros<T>();
+  const T* const zeros = lxs::zeros<T>();
 
   acc_t acc[RI][TJ], prod[RI][TJ];
   T fa[KB][RI], fb[KB][TJ];

@@ -94,3 +94,19 @@ def make_stack_workload(m, n, k, stack_size, nc=None, na=None, nb=None, dtype=np
     else:
         c = rng.uniform(-1.0, 1.0, size=(nc, mn)).astype(dtype)
     return StackWorkload(m, n, k, dtype, stack, a, b, c, nc, na, nb)
+
+
+def mixed_shapes(count, seed, dim_range=(4, 64), len_range=(1000, 200000)):
+    """(m, n, k, stack_size) drawn log-uniformly: the mixed-shape batch of 24 stacks uses the defaults."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.log(dim_range[0]), np.log(dim_range[1])
+    dims = np.rint(np.exp(rng.uniform(lo, hi, size=(count, 3)))).astype(int)
+    lens = np.rint(np.exp(rng.uniform(np.log(len_range[0]), np.log(len_range[1]), size=count))).astype(int)
+    return [(int(d[0]), int(d[1]), int(d[2]), int(s)) for d, s in zip(dims, lens)]
+
+
+def make_mixed_batch(count=24, seed=25071975, dtype=np.float64, dim_range=(4, 64), len_range=(1000, 200000),
+                     c_init="zero"):
+    """Stack workloads for a mixed-shape batch: shapes and lengths from mixed_shapes(), pools per stack."""
+    return [make_stack_workload(m, n, k, s, dtype=dtype, seed=seed + 7 * i, c_init=c_init)
+            for i, (m, n, k, s) in enumerate(mixed_shapes(count, seed, dim_range, len_range))]
