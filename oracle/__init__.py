@@ -42,6 +42,8 @@ def lib():
         L.oracle_cpu_stack.argtypes = [i, i, i, i, i, vp, vp, vp, vp, i, i, i]
         L.oracle_cpu_stack.restype = d
         L.oracle_max_threads.restype = i
+        L.oracle_gemm_dd.argtypes = [ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, vp, vp, i, vp, vp]
+        L.oracle_gemm_dd.restype = i
         _lib = L
     return _lib
 
@@ -120,3 +122,25 @@ def cpu_stack(m, n, k, stack, a, b, c, nrepeat=1, warmup=0, nthreads=0):
 
 def max_threads():
     return lib().oracle_max_threads()
+
+
+def gemm_dd(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, rows=None, cols=None):
+    """Double-double reference of C = alpha*op(A)*op(B) + beta*C (column-major); (hi, lo) for all
+    entries (shape n-by-m... returned column-major as flat arrays) or for the listed (rows, cols)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    c = None if c is None else np.ascontiguousarray(c, dtype=np.float64)
+    if rows is not None:
+        rows = np.ascontiguousarray(rows, dtype=np.int32)
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        count = len(rows)
+    else:
+        count = m * n
+    hi = np.empty(count, dtype=np.float64)
+    lo = np.empty(count, dtype=np.float64)
+    rc = lib().oracle_gemm_dd(transa.encode(), transb.encode(), m, n, k, alpha, a.ctypes.data, lda, b.ctypes.data, ldb,
+                              beta, None if c is None else c.ctypes.data, ldc, hi.ctypes.data, lo.ctypes.data,
+                              0 if rows is None else count, None if rows is None else rows.ctypes.data,
+                              None if rows is None else cols.ctypes.data)
+    assert rc == 0
+    return hi, lo

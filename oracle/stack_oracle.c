@@ -196,3 +196,48 @@ int oracle_max_threads(void) {
   return 1;
 #endif
 }
+
+/*
+ * GEMM oracle for the Ozaki emulation: C = alpha * op(A) * op(B) + beta * C, column-major BLAS
+ * semantics (samples/ozaki/ozaki_opencl.h:420-421), every product exact (TwoProd) and summed in
+ * double-double; alpha/beta applied in double-double too. Only the entries listed in (rows, cols)
+ * are computed when nsample > 0 (size-independent spot checks of large products), else all.
+ */
+static inline void dd_mul_d(double hi, double lo, double b, double* rhi, double* rlo) {
+  const double p = hi * b, e = fma(hi, b, -p);
+  double s, t;
+  s = p;
+  t = e + lo * b;
+  *rhi = s + t;
+  *rlo = t - (*rhi - s);
+}
+
+int oracle_gemm_dd(char transa, char transb, int m, int n, int k, double alpha, const double* a, int lda, const double* b,
+  int ldb, double beta, const double* c_in, int ldc, double* c_hi, double* c_lo, int nsample, const int* rows,
+  const int* cols) {
+  const long count = (0 < nsample ? nsample : (long)m * n);
+  long x;
+  if (0 > m || 0 > n || 0 > k) return 1;
+#if defined(_OPENMP)
+#  pragma omp parallel for schedule(dynamic, 16)
+#endif
+  for (x = 0; x < count; ++x) {
+    const int i = (0 < nsample ? rows[x] : (int)(x % m)), j = (0 < nsample ? cols[x] : (int)(x / m));
+    double hi = 0, lo = 0, rh, rl, ch, cl;
+    int l;
+    for (l = 0; l < k; ++l) {
+      const double av = ('N' == transa || 'n' == transa) ? a[i + (long)l * lda] : a[l + (long)i * lda];
+      const double bv = ('N' == transb || 'n' == transb) ? b[l + (long)j * ldb] : b[j + (long)l * ldb];
+      const double prod = av * bv, err = fma(av, bv, -prod);
+      dd_add(&hi, &lo, prod, err);
+    }
+    dd_mul_d(hi, lo, alpha, &rh, &rl);
+    ch = 0;
+    cl = 0;
+    if (0 != beta && NULL != c_in) dd_mul_d(c_in[i + (long)j * ldc], 0.0, beta, &ch, &cl);
+    dd_add(&rh, &rl, ch, cl);
+    c_hi[x] = rh;
+    c_lo[x] = rl;
+  }
+  return 0;
+}
