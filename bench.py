@@ -49,6 +49,8 @@ def parse():
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC child runs")
     p.add_argument("--no-small", action="store_true", help="skip the 30000-entry stack line")
     p.add_argument("--no-batch", action="store_true", help="skip the batched mixed-shape measurement")
+    p.add_argument("--no-ozaki", action="store_true", help="skip the Ozaki-int8 DGEMM measurement")
+    p.add_argument("--ozaki-sizes", default="1024,4096,8192", help="square DGEMM sizes for the Ozaki section")
     p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -151,6 +153,148 @@ def batched(lx, stream, steps, warmup):
         "alg_gbs": round(info["bytes"] / (prof["ms"] / max(prof["count"], 1) * 1e-3) * 1e-9, 1),
         "separate_launches_ms_per_step": round(single_ms, 4),
         "separate_launches_gflops": round(info["flops"] / (single_ms * 1e-3) * 1e-9, 1),
+    }
+
+
+def _rowcol(rng, rows, cols, e, per_row):
+    """mantissas uniform +-[1,2) times 2^x, x uniform in [-e, e] per row (per_row) or per column"""
+    mant = rng.uniform(1.0, 2.0, (rows, cols))
+    mant *= np.where(rng.integers(0, 2, (rows, cols), dtype=np.int8) != 0, 1.0, -1.0)
+    return mant * np.exp2(rng.integers(-e, e + 1, (rows, 1) if per_row else (1, cols))).astype(np.float64)
+
+
+def _sampled_error(A, B, c_colmajor, rng, count=2048):
+    """normwise error over sampled entries against 80-bit extended-precision dot products"""
+    if np.finfo(np.longdouble).nmant < 63:
+        return None
+    m, n = A.shape[0], B.shape[1]
+    i, j = rng.integers(0, m, count), rng.integers(0, n, count)
+    ref = (A[i].astype(np.longdouble) * B[:, j].T.astype(np.longdouble)).sum(axis=1)
+    got = c_colmajor[i + j * m].astype(np.longdouble)
+    return float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum()))
+
+
+class _RocBlas:
+    """The vendor FP64 GEMM (rocBLAS) as a second comparator; None when the library is absent."""
+
+    def __init__(self, hip_stream):
+        import ctypes
+
+        self.ct = ctypes
+        self.lib = None
+        for name in ("librocblas.so", "/opt/rocm/lib/librocblas.so"):
+            try:
+                self.lib = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if self.lib is None:
+            return
+        self.handle = ctypes.c_void_p()
+        assert self.lib.rocblas_create_handle(ctypes.byref(self.handle)) == 0
+        assert self.lib.rocblas_set_stream(self.handle, ctypes.c_void_p(hip_stream)) == 0
+        self.lib.rocblas_dgemm.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p,
+                                           ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_double), ctypes.c_void_p, ctypes.c_int]
+
+    def dgemm(self, n, a, b, c):
+        one, zero = self.ct.c_double(1.0), self.ct.c_double(0.0)
+        rc = self.lib.rocblas_dgemm(self.handle, 111, 111, n, n, n, self.ct.byref(one), a.ptr, n, b.ptr, n,
+                                    self.ct.byref(zero), c.ptr, n)
+        assert rc == 0, "rocblas_dgemm returned %d" % rc
+
+    def close(self):
+        if self.lib is not None:
+            self.lib.rocblas_destroy_handle(self.handle)
+
+
+def _event_ms(lx, stream, fn, reps):
+    fn()  # warmup (first call grows the scratch)
+    stream.sync()
+    e0, e1 = lx.Event(), lx.Event()
+    e0.record(stream)
+    for _ in range(reps):
+        fn()
+    e1.record(stream)
+    e1.sync()
+    ms = e0.elapsed_ms(e1) / reps
+    e0.close()
+    e1.close()
+    return ms
+
+
+OZAKI_CONFIGS = ((7, 6), (8, 7), (8, 0))  # (slices, trim): triangle at 7 and 8 slices, the reference default
+
+
+def ozaki_section(lx, stream, sizes, reps=3):
+    """Square DGEMM N x N x N: native FP64 matrix cores (own kernel and rocBLAS) against Ozaki-int8."""
+    rng = np.random.default_rng(25071975)
+    roc = _RocBlas(stream.native())
+    results = []
+    for n in sizes:
+        flops = 2.0 * n ** 3
+        entry = {"n": n}
+        for e in ((10, 0, 30) if n <= 4096 else (10,)):  # e = 10 first: it carries the timing
+            A, B = _rowcol(rng, n, n, e, True), _rowcol(rng, n, n, e, False)
+            a, b = np.asfortranarray(A).ravel(order="F"), np.asfortranarray(B).ravel(order="F")
+            da, db = lx.DeviceBuffer.from_array(a), lx.DeviceBuffer.from_array(b)
+            dc = lx.DeviceBuffer(8 * n * n)
+            out = np.empty(n * n)
+            if e == 10:  # timing (independent of the exponent range)
+                ms = _event_ms(lx, stream, lambda: lx.dgemm_native("N", "N", n, n, n, 1.0, da, n, db, n, 0.0, dc, n,
+                                                                   stream), reps)
+                dc.download(out)
+                entry["native_ms"] = round(ms, 4)
+                entry["native_tflops"] = round(flops / ms * 1e-9, 2)
+                entry["native_error"] = _sampled_error(A, B, out, rng)
+                if roc.lib is not None:
+                    ms = _event_ms(lx, stream, lambda: roc.dgemm(n, da, db, dc), reps)
+                    dc.download(out)
+                    entry["rocblas_ms"] = round(ms, 4)
+                    entry["rocblas_tflops"] = round(flops / ms * 1e-9, 2)
+                    entry["rocblas_error"] = _sampled_error(A, B, out, rng)
+                entry["configs"] = []
+            for cfg_i, (sl, trim) in enumerate(OZAKI_CONFIGS):
+                oz = lx.Ozaki(nslices=sl, trim=trim)
+                pairs = sum(1 for p in range(sl) for q in range(sl) if p + q <= 2 * (sl - 1) - trim)
+                if e == 10:
+                    lx.profile_enable(True)
+                    lx.profile_reset()
+                    ms = _event_ms(lx, stream, lambda: oz.gemm("N", "N", n, n, n, 1.0, da, n, db, n, 0.0, dc, n,
+                                                               stream), reps)
+                    i8 = lx.profile_get("ozaki_i8")
+                    pre = lx.profile_get("ozaki_pre")
+                    lx.profile_enable(False)
+                    i8_ms = i8["ms"] / max(i8["count"], 1)
+                    best_native = min(entry["native_ms"], entry.get("rocblas_ms", float("inf")))
+                    entry["configs"].append({
+                        "slices": sl, "trim": trim, "pairs": pairs, "ms": round(ms, 4),
+                        "fp64_equiv_tflops": round(flops / ms * 1e-9, 2),
+                        "speedup_vs_native": round(entry["native_ms"] / ms, 3),
+                        "speedup_vs_best_native": round(best_native / ms, 3),
+                        "i8_kernel_ms": round(i8_ms, 4),
+                        "i8_tops": round(i8["flops"] / max(i8["count"], 1) / i8_ms * 1e-9, 1),
+                        "i8_frac_of_5pops": round(i8["flops"] / max(i8["count"], 1) / i8_ms * 1e-9 / 5000.0, 4),
+                        "preprocess_ms": round(pre["ms"] / max(pre["count"], 1), 4),
+                        "errors": {}})
+                else:
+                    oz.gemm("N", "N", n, n, n, 1.0, da, n, db, n, 0.0, dc, n, stream)
+                stream.sync()
+                dc.download(out)
+                oz.close()
+                entry["configs"][cfg_i]["errors"]["e=%d" % e] = _sampled_error(A, B, out, rng)
+            for x in (da, db, dc):
+                x.free()
+            del A, B, a, b
+        results.append(entry)
+    roc.close()
+    return {
+        "workload": "square DGEMM NN; A(i,k) = m 2^x_i, B(k,j) = m 2^y_j, m uniform +-[1,2), x, y uniform integers "
+        "in [-e, e]; timing at e=10",
+        "error": "normwise over 2048 sampled entries against 80-bit extended-precision dot products",
+        "native": "own v_mfma_f64_16x16x4 kernel (ozaki_dgemm_native) and rocBLAS dgemm",
+        "results": results,
     }
 
 
@@ -280,6 +424,7 @@ def main():
                  "gflops": round(ws.flops() / (s_step * 1e-3) * 1e-9, 1), "kernel_ms": round(s_kernel, 5),
                  "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
     batch = None if args.no_batch else batched(lx, stream, args.steps, args.warmup)
+    ozaki = None if args.no_ozaki else ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
     ncus, arch = lx.device_info()
     stream.close()
     lx.finalize()
@@ -332,6 +477,7 @@ def main():
         "cpu_baseline": cpu,
         "stack_30000": small,
         "batched_mixed": batch,
+        "ozaki": ozaki,
         "reference_published_gflops_23cubed_fp64_s30000": {"V100": 2459.7, "P100": 1657.8},
     }
     print(json.dumps(line))

@@ -1,0 +1,97 @@
+/*
+ * Ozaki scheme I: FP64/FP32 GEMM emulated with int8 matrix-core products (MI355X, gfx950).
+ *
+ * Drop-in for the reference's samples/ozaki/ozaki_opencl.h:328-440 (same entry points, argument
+ * meaning and return convention: EXIT_SUCCESS / EXIT_FAILURE). The context is caller-allocated as in
+ * the reference; its public fields carry the resolved settings, `impl` owns the device state.
+ *
+ * Scheme: every row of op(A) and column of op(B) is scaled by a power of two and cut into nslices
+ * signed 7-bit digits (int8 planes); slice pairs (p, q) with p + q <= cutoff = 2(nslices-1) - oztrim
+ * are multiplied with int8 MFMA, accumulated exactly in int32 per level p + q and recombined in FP64.
+ * Only Scheme 1 (mantissa slicing) is built: kind 2 (CRT) is rejected, kind 0/3 (adaptive) resolves
+ * to Scheme 1.
+ */
+#ifndef OZAKI_H
+#define OZAKI_H
+
+#include "libxstream.h"
+#include <stddef.h>
+
+/** samples/ozaki/ozaki_opencl.h:58 */
+typedef enum ozaki_flags_t { OZAKI_TRIANGULAR = 1, OZAKI_SYMMETRIZE = 2 } ozaki_flags_t;
+
+/**
+ * Number of slice products for nslices and cutoff co under flags (samples/ozaki/ozaki_opencl.h:60-73):
+ * flags 0 and 3 both cover every pair with p + q <= co.
+ */
+static inline int ozaki_count_pairs(int nslices, int co, int flags) {
+  int p, q, count = 0;
+  for (p = 0; p < nslices && p <= co; ++p) {
+    for (q = ((0 != (flags & OZAKI_TRIANGULAR)) ? p : 0); q < nslices && p + q <= co; ++q) {
+      count += ((0 != (flags & OZAKI_SYMMETRIZE)) && p != q) ? 2 : 1;
+    }
+  }
+  return count;
+}
+
+typedef struct ozaki_context_t {
+  int use_double; /* 1: fp64, 0: fp32 */
+  int kind;       /* resolved scheme: 1 = mantissa slicing */
+  int ndecomp;    /* number of decomposition components (= nslices) */
+  int nslices;    /* int8 digit planes per operand, 1..8 (default fp64: 8, fp32: 4) */
+  int nprimes;    /* CRT primes: 0 (Scheme 2 is not built) */
+  int ozflags;    /* OZAKI_TRIANGULAR | OZAKI_SYMMETRIZE: 0 and 3 compute the full sum */
+  int oztrim;     /* levels dropped from the top of the pair triangle, 0..2(nslices-1) */
+  int verbosity;  /* 0: quiet, 1: info, 2+: debug */
+  int tm, tn;     /* output tile of one work-group (128 x 128) */
+  int maxk;       /* K per preprocessing pass (0: all of K) */
+  int flush;      /* k-blocks of 64 accumulated in int32 before an FP64 fold (1..2080) */
+  void* impl;     /* device state owned by the context */
+} ozaki_context_t;
+
+/** samples/ozaki/ozaki_opencl.h:328; tm/tn: 0 or 128; ozgroups is accepted and ignored (Scheme 2) */
+LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_double, int kind, int verbosity, int ndecomp,
+  int ozflags, int oztrim, int ozgroups, int maxk);
+/** samples/ozaki/ozaki_opencl.h:330 */
+LIBXSTREAM_API void ozaki_destroy(ozaki_context_t* ctx);
+
+/** samples/ozaki/ozaki_opencl.h:347: device bytes one call of this shape can need (upper bound) */
+LIBXSTREAM_API size_t ozaki_scratch_size(const ozaki_context_t* ctx, char transa, char transb, int M, int N, int K,
+  int lda, int ldb, int ldc);
+/** samples/ozaki/ozaki_opencl.h:349: install caller-owned device memory (NULL reverts to the internal arena) */
+LIBXSTREAM_API int ozaki_scratch_set(ozaki_context_t* ctx, void* dev_mem, size_t nbytes);
+/** samples/ozaki/ozaki_opencl.h:358 */
+LIBXSTREAM_API void* ozaki_scratch_get(const ozaki_context_t* ctx, size_t* nbytes);
+
+/** samples/ozaki/ozaki_opencl.h:361-363 */
+typedef struct ozaki_tile_t {
+  int m, n;
+} ozaki_tile_t;
+/** samples/ozaki/ozaki_opencl.h:384: the output tile the GEMM kernel uses (fixed 128 x 128) */
+LIBXSTREAM_API ozaki_tile_t ozaki_tile_select(const ozaki_context_t* ctx, int M, int N, int rtm, int rtn);
+/** samples/ozaki/ozaki_opencl.h:411: N-panel width; the device path is not panelled, so N */
+LIBXSTREAM_API int ozaki_npanel(const ozaki_context_t* ctx, int M, int N, int tm, int tn);
+
+/**
+ * samples/ozaki/ozaki_opencl.h:420: C = alpha op(A) op(B) + beta C, column-major, op = 'N'/'T'/'C'.
+ * dev != 0: a, b, c are device pointers; dev == 0: host pointers (copied in and out on stream).
+ * Enqueued on stream (NULL: the calling thread's default stream); the caller synchronizes.
+ */
+LIBXSTREAM_API int ozaki_gemm(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M, int N,
+  int K, double alpha, const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc, int dev);
+
+/** samples/ozaki/ozaki_opencl.h:430 (3M method); alpha/beta point to (re, im); device pointers */
+LIBXSTREAM_API int ozaki_gemm_complex(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M,
+  int N, int K, const double* alpha, const void* a, int lda, const void* b, int ldb, const double* beta, void* c, int ldc);
+
+/** samples/ozaki/ozaki_opencl.h:440: no preprocessing cache is kept, so nothing can go stale */
+LIBXSTREAM_API void ozaki_invalidate_cache(ozaki_context_t* ctx, const void* a, const void* b);
+
+/**
+ * No counterpart in the reference (its comparator is the vendor BLAS): native FP64 matrix-core GEMM
+ * (v_mfma_f64_16x16x4f64), C = alpha op(A) op(B) + beta C on device pointers, column-major.
+ */
+LIBXSTREAM_API int ozaki_dgemm_native(libxstream_stream_t* stream, char transa, char transb, int M, int N, int K,
+  double alpha, const double* a, int lda, const double* b, int ldb, double beta, double* c, int ldc);
+
+#endif /*OZAKI_H*/

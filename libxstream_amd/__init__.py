@@ -64,6 +64,13 @@ class BatchStack(ctypes.Structure):
     ]
 
 
+class OzakiContext(ctypes.Structure):
+    """include/ozaki.h: ozaki_context_t (caller-allocated, filled by ozaki_init)."""
+    _fields_ = [(name, ctypes.c_int) for name in ("use_double", "kind", "ndecomp", "nslices", "nprimes", "ozflags",
+                                                  "oztrim", "verbosity", "tm", "tn", "maxk", "flush")] + [
+        ("impl", ctypes.c_void_p)]
+
+
 def _declare(L):
     vp, i, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
     pvp, pi, psz = ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)
@@ -118,6 +125,34 @@ def _declare(L):
         "c_dbcsr_acc_get_ndevices": (i, [pi]),
         "c_dbcsr_acc_set_active_device": (i, [i]),
         "c_calculate_norms": (i, [vp, i, vp, vp, vp, vp]),
+        "libsmm_acc_mlp_create": (i, [pvp, i, i, ctypes.c_ulonglong]),
+        "libsmm_acc_mlp_destroy": (i, [vp]),
+        "libsmm_acc_mlp_heads": (i, [vp, vp, i]),
+        "libsmm_acc_mlp_features": (i, [i, i, i, i, vp]),
+        "libsmm_acc_mlp_params": (i, [vp, vp, i]),
+        "libsmm_acc_mlp_set_params": (i, [vp, vp, i]),
+        "libsmm_acc_mlp_gradient": (i, [vp, vp, vp, vp, i, vp, ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_mlp_train": (i, [vp, vp, vp, vp, i, i, ctypes.c_double, ctypes.c_double,
+                                     ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_mlp_forward": (i, [vp, vp, i, vp]),
+        "libsmm_acc_mlp_save": (i, [vp, ctypes.c_char_p]),
+        "libsmm_acc_mlp_load": (i, [pvp, ctypes.c_char_p]),
+        "libsmm_acc_mlp_predict_config": (i, [vp, i, i, i, i, ctypes.POINTER(SmmConfig),
+                                              ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_learned_set": (i, [vp, ctypes.c_double]),
+        "ozaki_init": (i, [ctypes.POINTER(OzakiContext), i, i, i, i, i, i, i, i, i, i]),
+        "ozaki_destroy": (None, [ctypes.POINTER(OzakiContext)]),
+        "ozaki_scratch_size": (sz, [ctypes.POINTER(OzakiContext), ctypes.c_char, ctypes.c_char, i, i, i, i, i, i]),
+        "ozaki_scratch_set": (i, [ctypes.POINTER(OzakiContext), vp, sz]),
+        "ozaki_scratch_get": (vp, [ctypes.POINTER(OzakiContext), psz]),
+        "ozaki_gemm": (i, [ctypes.POINTER(OzakiContext), vp, ctypes.c_char, ctypes.c_char, i, i, i, ctypes.c_double,
+                           vp, i, vp, i, ctypes.c_double, vp, i, i]),
+        "ozaki_gemm_complex": (i, [ctypes.POINTER(OzakiContext), vp, ctypes.c_char, ctypes.c_char, i, i, i,
+                                   ctypes.POINTER(ctypes.c_double), vp, i, vp, i, ctypes.POINTER(ctypes.c_double), vp,
+                                   i]),
+        "ozaki_invalidate_cache": (None, [ctypes.POINTER(OzakiContext), vp, vp]),
+        "ozaki_dgemm_native": (i, [vp, ctypes.c_char, ctypes.c_char, i, i, i, ctypes.c_double, vp, i, vp, i,
+                                   ctypes.c_double, vp, i]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -351,3 +386,140 @@ class Batch:
         if self.handle:
             check(lib().libsmm_acc_batch_destroy(self.handle), "libsmm_acc_batch_destroy")
             self.handle = None
+
+
+class Ozaki:
+    """FP64/FP32 GEMM emulated with int8 matrix cores (include/ozaki.h, Ozaki scheme I)."""
+
+    def __init__(self, use_double=True, nslices=0, trim=0, flags=3, maxk=0, verbosity=0):
+        self.ctx = OzakiContext()
+        check(lib().ozaki_init(ctypes.byref(self.ctx), 0, 0, 1 if use_double else 0, 1, verbosity, nslices, flags, trim,
+                               0, maxk), "ozaki_init")
+
+    def gemm(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dev=True):
+        """C = alpha op(A) op(B) + beta C (column-major). dev: a, b, c are DeviceBuffers (else numpy arrays)."""
+        ptr = (lambda x: x.ptr) if dev else (lambda x: x.ctypes.data)
+        check(lib().ozaki_gemm(ctypes.byref(self.ctx), stream.handle if stream else None, transa.encode(),
+                               transb.encode(), m, n, k, alpha, ptr(a), lda, ptr(b), ldb, beta, ptr(c), ldc,
+                               1 if dev else 0), "ozaki_gemm")
+
+    def gemm_complex(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+        """Complex C = alpha op(A) op(B) + beta C on DeviceBuffers of interleaved (re, im) values."""
+        al = (ctypes.c_double * 2)(alpha.real, alpha.imag)
+        be = (ctypes.c_double * 2)(beta.real, beta.imag)
+        check(lib().ozaki_gemm_complex(ctypes.byref(self.ctx), stream.handle if stream else None, transa.encode(),
+                                       transb.encode(), m, n, k, al, a.ptr, lda, b.ptr, ldb, be, c.ptr, ldc),
+              "ozaki_gemm_complex")
+
+    def scratch_size(self, transa, transb, m, n, k, lda, ldb, ldc):
+        return lib().ozaki_scratch_size(ctypes.byref(self.ctx), transa.encode(), transb.encode(), m, n, k, lda, ldb,
+                                        ldc)
+
+    def scratch_set(self, buf):
+        check(lib().ozaki_scratch_set(ctypes.byref(self.ctx), buf.ptr if buf else None, buf.nbytes if buf else 0),
+              "ozaki_scratch_set")
+
+    def close(self):
+        if self.ctx.impl:
+            lib().ozaki_destroy(ctypes.byref(self.ctx))
+
+
+def dgemm_native(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+    """Native FP64 matrix-core GEMM on DeviceBuffers (the Ozaki comparator)."""
+    check(lib().ozaki_dgemm_native(stream.handle if stream else None, transa.encode(), transb.encode(), m, n, k, alpha,
+                                   a.ptr, lda, b.ptr, ldb, beta, c.ptr, ldc), "ozaki_dgemm_native")
+
+
+def mlp_heads():
+    """(segment starts incl. the end, class values per output: -1 for the regression output)"""
+    seg = np.zeros(64, dtype=np.int32)
+    vals = np.zeros(64, dtype=np.int32)
+    nh = lib().libsmm_acc_mlp_heads(seg.ctypes.data, vals.ctypes.data, 64)
+    seg = seg[:nh + 1].copy()
+    return seg, vals[:seg[-1]].copy()
+
+
+def mlp_features(tid, m, n, k):
+    f = np.zeros(8, dtype=np.float32)
+    check(lib().libsmm_acc_mlp_features(tid, m, n, k, f.ctypes.data), "libsmm_acc_mlp_features")
+    return f
+
+
+class Mlp:
+    """The learned predictor's network (libsmm_acc_mlp_*): training and inference on the GPU."""
+
+    def __init__(self, hidden=64, nhidden=3, seed=1, path=None):
+        h = ctypes.c_void_p()
+        if path is not None:
+            check(lib().libsmm_acc_mlp_load(ctypes.byref(h), os.fsencode(path)), "libsmm_acc_mlp_load")
+        else:
+            check(lib().libsmm_acc_mlp_create(ctypes.byref(h), hidden, nhidden, seed), "libsmm_acc_mlp_create")
+        self.handle = h
+
+    @property
+    def nparams(self):
+        return lib().libsmm_acc_mlp_params(self.handle, None, 0)
+
+    def params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        lib().libsmm_acc_mlp_params(self.handle, p.ctypes.data, p.size)
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        check(lib().libsmm_acc_mlp_set_params(self.handle, p.ctypes.data, p.size), "libsmm_acc_mlp_set_params")
+
+    @staticmethod
+    def _hw(head_weights):
+        return None if head_weights is None else np.ascontiguousarray(head_weights, dtype=np.float32)
+
+    def gradient(self, x, t, head_weights=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        hw = self._hw(head_weights)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = ctypes.c_double(0)
+        check(lib().libsmm_acc_mlp_gradient(self.handle, x.ctypes.data, t.ctypes.data,
+                                            None if hw is None else hw.ctypes.data, x.shape[0], g.ctypes.data,
+                                            ctypes.byref(loss)), "libsmm_acc_mlp_gradient")
+        return loss.value, g
+
+    def train(self, x, t, epochs, lr=1e-3, weight_decay=0.0, head_weights=None):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        hw = self._hw(head_weights)
+        loss = ctypes.c_double(0)
+        check(lib().libsmm_acc_mlp_train(self.handle, x.ctypes.data, t.ctypes.data,
+                                         None if hw is None else hw.ctypes.data, x.shape[0], epochs, lr, weight_decay,
+                                         ctypes.byref(loss)), "libsmm_acc_mlp_train")
+        return loss.value
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        seg, _ = mlp_heads()
+        out = np.zeros((x.shape[0], int(seg[-1])), dtype=np.float32)
+        check(lib().libsmm_acc_mlp_forward(self.handle, x.ctypes.data, x.shape[0], out.ctypes.data),
+              "libsmm_acc_mlp_forward")
+        return out
+
+    def predict_config(self, tid, m, n, k):
+        cfg = SmmConfig()
+        conf = ctypes.c_double(0)
+        check(lib().libsmm_acc_mlp_predict_config(self.handle, tid, m, n, k, ctypes.byref(cfg), ctypes.byref(conf)),
+              "libsmm_acc_mlp_predict_config")
+        return cfg, conf.value
+
+    def save(self, path):
+        check(lib().libsmm_acc_mlp_save(self.handle, os.fsencode(path)), "libsmm_acc_mlp_save")
+
+    def install(self, min_confidence=0.0):
+        check(lib().libsmm_acc_learned_set(self.handle, min_confidence), "libsmm_acc_learned_set")
+
+    def close(self):
+        if self.handle:
+            lib().libsmm_acc_mlp_destroy(self.handle)
+            self.handle = None
+
+
+def learned_clear():
+    check(lib().libsmm_acc_learned_set(None, 0.0), "libsmm_acc_learned_set")

@@ -72,6 +72,32 @@ int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* dat
   void* hip_stream);
 int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream);
 
+/* Ozaki scheme I (ozaki.hip). Operand element (r, k) at p[r * rs + k * ks] (rs or ks is 1);
+   planes: s x Rp x Kp int8 (Rp multiple of 128, Kp multiple of 64); e: Rp ints; occ: 8 ints */
+int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int s, int Rp, int Kp, int* e,
+  int* occ, signed char* planes, void* hip_stream);
+int lxs_oz_gemm(int dtype, const signed char* pa, const signed char* pb, const int* ea, const int* eb,
+  const int* occa, const int* occb, int s, int cutoff, int flush, int Mp, int Np, int Kp, int M, int N, double alpha,
+  double beta, void* c, int ldc, void* hip_stream);
+int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* hip_stream);
+int lxs_dgemm(const double* a, long ars, long aks, const double* b, long bks, long bcs, int M, int N, int K,
+  double alpha, double beta, double* c, int ldc, void* hip_stream);
+/* complex 3M helpers: split interleaved (rows x cols, ld) into re, im, re + im (ld rows), conj negates im;
+   combine C = alpha (t1 - t2 + i (t3 - t1 - t2)) + beta C */
+int lxs_oz_csplit(int dtype, const void* z, int ld, int rows, int cols, int conj, void* re, void* im, void* sum,
+  void* hip_stream);
+int lxs_oz_ccombine(int dtype, const void* t1, const void* t2, const void* t3, int M, int N, const double* alpha,
+  const double* beta, void* c, int ldc, void* hip_stream);
+
+/* learned predictor MLP (mlp.hip): activations [batch][features], weights [out][in] then bias [out] */
+int lxs_mlp_forward(const float* x, const float* w, int batch, int nin, int nout, int relu, float* y, void* hip_stream);
+int lxs_mlp_loss(const float* z, const float* tgt, const int* seg, const float* hw, int nseg, int batch, int nh,
+  float* dz, float* loss, void* hip_stream);
+int lxs_mlp_backward(const float* x, const float* y, const float* w, float* dz, int batch, int nin, int nout, int relu,
+  float* dw, float* dx, void* hip_stream);
+int lxs_mlp_adam(float* p, const float* g, float* m, float* v, int n, float lr, float beta1, float beta2, float eps,
+  int step, float wd, void* hip_stream);
+
 #if defined(__cplusplus)
 }
 #endif

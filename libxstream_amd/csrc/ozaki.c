@@ -1,0 +1,383 @@
+/*
+ * Ozaki scheme I host layer: context, device scratch, K passes and the complex 3M method.
+ *
+ * Entry points and argument meaning follow samples/ozaki/ozaki_opencl.h:328-440; settings are
+ * resolved as in samples/ozaki/ozaki_opencl.c:348-475 (defaults fp64 8 slices, fp32 4; cutoff
+ * 2(nslices-1) - oztrim). The device work is ozaki.hip: per-operand preprocessing into int8 digit
+ * planes, then one fused level GEMM per K pass.
+ */
+#include "lxs_internal.h"
+#include "../../include/ozaki.h"
+#include <string.h>
+
+#define OZ_ALIGN 256
+#define OZ_TILE_MN 128
+#define OZ_TILE_K 64
+#define OZ_FLUSH_MAX 2080 /* 2080 * 64 * 127^2 < 2^31 */
+
+typedef struct oz_impl_t {
+  void* arena; /* internal scratch, grown on demand */
+  size_t arena_size;
+  void* user; /* caller-owned scratch (ozaki_scratch_set) */
+  size_t user_size;
+  void* carena; /* complex 3M operands and products */
+  size_t carena_size;
+} oz_impl_t;
+
+static size_t oz_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+static int oz_env_int(const char* name, int fallback) {
+  const char* const env = getenv(name);
+  return (NULL != env && '\0' != *env) ? atoi(env) : fallback;
+}
+
+LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_double, int kind, int verbosity, int ndecomp,
+  int ozflags, int oztrim, int ozgroups, int maxk) {
+  oz_impl_t* impl;
+  int nslices;
+  (void)ozgroups;
+  if (NULL == ctx) return EXIT_FAILURE;
+  memset(ctx, 0, sizeof(*ctx));
+  if (0 >= kind || 3 < kind) kind = 3; /* no request: adaptive */
+  if (2 == kind) {
+    lxs_error(EXIT_FAILURE, "ozaki_init: Scheme 2 (CRT) is not available, use kind 1 or 0");
+    return EXIT_FAILURE;
+  }
+  if ((0 != tm && OZ_TILE_MN != tm) || (0 != tn && OZ_TILE_MN != tn)) {
+    lxs_error(EXIT_FAILURE, "ozaki_init: the output tile is 128 x 128");
+    return EXIT_FAILURE;
+  }
+  nslices = (0 < ndecomp ? ndecomp : (0 != use_double ? 8 : 4));
+  if (8 < nslices) nslices = 8; /* 8 digits of 7 bits hold a full FP64 mantissa */
+  if (0 > ozflags) ozflags = OZAKI_TRIANGULAR | OZAKI_SYMMETRIZE;
+  if (1 == ozflags || 2 == ozflags) {
+    lxs_error(EXIT_FAILURE, "ozaki_init: ozflags 1 and 2 (symmetric operands only) are not supported, use 0 or 3");
+    return EXIT_FAILURE;
+  }
+  if (0 > oztrim) oztrim = 0;
+  if (oztrim > 2 * (nslices - 1)) oztrim = 2 * (nslices - 1);
+  impl = (oz_impl_t*)calloc(1, sizeof(oz_impl_t));
+  if (NULL == impl) return EXIT_FAILURE;
+  ctx->use_double = (0 != use_double ? 1 : 0);
+  ctx->kind = 1;
+  ctx->nslices = ctx->ndecomp = nslices;
+  ctx->nprimes = 0;
+  ctx->ozflags = ozflags;
+  ctx->oztrim = oztrim;
+  ctx->verbosity = verbosity;
+  ctx->tm = ctx->tn = OZ_TILE_MN;
+  ctx->maxk = (0 < maxk ? maxk : 0);
+  ctx->flush = oz_env_int("LIBXSTREAM_OZAKI_FLUSH", OZ_FLUSH_MAX);
+  if (1 > ctx->flush || OZ_FLUSH_MAX < ctx->flush) ctx->flush = OZ_FLUSH_MAX;
+  ctx->impl = impl;
+  if (0 != verbosity) {
+    fprintf(stderr, "INFO OZAKI: scheme 1, %s, nslices=%d trim=%d cutoff=%d pairs=%d tile=%dx%d\n",
+      ctx->use_double ? "fp64" : "fp32", nslices, oztrim, 2 * (nslices - 1) - oztrim,
+      ozaki_count_pairs(nslices, 2 * (nslices - 1) - oztrim, 0), OZ_TILE_MN, OZ_TILE_MN);
+  }
+  return EXIT_SUCCESS;
+}
+
+LIBXSTREAM_API void ozaki_destroy(ozaki_context_t* ctx) {
+  if (NULL != ctx && NULL != ctx->impl) {
+    oz_impl_t* const impl = (oz_impl_t*)ctx->impl;
+    if (NULL != impl->arena || NULL != impl->carena) {
+      (void)hipDeviceSynchronize();
+      if (NULL != impl->arena) (void)hipFree(impl->arena);
+      if (NULL != impl->carena) (void)hipFree(impl->carena);
+    }
+    free(impl);
+    ctx->impl = NULL;
+  }
+}
+
+/* rows and columns of a stored operand: op = 'N' keeps (rows, cols) of op(X), 'T'/'C' swaps them */
+static int oz_trans(char t) {
+  switch (t) {
+    case 'N': case 'n': return 0;
+    case 'T': case 't': return 1;
+    case 'C': case 'c': return 2;
+    default: return -1;
+  }
+}
+
+typedef struct oz_layout_t {
+  int Mp, Np, Kp, kpass;
+  size_t planes_a, planes_b, ea, eb, occ, a, b, c, total; /* byte offsets and total */
+} oz_layout_t;
+
+static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
+  int dev, oz_layout_t* l) {
+  const size_t ts = (0 != ctx->use_double ? 8 : 4);
+  const int kpass = (0 < ctx->maxk && ctx->maxk < K) ? ctx->maxk : K;
+  size_t off = 0;
+  l->Mp = (int)oz_up((size_t)(0 < M ? M : 1), OZ_TILE_MN);
+  l->Np = (int)oz_up((size_t)(0 < N ? N : 1), OZ_TILE_MN);
+  l->Kp = (int)oz_up((size_t)(0 < kpass ? kpass : 1), OZ_TILE_K);
+  l->kpass = kpass;
+  l->planes_a = off;
+  off += oz_up((size_t)ctx->nslices * l->Mp * l->Kp, OZ_ALIGN);
+  l->planes_b = off;
+  off += oz_up((size_t)ctx->nslices * l->Np * l->Kp, OZ_ALIGN);
+  l->ea = off;
+  off += oz_up(sizeof(int) * (size_t)l->Mp, OZ_ALIGN);
+  l->eb = off;
+  off += oz_up(sizeof(int) * (size_t)l->Np, OZ_ALIGN);
+  l->occ = off;
+  off += OZ_ALIGN; /* 2 x 8 ints */
+  l->a = l->b = l->c = 0;
+  if (0 == dev) {
+    l->a = off;
+    off += oz_up(ts * (size_t)lda * (size_t)(0 == ta ? K : M), OZ_ALIGN);
+    l->b = off;
+    off += oz_up(ts * (size_t)ldb * (size_t)(0 == tb ? N : K), OZ_ALIGN);
+    l->c = off;
+    off += oz_up(ts * (size_t)ldc * (size_t)N, OZ_ALIGN);
+  }
+  l->total = off;
+}
+
+LIBXSTREAM_API size_t ozaki_scratch_size(const ozaki_context_t* ctx, char transa, char transb, int M, int N, int K,
+  int lda, int ldb, int ldc) {
+  oz_layout_t l;
+  const int ta = oz_trans(transa), tb = oz_trans(transb);
+  if (NULL == ctx || 0 > ta || 0 > tb || 0 > M || 0 > N || 0 > K) return 0;
+  oz_layout(ctx, ta, tb, M, N, K, lda, ldb, ldc, 0 /*host operands: upper bound*/, &l);
+  return l.total;
+}
+
+LIBXSTREAM_API int ozaki_scratch_set(ozaki_context_t* ctx, void* dev_mem, size_t nbytes) {
+  oz_impl_t* impl;
+  if (NULL == ctx || NULL == ctx->impl) return EXIT_FAILURE;
+  impl = (oz_impl_t*)ctx->impl;
+  impl->user = dev_mem;
+  impl->user_size = (NULL != dev_mem ? nbytes : 0);
+  if (NULL == dev_mem && NULL != impl->arena) { /* revert to the internal arena: release it */
+    (void)hipDeviceSynchronize();
+    (void)hipFree(impl->arena);
+    impl->arena = NULL;
+    impl->arena_size = 0;
+  }
+  return EXIT_SUCCESS;
+}
+
+LIBXSTREAM_API void* ozaki_scratch_get(const ozaki_context_t* ctx, size_t* nbytes) {
+  const oz_impl_t* impl = (NULL != ctx ? (const oz_impl_t*)ctx->impl : NULL);
+  void* ptr = NULL;
+  size_t size = 0;
+  if (NULL != impl) {
+    if (NULL != impl->user) {
+      ptr = impl->user;
+      size = impl->user_size;
+    }
+    else {
+      ptr = impl->arena;
+      size = impl->arena_size;
+    }
+  }
+  if (NULL != nbytes) *nbytes = size;
+  return ptr;
+}
+
+LIBXSTREAM_API ozaki_tile_t ozaki_tile_select(const ozaki_context_t* ctx, int M, int N, int rtm, int rtn) {
+  ozaki_tile_t tile;
+  (void)ctx, (void)M, (void)N, (void)rtm, (void)rtn;
+  tile.m = tile.n = OZ_TILE_MN;
+  return tile;
+}
+
+LIBXSTREAM_API int ozaki_npanel(const ozaki_context_t* ctx, int M, int N, int tm, int tn) {
+  (void)ctx, (void)M, (void)tm, (void)tn;
+  return N;
+}
+
+LIBXSTREAM_API void ozaki_invalidate_cache(ozaki_context_t* ctx, const void* a, const void* b) {
+  (void)ctx, (void)a, (void)b;
+}
+
+/* device memory of at least nbytes: the caller's scratch when large enough, else the internal arena */
+static char* oz_scratch(oz_impl_t* impl, size_t nbytes, hipStream_t hs) {
+  if (NULL != impl->user && impl->user_size >= nbytes) return (char*)impl->user;
+  if (impl->arena_size < nbytes) {
+    if (NULL != impl->arena) {
+      if (0 != lxs_hip(hipStreamSynchronize(hs), "ozaki_gemm: stream synchronize")) return NULL;
+      (void)hipFree(impl->arena);
+      impl->arena = NULL;
+      impl->arena_size = 0;
+    }
+    if (0 != lxs_hip(hipMalloc(&impl->arena, nbytes), "ozaki_gemm: scratch allocation")) {
+      impl->arena = NULL;
+      return NULL;
+    }
+    impl->arena_size = nbytes;
+  }
+  return (char*)impl->arena;
+}
+
+static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, int M, int N, int K, double alpha,
+  const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc, int dev) {
+  oz_impl_t* const impl = (oz_impl_t*)ctx->impl;
+  const int dtype = (0 != ctx->use_double ? 3 : 1);
+  const size_t ts = (0 != ctx->use_double ? 8 : 4);
+  const int arows = (0 == ta ? M : K), acols = (0 == ta ? K : M);
+  const int brows = (0 == tb ? K : N), bcols = (0 == tb ? N : K);
+  const int s = ctx->nslices, cutoff = 2 * (ctx->nslices - 1) - ctx->oztrim;
+  oz_layout_t l;
+  char* base;
+  const char *da = (const char*)a, *db = (const char*)b;
+  char* dc = (char*)c;
+  int result = 0, k0;
+  if (lda < (1 > arows ? 1 : arows) || ldb < (1 > brows ? 1 : brows) || ldc < (1 > M ? 1 : M)) {
+    return lxs_error(EXIT_FAILURE, "ozaki_gemm: leading dimension too small");
+  }
+  if (0 == M || 0 == N) return EXIT_SUCCESS;
+  if ((0 != K && (NULL == a || NULL == b)) || NULL == c) return lxs_error(EXIT_FAILURE, "ozaki_gemm: NULL operand");
+  oz_layout(ctx, ta, tb, M, N, K, lda, ldb, ldc, dev, &l);
+  base = oz_scratch(impl, l.total, hs);
+  if (NULL == base) return EXIT_FAILURE;
+  if (0 == dev) { /* host operands: copy in */
+    da = base + l.a;
+    db = base + l.b;
+    dc = base + l.c;
+    if (0 != K) {
+      result |= lxs_hip(hipMemcpy2DAsync(base + l.a, ts * lda, a, ts * lda, ts * arows, acols, hipMemcpyHostToDevice, hs),
+        "ozaki_gemm: copy A");
+      result |= lxs_hip(hipMemcpy2DAsync(base + l.b, ts * ldb, b, ts * ldb, ts * brows, bcols, hipMemcpyHostToDevice, hs),
+        "ozaki_gemm: copy B");
+    }
+    if (0 != beta) {
+      result |= lxs_hip(hipMemcpy2DAsync(dc, ts * ldc, c, ts * ldc, ts * M, N, hipMemcpyHostToDevice, hs), "ozaki_gemm: copy C");
+    }
+    if (0 != result) return EXIT_FAILURE;
+  }
+  if (0 == K || 0 == alpha) {
+    result = lxs_oz_scale(dtype, dc, ldc, M, N, beta, hs);
+  }
+  else {
+    /* op(A)(i, k) at da[i * ars + k * aks]; op(B)(k, j) at db[k * bks + j * bcs] */
+    const long ars = (0 == ta ? 1 : lda), aks = (0 == ta ? lda : 1);
+    const long bks = (0 == tb ? 1 : ldb), bcs = (0 == tb ? ldb : 1);
+    int* const occ = (int*)(base + l.occ);
+    const double npairs = ozaki_count_pairs(s, cutoff, 0);
+    for (k0 = 0; k0 < K && 0 == result; k0 += l.kpass) {
+      const int kk = (K - k0 < l.kpass ? K - k0 : l.kpass);
+      const int Kp = (int)oz_up((size_t)kk, OZ_TILE_K);
+      void* token = lxs_profile_begin(hs);
+      result = lxs_oz_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, s, l.Mp, Kp, (int*)(base + l.ea), occ,
+        (signed char*)(base + l.planes_a), hs);
+      if (0 == result) {
+        result = lxs_oz_preprocess(dtype, db + ts * (size_t)k0 * bks, bcs, bks, N, kk, s, l.Np, Kp, (int*)(base + l.eb),
+          occ + 8, (signed char*)(base + l.planes_b), hs);
+      }
+      lxs_profile_end(token, hs, "ozaki_pre", 0.0, (double)ts * ((double)M + N) * kk + (double)s * (l.Mp + l.Np) * Kp);
+      if (0 == result) { /* int8 work of every pair up to the cutoff (pairs of all-zero planes are skipped) */
+        token = lxs_profile_begin(hs);
+        result = lxs_oz_gemm(dtype, (const signed char*)(base + l.planes_a), (const signed char*)(base + l.planes_b),
+          (const int*)(base + l.ea), (const int*)(base + l.eb), occ, occ + 8, s, cutoff, ctx->flush, l.Mp, l.Np, Kp, M, N,
+          alpha, (0 == k0 ? beta : 1.0), dc, ldc, hs);
+        lxs_profile_end(token, hs, "ozaki_i8", 2.0 * npairs * l.Mp * l.Np * Kp,
+          (double)s * (l.Mp + l.Np) * Kp + (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2));
+      }
+    }
+  }
+  if (0 != result) return lxs_error(EXIT_FAILURE, "ozaki_gemm: kernel launch failed");
+  if (0 == dev) {
+    result = lxs_hip(hipMemcpy2DAsync(c, ts * ldc, dc, ts * ldc, ts * M, N, hipMemcpyDeviceToHost, hs), "ozaki_gemm: copy C back");
+  }
+  return (0 == result ? EXIT_SUCCESS : EXIT_FAILURE);
+}
+
+LIBXSTREAM_API int ozaki_gemm(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M, int N,
+  int K, double alpha, const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc, int dev) {
+  const int ta = oz_trans(transa), tb = oz_trans(transb);
+  hipStream_t hs;
+  void* token;
+  int result;
+  if (NULL == ctx || NULL == ctx->impl) return lxs_error(EXIT_FAILURE, "ozaki_gemm: context not initialised");
+  if (0 > ta || 0 > tb || 0 > M || 0 > N || 0 > K) return lxs_error(EXIT_FAILURE, "ozaki_gemm: invalid argument");
+  if (0 != lxs_ensure_device()) return EXIT_FAILURE;
+  hs = lxs_stream_hip(stream);
+  token = lxs_profile_begin(hs);
+  result = oz_gemm_real(ctx, hs, ta, tb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, dev);
+  lxs_profile_end(token, hs, "ozaki", 2.0 * M * N * K, (double)(0 != ctx->use_double ? 8 : 4) * ((double)M * K + (double)K * N + 2.0 * M * N));
+  return result;
+}
+
+LIBXSTREAM_API int ozaki_gemm_complex(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M,
+  int N, int K, const double* alpha, const void* a, int lda, const void* b, int ldb, const double* beta, void* c, int ldc) {
+  const int ta = oz_trans(transa), tb = oz_trans(transb);
+  const int dtype = (NULL != ctx && 0 != ctx->use_double ? 3 : 1);
+  const size_t ts = (3 == dtype ? 8 : 4);
+  oz_impl_t* impl;
+  hipStream_t hs;
+  int arows, acols, brows, bcols, result = 0;
+  size_t na, nb, nc, need;
+  char* base;
+  if (NULL == ctx || NULL == ctx->impl || NULL == alpha || NULL == beta) {
+    return lxs_error(EXIT_FAILURE, "ozaki_gemm_complex: invalid argument");
+  }
+  if (0 > ta || 0 > tb || 0 > M || 0 > N || 0 > K) return lxs_error(EXIT_FAILURE, "ozaki_gemm_complex: invalid argument");
+  arows = (0 == ta ? M : K), acols = (0 == ta ? K : M);
+  brows = (0 == tb ? K : N), bcols = (0 == tb ? N : K);
+  if (lda < (1 > arows ? 1 : arows) || ldb < (1 > brows ? 1 : brows) || ldc < (1 > M ? 1 : M)) {
+    return lxs_error(EXIT_FAILURE, "ozaki_gemm_complex: leading dimension too small");
+  }
+  if (0 == M || 0 == N) return EXIT_SUCCESS;
+  if (0 != lxs_ensure_device()) return EXIT_FAILURE;
+  impl = (oz_impl_t*)ctx->impl;
+  hs = lxs_stream_hip(stream);
+  na = oz_up(ts * (size_t)arows * acols, OZ_ALIGN);
+  nb = oz_up(ts * (size_t)brows * bcols, OZ_ALIGN);
+  nc = oz_up(ts * (size_t)M * N, OZ_ALIGN);
+  need = 3 * (na + nb + nc);
+  if (impl->carena_size < need) {
+    if (NULL != impl->carena) {
+      if (0 != lxs_hip(hipStreamSynchronize(hs), "ozaki_gemm_complex: stream synchronize")) return EXIT_FAILURE;
+      (void)hipFree(impl->carena);
+      impl->carena = NULL;
+      impl->carena_size = 0;
+    }
+    if (0 != lxs_hip(hipMalloc(&impl->carena, need), "ozaki_gemm_complex: scratch allocation")) return EXIT_FAILURE;
+    impl->carena_size = need;
+  }
+  base = (char*)impl->carena;
+  {
+    char *const ar = base, *const ai = base + na, *const as = base + 2 * na;
+    char *const br = base + 3 * na, *const bi = br + nb, *const bs = br + 2 * nb;
+    char *const t1 = base + 3 * (na + nb), *const t2 = t1 + nc, *const t3 = t1 + 2 * nc;
+    const char tra = (0 == ta ? 'N' : 'T'), trb = (0 == tb ? 'N' : 'T');
+    if (0 < K) {
+      result |= lxs_oz_csplit(dtype, a, lda, arows, acols, 2 == ta, ar, ai, as, hs);
+      result |= lxs_oz_csplit(dtype, b, ldb, brows, bcols, 2 == tb, br, bi, bs, hs);
+    }
+    if (0 != result) return lxs_error(EXIT_FAILURE, "ozaki_gemm_complex: kernel launch failed");
+    if (EXIT_SUCCESS != ozaki_gemm(ctx, stream, tra, trb, M, N, K, 1.0, ar, arows, br, brows, 0.0, t1, M, 1) ||
+        EXIT_SUCCESS != ozaki_gemm(ctx, stream, tra, trb, M, N, K, 1.0, ai, arows, bi, brows, 0.0, t2, M, 1) ||
+        EXIT_SUCCESS != ozaki_gemm(ctx, stream, tra, trb, M, N, K, 1.0, as, arows, bs, brows, 0.0, t3, M, 1)) {
+      return EXIT_FAILURE;
+    }
+    result = lxs_oz_ccombine(dtype, t1, t2, t3, M, N, alpha, beta, c, ldc, hs);
+  }
+  return (0 == result ? EXIT_SUCCESS : lxs_error(EXIT_FAILURE, "ozaki_gemm_complex: kernel launch failed"));
+}
+
+LIBXSTREAM_API int ozaki_dgemm_native(libxstream_stream_t* stream, char transa, char transb, int M, int N, int K,
+  double alpha, const double* a, int lda, const double* b, int ldb, double beta, double* c, int ldc) {
+  const int ta = oz_trans(transa), tb = oz_trans(transb);
+  hipStream_t hs;
+  void* token;
+  int result;
+  if (0 > ta || 0 > tb || 0 > M || 0 > N || 0 > K) return lxs_error(EXIT_FAILURE, "ozaki_dgemm_native: invalid argument");
+  if (lda < (1 > (0 == ta ? M : K) ? 1 : (0 == ta ? M : K)) || ldb < (1 > (0 == tb ? K : N) ? 1 : (0 == tb ? K : N)) ||
+      ldc < (1 > M ? 1 : M)) {
+    return lxs_error(EXIT_FAILURE, "ozaki_dgemm_native: leading dimension too small");
+  }
+  if (0 == M || 0 == N) return EXIT_SUCCESS;
+  if (0 != lxs_ensure_device()) return EXIT_FAILURE;
+  hs = lxs_stream_hip(stream);
+  token = lxs_profile_begin(hs);
+  result = lxs_dgemm(a, 0 == ta ? 1 : lda, 0 == ta ? lda : 1, b, 0 == tb ? 1 : ldb, 0 == tb ? ldb : 1, M, N, K, alpha,
+    beta, c, ldc, hs);
+  lxs_profile_end(token, hs, "dgemm", 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
+  return (0 == result ? EXIT_SUCCESS : lxs_error(EXIT_FAILURE, "ozaki_dgemm_native: kernel launch failed"));
+}

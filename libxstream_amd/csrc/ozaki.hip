@@ -1,0 +1,517 @@
+/*
+ * FP64 (and FP32) GEMM emulated with int8 matrix cores (Ozaki scheme I, mantissa slicing), and a
+ * native FP64 matrix-core GEMM as its comparator. gfx950.
+ *
+ * Mathematics (the textbook scheme that samples/ozaki/kernels/ozaki1_int8.cl:432-589 and
+ * ozaki_common.cl:789-825 also implement): every row i of op(A) and column j of op(B) is scaled by
+ * a power of two 2^ea_i (2^eb_j) above its largest magnitude, so |x| < 1, and cut into s signed
+ * 7-bit digits by truncation,
+ *   x = sum_{p=0..s-1} d_p 2^(-7(p+1)) + tail,  d_p in [-127, 127],  |tail| < 2^(-7s).
+ * Then (AB)(i,j) ~ 2^(ea_i + eb_j) sum_{L=0..cutoff} 2^(-7(L+2)) sum_{p+q=L} (A_p B_q)(i,j), with
+ * A_p, B_q int8 digit planes. Every inner sum is an int8 GEMM accumulated exactly in int32.
+ * cutoff = 2(s-1) - trim: trim 0 takes all s^2 slice pairs, trim s-1 the triangle p+q <= s-1.
+ * Zero, subnormal and non-finite inputs count as zero (ozaki_common.cl:784-805).
+ *
+ * Kernels (new design for MI355X):
+ *   oz_init     exponents := none, occupancy := 0
+ *   oz_exp      per-row (A) / per-column (B) exponent: max over the row of (biased exponent - bias + 1)
+ *   oz_slice    digits of 64x64 tiles into s planes [s][Rp][Kp] (k contiguous, rows padded to 128,
+ *               k to 64, padding zero), so GEMM operand tiles are plain 16-byte vectors; sets one
+ *               occupancy flag per plane that holds a nonzero digit
+ *   oz_gemm     128x128 output tile per work-group (4 wavefronts of 64x64, v_mfma_i32_32x32x32_i8).
+ *               Levels L = cutoff .. 0 (smallest weight first). The pairs of a level with both planes
+ *               occupied accumulate in int32 over K, folded into FP64 registers as cd += 2^(-7(L+2)) acc
+ *               at the level's end, or after every `flush` k-blocks of 64 (exactness bound
+ *               flush * 64 * 127^2 < 2^31). Epilogue C = fma(alpha, 2^(ea+eb) cd, beta C).
+ *               Operand tiles are staged through LDS, double-buffered.
+ *   dgemm_f64   128x128 tiles with v_mfma_f64_16x16x4f64 on FP64 operands (the comparator).
+ */
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "lxs_kernels.h"
+
+namespace {
+
+typedef int i4_t __attribute__((ext_vector_type(4)));
+typedef int i16_t __attribute__((ext_vector_type(16)));
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+constexpr int OZ_TILE = 64; /* rows x k of a preprocessing tile */
+constexpr int OZ_EXP_NONE = -(1 << 30);
+
+/* exponent ex with |v| < 2^ex for a normal finite v, OZ_EXP_NONE otherwise */
+__device__ __forceinline__ int oz_expo(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int be = (int)((bits >> 52) & 0x7FF);
+  return (0 == be || 0x7FF == be) ? OZ_EXP_NONE : be - 1022;
+}
+__device__ __forceinline__ int oz_expo(float v) {
+  const int bits = __float_as_int(v);
+  const int be = (bits >> 23) & 0xFF;
+  return (0 == be || 0xFF == be) ? OZ_EXP_NONE : be - 126;
+}
+
+/* load a 64x64 tile (rows r0.., k k0..) into LDS t[r][k] as double, coalesced along the contiguous
+   dimension; elements outside R x K, zero, subnormal or non-finite become 0 */
+template <typename T>
+__device__ __forceinline__ void oz_load_tile(double (*t)[OZ_TILE + 1], const T* __restrict__ p, long rs, long ks,
+  int r0, int k0, int R, int K) {
+  const int tid = threadIdx.x;
+  if (1 == ks) { /* k contiguous: 16 threads per row, 4 consecutive k each; 16 rows per pass */
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+      const int r = pass * 16 + (tid >> 4), kq = (tid & 15) * 4;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int gr = r0 + r, gk = k0 + kq + x;
+        const T v = (gr < R && gk < K) ? p[(long)gr * rs + gk] : T(0);
+        t[r][kq + x] = (OZ_EXP_NONE != oz_expo(v)) ? (double)v : 0.0;
+      }
+    }
+  }
+  else { /* rows contiguous: 16 threads per k, 4 consecutive rows each; 16 k per pass */
+#pragma unroll
+    for (int pass = 0; pass < 4; ++pass) {
+      const int k = pass * 16 + (tid >> 4), rq = (tid & 15) * 4;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int gr = r0 + rq + x, gk = k0 + k;
+        const T v = (gr < R && gk < K) ? p[gr + (long)gk * ks] : T(0);
+        t[rq + x][k] = (OZ_EXP_NONE != oz_expo(v)) ? (double)v : 0.0;
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) oz_init(int* __restrict__ e, int n, int* __restrict__ occ) {
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) e[i] = OZ_EXP_NONE;
+  if (0 == blockIdx.x && (int)threadIdx.x < 8) occ[threadIdx.x] = 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) oz_exp(const T* __restrict__ p, long rs, long ks, int R, int K,
+  int* __restrict__ e) {
+  __shared__ double t[OZ_TILE][OZ_TILE + 1];
+  const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
+  oz_load_tile<T>(t, p, rs, ks, r0, k0, R, K);
+  __syncthreads();
+  { /* 4 threads per row, 16 k each, then a 4-lane maximum */
+    const int r = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
+    int ex = OZ_EXP_NONE;
+#pragma unroll
+    for (int x = 0; x < 16; ++x) ex = max(ex, oz_expo(t[r][kq + x]));
+    ex = max(ex, __shfl_xor(ex, 1));
+    ex = max(ex, __shfl_xor(ex, 2));
+    if (0 == (threadIdx.x & 3) && OZ_EXP_NONE != ex && r0 + r < R) atomicMax(e + r0 + r, ex);
+  }
+}
+
+/* s int8 digits of 16 consecutive k of one row, written as one 16-byte vector per plane */
+template <typename T>
+__global__ void __launch_bounds__(256) oz_slice(const T* __restrict__ p, long rs, long ks, int R, int K,
+  const int* __restrict__ e, int s, int Rp, int Kp, int8_t* __restrict__ planes, int* __restrict__ occ) {
+  __shared__ double t[OZ_TILE][OZ_TILE + 1];
+  const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
+  const int tid = threadIdx.x, r = tid >> 2, kq = (tid & 3) * 16;
+  oz_load_tile<T>(t, p, rs, ks, r0, k0, R, K);
+  __syncthreads();
+  const int row = r0 + r;
+  const int ex = (row < R ? e[row] : OZ_EXP_NONE);
+  double x[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) x[i] = (OZ_EXP_NONE != ex ? ldexp(t[r][kq + i], -ex) : 0.0); /* |x| < 1, exact */
+  for (int pl = 0; pl < s; ++pl) {
+    union {
+      int8_t b[16];
+      i4_t v;
+    } d;
+    int nz = 0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const double y = x[i] * 128.0;
+      const double q = trunc(y); /* |q| <= 127 */
+      d.b[i] = (int8_t)q;
+      nz |= (int)(0.0 != q);
+      x[i] = y - q; /* exact */
+    }
+    *(i4_t*)(planes + ((long)pl * Rp + row) * Kp + k0 + kq) = d.v;
+    if (0 != __ballot(nz) && 0 == (tid & 63) && 0 == __hip_atomic_load(occ + pl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+      atomicOr(occ + pl, 1);
+    }
+  }
+}
+
+/* element-wise kernels stride over the M x N elements with a 1-D grid */
+constexpr int EW_BLOCKS = 1024;
+
+template <typename T>
+__global__ void __launch_bounds__(256) oz_scale_c(T* __restrict__ c, int ldc, int M, int N, double beta) {
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < (long)M * N; x += (long)gridDim.x * 256) {
+    T* const pc = c + (x % M) + (x / M) * (long)ldc;
+    *pc = (0 != beta ? (T)(beta * (double)*pc) : T(0));
+  }
+}
+
+/* ---- int8 level GEMM ------------------------------------------------------------------------- */
+
+constexpr int GT = 128;      /* output tile (rows and columns) per work-group */
+constexpr int GK = 64;       /* k per LDS stage ("k-block") */
+constexpr int GLD = GK + 16; /* LDS row stride in bytes (16-byte reads of 8 consecutive rows hit distinct banks) */
+
+/* v_mfma_i32_32x32x32_i8 lane layout: A operand lane l holds row l%32, k = 16(l/32) + byte; B operand
+   lane l holds column l%32, same k; D lane l register r holds column l%32, row 8(r/4) + 4(l/32) + r%4 */
+__device__ __forceinline__ int oz_drow(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+
+template <typename T>
+__global__ void __launch_bounds__(256) oz_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb,
+  const int* __restrict__ ea, const int* __restrict__ eb, const int* __restrict__ occa, const int* __restrict__ occb,
+  int s, int cutoff, int flush, int Mp, int Np, int Kp, int M, int N, double alpha, double beta, T* __restrict__ c,
+  int ldc) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][2][GT * GLD]; /* [buffer][A|B][128 rows x GLD] */
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = wave >> 1, wc = wave & 1; /* 64x64 quadrant of the wavefront */
+  const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
+  const int nkb = Kp / GK;
+  /* staging: thread t copies 32 bytes of row t/2 of both tiles */
+  const int srow = tid >> 1, shalf = (tid & 1) * 32;
+  unsigned int oa = 0, ob = 0; /* occupied planes */
+  for (int p = 0; p < s; ++p) {
+    oa |= (0 != occa[p] ? 1u : 0u) << p;
+    ob |= (0 != occb[p] ? 1u : 0u) << p;
+  }
+  oa = __builtin_amdgcn_readfirstlane(oa);
+  ob = __builtin_amdgcn_readfirstlane(ob);
+  d4_t cd[2][2][4];
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) cd[a][b][q] = d4_t{0, 0, 0, 0};
+    }
+  }
+  i16_t acc[2][2];
+  i4_t ra[2], rb[2];
+  for (int L = cutoff; L >= 0; --L) {
+    /* pairs (p, L - p) of this level with both planes occupied, as a bit mask over p */
+    unsigned int mask = 0;
+    for (int p = (L - s + 1 > 0 ? L - s + 1 : 0); p <= (L < s - 1 ? L : s - 1); ++p) {
+      if ((oa >> p) & (ob >> (L - p)) & 1u) mask |= 1u << p;
+    }
+    if (0 == mask) continue;
+    const double scale = ldexp(1.0, -7 * (L + 2));
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+#pragma unroll
+      for (int b = 0; b < 2; ++b) acc[a][b] = i16_t{0};
+    }
+    /* k-blocks in order: pairs by increasing p, k-blocks by increasing k */
+    int fp = __builtin_ctz(mask), fkb = 0; /* next k-block to fetch */
+    bool more = true;
+#define OZ_FETCH() \
+  do { \
+    const int8_t* const sa_ = pa + ((long)fp * Mp + row0 + srow) * Kp + fkb * GK + shalf; \
+    const int8_t* const sb_ = pb + ((long)(L - fp) * Np + col0 + srow) * Kp + fkb * GK + shalf; \
+    ra[0] = *(const i4_t*)sa_; \
+    ra[1] = *(const i4_t*)(sa_ + 16); \
+    rb[0] = *(const i4_t*)sb_; \
+    rb[1] = *(const i4_t*)(sb_ + 16); \
+    if (++fkb == nkb) { \
+      const unsigned int rest_ = mask & ~((2u << fp) - 1u); \
+      fkb = 0; \
+      more = (0 != rest_); \
+      fp = (more ? __builtin_ctz(rest_) : fp); \
+    } \
+  } while (0)
+#define OZ_STORE(BUF) \
+  do { \
+    *(i4_t*)(&lds[BUF][0][srow * GLD + shalf]) = ra[0]; \
+    *(i4_t*)(&lds[BUF][0][srow * GLD + shalf + 16]) = ra[1]; \
+    *(i4_t*)(&lds[BUF][1][srow * GLD + shalf]) = rb[0]; \
+    *(i4_t*)(&lds[BUF][1][srow * GLD + shalf + 16]) = rb[1]; \
+  } while (0)
+    OZ_FETCH();
+    __syncthreads(); /* the previous level's last reads of the LDS buffers are done */
+    OZ_STORE(0);
+    __syncthreads();
+    int buf = 0, cnt = 0;
+    for (;;) {
+      const bool last = !more;
+      if (!last) OZ_FETCH();
+#pragma unroll
+      for (int kk = 0; kk < GK; kk += 32) {
+        i4_t fa[2], fb[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          fa[a] = *(const i4_t*)(&lds[buf][0][(64 * wr + 32 * a + (lane & 31)) * GLD + kk + 16 * (lane >> 5)]);
+          fb[a] = *(const i4_t*)(&lds[buf][1][(64 * wc + 32 * a + (lane & 31)) * GLD + kk + 16 * (lane >> 5)]);
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+          for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
+        }
+      }
+      if (++cnt == flush || last) { /* fold the exact int32 partial sums into FP64 */
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+#pragma unroll
+            for (int r = 0; r < 16; ++r) cd[a][b][r >> 2][r & 3] += scale * (double)acc[a][b][r];
+            acc[a][b] = i16_t{0};
+          }
+        }
+        cnt = 0;
+      }
+      if (last) break;
+      OZ_STORE(buf ^ 1);
+      __syncthreads();
+      buf ^= 1;
+    }
+#undef OZ_FETCH
+#undef OZ_STORE
+  }
+  /* epilogue: C = fma(alpha, 2^(ea+eb) cd, beta C) */
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = col0 + 64 * wc + 32 * b + (lane & 31);
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int i = row0 + 64 * wr + 32 * a + oz_drow(lane, r);
+        if (i < M && j < N) {
+          const int ex = ea[i], ey = eb[j];
+          const double v = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(cd[a][b][r >> 2][r & 3], ex + ey) : 0.0;
+          T* const pc = c + i + (long)j * ldc;
+          *pc = (T)(0 != beta ? fma(alpha, v, beta * (double)*pc) : alpha * v);
+        }
+      }
+    }
+  }
+}
+
+/* ---- native FP64 GEMM (comparator) ----------------------------------------------------------- */
+
+constexpr int DK = 16; /* k per LDS stage */
+
+/* v_mfma_f64_16x16x4f64: A lane l -> A[l%16][l/16], B lane l -> B[l/16][l%16], D lane l reg r -> (l/16 + 4r, l%16) */
+__global__ void __launch_bounds__(256) dgemm_f64(const double* __restrict__ a, long ars, long aks,
+  const double* __restrict__ b, long bks, long bcs, int M, int N, int K, double alpha, double beta,
+  double* __restrict__ c, int ldc) {
+  __shared__ double la[2][DK][GT], lb[2][DK][GT]; /* [buffer][k][row / column] */
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wr = wave >> 1, wc = wave & 1;
+  const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
+  const int nkb = (K + DK - 1) / DK;
+  d4_t acc[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+#pragma unroll
+    for (int y = 0; y < 4; ++y) acc[x][y] = d4_t{0, 0, 0, 0};
+  }
+  /* staging: 128 x 16 elements per operand, 8 per thread; element e = tid + 256 x */
+  double ra[8], rb[8];
+  auto fetch = [&](int kb) {
+    const int k0 = kb * DK;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      const int e = tid + 256 * x;
+      const int r = (1 == ars ? (e & 127) : (e >> 4)), k = (1 == ars ? (e >> 7) : (e & 15));
+      const int gi = row0 + r, gk = k0 + k;
+      ra[x] = (gi < M && gk < K) ? a[(long)gi * ars + (long)gk * aks] : 0.0;
+      const int cc = (1 == bcs ? (e & 127) : (e >> 4)), kc = (1 == bcs ? (e >> 7) : (e & 15));
+      const int gj = col0 + cc, gk2 = k0 + kc;
+      rb[x] = (gj < N && gk2 < K) ? b[(long)gk2 * bks + (long)gj * bcs] : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      const int e = tid + 256 * x;
+      if (1 == ars) la[buf][e >> 7][e & 127] = ra[x];
+      else la[buf][e & 15][e >> 4] = ra[x];
+      if (1 == bcs) lb[buf][e >> 7][e & 127] = rb[x];
+      else lb[buf][e & 15][e >> 4] = rb[x];
+    }
+  };
+  fetch(0);
+  store(0);
+  __syncthreads();
+  for (int kb = 0; kb < nkb; ++kb) {
+    const int buf = kb & 1;
+    if (kb + 1 < nkb) fetch(kb + 1);
+#pragma unroll
+    for (int ks = 0; ks < DK; ks += 4) {
+      double fa[4], fb[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        fa[x] = la[buf][ks + (lane >> 4)][64 * wr + 16 * x + (lane & 15)];
+        fb[x] = lb[buf][ks + (lane >> 4)][64 * wc + 16 * x + (lane & 15)];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[x], fb[y], acc[x][y], 0, 0, 0);
+      }
+    }
+    if (kb + 1 < nkb) store(buf ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int j = col0 + 64 * wc + 16 * y + (lane & 15);
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int i = row0 + 64 * wr + 16 * x + (lane >> 4) + 4 * r;
+        if (i < M && j < N) {
+          double* const pc = c + i + (long)j * ldc;
+          *pc = (0 != beta ? fma(alpha, acc[x][y][r], beta * *pc) : alpha * acc[x][y][r]);
+        }
+      }
+    }
+  }
+}
+
+/* ---- complex 3M helpers ---------------------------------------------------------------------- */
+
+template <typename T>
+__global__ void __launch_bounds__(256) oz_csplit(const T* __restrict__ z, int ld, int rows, int cols, int conj, T* __restrict__ re,
+  T* __restrict__ im, T* __restrict__ sum) {
+  for (long o = blockIdx.x * 256L + threadIdx.x; o < (long)rows * cols; o += (long)gridDim.x * 256) {
+    const long zi = 2 * ((o % rows) + (o / rows) * (long)ld);
+    const T x = z[zi], y = (0 != conj ? -z[zi + 1] : z[zi + 1]);
+    re[o] = x;
+    im[o] = y;
+    sum[o] = x + y;
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) oz_ccombine(const T* __restrict__ t1, const T* __restrict__ t2,
+  const T* __restrict__ t3, int M, int N, double ar, double ai, double br, double bi, T* __restrict__ c, int ldc) {
+  for (long o = blockIdx.x * 256L + threadIdx.x; o < (long)M * N; o += (long)gridDim.x * 256) {
+    const long i = o % M, j = o / M;
+    const double pr = (double)t1[o] - (double)t2[o], pi = (double)t3[o] - (double)t1[o] - (double)t2[o];
+    T* const pc = c + 2 * (i + j * ldc);
+    double cr = ar * pr - ai * pi, ci = ar * pi + ai * pr;
+    if (0 != br || 0 != bi) {
+      const double xr = (double)pc[0], xi = (double)pc[1];
+      cr += br * xr - bi * xi;
+      ci += br * xi + bi * xr;
+    }
+    pc[0] = (T)cr;
+    pc[1] = (T)ci;
+  }
+}
+
+} /* namespace */
+
+/* digit planes, exponents and occupancy of one operand: element (r, k) at p[r * rs + k * ks] */
+extern "C" int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int s, int Rp, int Kp,
+  int* e, int* occ, signed char* planes, void* hip_stream) {
+  const hipStream_t st = (hipStream_t)hip_stream;
+  const dim3 grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
+  if (0 != Rp % GT || 0 != Kp % GK || Rp < R || Kp < K || 0 >= Kp || 1 > s || 8 < s) return LXS_ELIMIT;
+  if (!((1 == rs) || (1 == ks))) return LXS_ELIMIT;
+  oz_init<<<(Rp + 255) / 256, 256, 0, st>>>(e, Rp, occ);
+  if (3 == dtype) {
+    oz_exp<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
+    oz_slice<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e, s, Rp, Kp, planes, occ);
+  }
+  else if (1 == dtype) {
+    oz_exp<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
+    oz_slice<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e, s, Rp, Kp, planes, occ);
+  }
+  else {
+    return LXS_ENOVARIANT;
+  }
+  return (int)hipGetLastError();
+}
+
+extern "C" int lxs_oz_gemm(int dtype, const signed char* pa, const signed char* pb, const int* ea, const int* eb, const int* occa,
+  const int* occb, int s, int cutoff, int flush, int Mp, int Np, int Kp, int M, int N, double alpha, double beta,
+  void* c, int ldc, void* hip_stream) {
+  const dim3 grid((unsigned int)(Mp / GT), (unsigned int)(Np / GT));
+  const hipStream_t st = (hipStream_t)hip_stream;
+  if (0 != Mp % GT || 0 != Np % GT || 0 != Kp % GK || 0 >= Kp || Mp < M || Np < N) return LXS_ELIMIT;
+  if (1 > s || 8 < s || 0 > cutoff || 2 * (s - 1) < cutoff || 1 > flush || 2080 < flush) return LXS_ELIMIT;
+  if (3 == dtype) {
+    oz_gemm<double><<<grid, 256, 0, st>>>(pa, pb, ea, eb, occa, occb, s, cutoff, flush, Mp, Np, Kp, M, N, alpha, beta,
+      (double*)c, ldc);
+  }
+  else if (1 == dtype) {
+    oz_gemm<float><<<grid, 256, 0, st>>>(pa, pb, ea, eb, occa, occb, s, cutoff, flush, Mp, Np, Kp, M, N, alpha, beta,
+      (float*)c, ldc);
+  }
+  else {
+    return LXS_ENOVARIANT;
+  }
+  return (int)hipGetLastError();
+}
+
+/* C = beta C (alpha = 0 or K = 0) */
+extern "C" int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* hip_stream) {
+  const long blocks = ((long)M * N + 255) / 256;
+  const unsigned int grid = (unsigned int)(blocks < EW_BLOCKS ? blocks : EW_BLOCKS);
+  if (0 >= M || 0 >= N) return 0;
+  if (3 == dtype) oz_scale_c<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((double*)c, ldc, M, N, beta);
+  else if (1 == dtype) oz_scale_c<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((float*)c, ldc, M, N, beta);
+  else return LXS_ENOVARIANT;
+  return (int)hipGetLastError();
+}
+
+/* native FP64: C = alpha op(A) op(B) + beta C; op(A)(i, k) at a[i * ars + k * aks], op(B)(k, j) at b[k * bks + j * bcs] */
+extern "C" int lxs_dgemm(const double* a, long ars, long aks, const double* b, long bks, long bcs, int M, int N, int K,
+  double alpha, double beta, double* c, int ldc, void* hip_stream) {
+  if (!((1 == ars) || (1 == aks)) || !((1 == bks) || (1 == bcs))) return LXS_ELIMIT;
+  if (0 >= M || 0 >= N) return 0;
+  if (0 >= K) return lxs_oz_scale(3, c, ldc, M, N, beta, hip_stream);
+  dgemm_f64<<<dim3((unsigned int)((M + GT - 1) / GT), (unsigned int)((N + GT - 1) / GT)), 256, 0,
+    (hipStream_t)hip_stream>>>(a, ars, aks, b, bks, bcs, M, N, K, alpha, beta, c, ldc);
+  return (int)hipGetLastError();
+}
+
+extern "C" int lxs_oz_csplit(int dtype, const void* z, int ld, int rows, int cols, int conj, void* re, void* im,
+  void* sum, void* hip_stream) {
+  const long blocks = ((long)rows * cols + 255) / 256;
+  const unsigned int grid = (unsigned int)(blocks < EW_BLOCKS ? blocks : EW_BLOCKS);
+  if (0 >= rows || 0 >= cols) return 0;
+  if (3 == dtype) {
+    oz_csplit<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const double*)z, ld, rows, cols, conj, (double*)re,
+      (double*)im, (double*)sum);
+  }
+  else if (1 == dtype) {
+    oz_csplit<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const float*)z, ld, rows, cols, conj, (float*)re, (float*)im,
+      (float*)sum);
+  }
+  else {
+    return LXS_ENOVARIANT;
+  }
+  return (int)hipGetLastError();
+}
+
+extern "C" int lxs_oz_ccombine(int dtype, const void* t1, const void* t2, const void* t3, int M, int N,
+  const double* alpha, const double* beta, void* c, int ldc, void* hip_stream) {
+  const long blocks = ((long)M * N + 255) / 256;
+  const unsigned int grid = (unsigned int)(blocks < EW_BLOCKS ? blocks : EW_BLOCKS);
+  if (0 >= M || 0 >= N) return 0;
+  if (3 == dtype) {
+    oz_ccombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const double*)t1, (const double*)t2,
+      (const double*)t3, M, N, alpha[0], alpha[1], beta[0], beta[1], (double*)c, ldc);
+  }
+  else if (1 == dtype) {
+    oz_ccombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const float*)t1, (const float*)t2, (const float*)t3,
+      M, N, alpha[0], alpha[1], beta[0], beta[1], (float*)c, ldc);
+  }
+  else {
+    return LXS_ENOVARIANT;
+  }
+  return (int)hipGetLastError();
+}

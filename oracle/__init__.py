@@ -1,9 +1,11 @@
 """ORACLE: test infrastructure only; never the product path.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this package. It wraps
-liboracle.so (stack_oracle.c, cpu_baseline.c): a double-double CPU restatement of the reference's
-stack multiply (samples/smm/acc_bench.c:357-418 host path, kernels/multiply.cl:12-20 layout), its
-generators (acc_bench.h:30-79) and the timed CPU baseline.
+liboracle.so (stack_oracle.c, ozaki_oracle.c, mlp_oracle.c, cpu_baseline.c): a double-double CPU
+restatement of the reference's stack multiply (samples/smm/acc_bench.c:357-418 host path,
+kernels/multiply.cl:12-20 layout), its generators (acc_bench.h:30-79), a double-double GEMM, Ozaki scheme I restated
+(samples/ozaki/kernels/ozaki1_int8.cl, ozaki_common.cl:789-825), the predictor network's loss and
+gradient in double precision, and the timed CPU baseline.
 """
 import ctypes
 import os
@@ -44,6 +46,14 @@ def lib():
         L.oracle_max_threads.restype = i
         L.oracle_gemm_dd.argtypes = [ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, vp, vp, i, vp, vp]
         L.oracle_gemm_dd.restype = i
+        L.oracle_oz_digits.argtypes = [i, vp, i, vp, vp]
+        L.oracle_oz_digits.restype = i
+        L.oracle_oz_digits_ref.argtypes = [i, vp, i, vp, vp]
+        L.oracle_oz_digits_ref.restype = i
+        L.oracle_ozaki_gemm.argtypes = [i, ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, i, i, i, i, i]
+        L.oracle_ozaki_gemm.restype = i
+        L.oracle_mlp_loss_grad.argtypes = [i, vp, vp, vp, vp, vp, vp, i, i, vp, vp, i]
+        L.oracle_mlp_loss_grad.restype = i
         _lib = L
     return _lib
 
@@ -144,3 +154,58 @@ def gemm_dd(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, rows=N
                               None if rows is None else cols.ctypes.data)
     assert rc == 0
     return hi, lo
+
+
+OZ_PLANT_NONE, OZ_PLANT_DROPLEVEL, OZ_PLANT_WEIGHT, OZ_PLANT_ROUND, OZ_PLANT_TRANSB = 0, 1, 2, 3, 4
+
+
+def oz_digits(v, s):
+    """(exponent, digits[s][n]) of one row in the device formulation (x = v/2^ex, d = trunc(128x))."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty((s, v.size), dtype=np.int8)
+    ex = ctypes.c_int(0)
+    assert lib().oracle_oz_digits(v.size, v.ctypes.data, s, out.ctypes.data, ctypes.byref(ex)) == 0
+    return ex.value, out
+
+
+def oz_digits_ref(v, s):
+    """(max biased exponent, digits[s][n]) of one row in the reference's aligned-mantissa formulation."""
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    out = np.empty((s, v.size), dtype=np.int8)
+    ex = ctypes.c_int(0)
+    assert lib().oracle_oz_digits_ref(v.size, v.ctypes.data, s, out.ctypes.data, ctypes.byref(ex)) == 0
+    return ex.value, out
+
+
+def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, trim=0, flush=2080, maxk=0,
+               plant=OZ_PLANT_NONE):
+    """Ozaki scheme I emulated with the device's operation order; returns the updated copy of c
+    (column-major flat, float64 or float32 like a)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    out = np.array(c, dtype=a.dtype, copy=True, order="C").reshape(-1)
+    use_double = 1 if a.dtype == np.float64 else 0
+    rc = lib().oracle_ozaki_gemm(use_double, transa.encode(), transb.encode(), m, n, k, alpha, a.ctypes.data, lda,
+                                 b.ctypes.data, ldb, beta, out.ctypes.data, ldc, s, trim, flush, maxk, plant)
+    assert rc == 0
+    return out
+
+
+MLP_PLANT_NONE, MLP_PLANT_NORELU, MLP_PLANT_NOBIAS = 0, 1, 2
+
+
+def mlp_loss_grad(width, params, x, tgt, seg, hw, plant=MLP_PLANT_NONE):
+    """(mean loss, gradient) of the predictor network in double precision."""
+    width = np.ascontiguousarray(width, dtype=np.int32)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    tgt = np.ascontiguousarray(tgt, dtype=np.float64)
+    seg = np.ascontiguousarray(seg, dtype=np.int32)
+    hw = np.ascontiguousarray(hw, dtype=np.float64)
+    grad = np.zeros_like(params)
+    loss = ctypes.c_double(0)
+    rc = lib().oracle_mlp_loss_grad(len(width) - 1, width.ctypes.data, params.ctypes.data, x.ctypes.data,
+                                    tgt.ctypes.data, seg.ctypes.data, hw.ctypes.data, len(seg) - 1, x.shape[0],
+                                    grad.ctypes.data, ctypes.byref(loss), plant)
+    assert rc == 0
+    return loss.value, grad
