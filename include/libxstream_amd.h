@@ -103,6 +103,44 @@ LIBXSTREAM_API int libsmm_acc_batch_replay(libsmm_acc_batch_t* batch, void* stre
 LIBXSTREAM_API int libsmm_acc_batch_info(const libsmm_acc_batch_t* batch, int* ntasks, double* flops, double* bytes);
 LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch);
 
+/**
+ * Learned predictor (FP32 MLP trained and evaluated on the GPU by the library's own kernels).
+ * Where the reference maps an untuned shape to the parameters of the nearest tuned triplet
+ * (samples/smm/smm_kernel.c:116-148, :206-213), the network scores every valid compiled
+ * configuration. Heads (libsmm_acc_mlp_heads): softmax classifiers over variant, tm, tn, nt, kt,
+ * bs, flush and a regression of log2 GFLOP/s. Features: libsmm_acc_mlp_features (8 floats).
+ * Targets for training: one row of nout floats per sample, soft class distributions per head
+ * (summing to 1, or all zero to skip the head) and the log2 GFLOP/s value.
+ */
+typedef struct libsmm_acc_mlp_t libsmm_acc_mlp_t;
+/** hidden: width of the hidden layers, nhidden: their number; weights seeded deterministically */
+LIBXSTREAM_API int libsmm_acc_mlp_create(libsmm_acc_mlp_t** model, int hidden, int nhidden, unsigned long long seed);
+LIBXSTREAM_API int libsmm_acc_mlp_destroy(libsmm_acc_mlp_t* model);
+/** head layout: seg[h] = first output of head h (seg[nheads] = outputs), values[] = class values
+ *  (-1 for the regression head); returns the number of heads */
+LIBXSTREAM_API int libsmm_acc_mlp_heads(int* seg, int* values, int cap);
+LIBXSTREAM_API int libsmm_acc_mlp_features(libsmm_acc_data_t datatype, int m, int n, int k, float* features);
+/** number of parameters; copies up to cap of them to params when not NULL */
+LIBXSTREAM_API int libsmm_acc_mlp_params(libsmm_acc_mlp_t* model, float* params, int cap);
+LIBXSTREAM_API int libsmm_acc_mlp_set_params(libsmm_acc_mlp_t* model, const float* params, int n);
+/** one forward and backward pass on the GPU: gradient of the mean loss (nparams floats) and the loss;
+ *  head_weights (one per head) may be NULL (all 1) */
+LIBXSTREAM_API int libsmm_acc_mlp_gradient(libsmm_acc_mlp_t* model, const float* x, const float* targets,
+  const float* head_weights, int batch, float* grad, double* loss);
+/** full-batch Adam on the GPU for epochs steps; loss of the final parameters when loss != NULL */
+LIBXSTREAM_API int libsmm_acc_mlp_train(libsmm_acc_mlp_t* model, const float* x, const float* targets,
+  const float* head_weights, int batch, int epochs, double lr, double weight_decay, double* loss);
+/** inference on the GPU: out[batch][outputs] */
+LIBXSTREAM_API int libsmm_acc_mlp_forward(libsmm_acc_mlp_t* model, const float* x, int batch, float* out);
+LIBXSTREAM_API int libsmm_acc_mlp_save(const libsmm_acc_mlp_t* model, const char* filename);
+LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* filename);
+/** best valid configuration for the shape by the network, with its share of the candidates' probability */
+LIBXSTREAM_API int libsmm_acc_mlp_predict_config(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
+  int k, libsmm_acc_smm_config_t* config, double* confidence);
+/** install (or remove: NULL) the model as predictor for shapes without a tuned configuration; a
+ *  prediction below min_confidence leaves the built-in default in place */
+LIBXSTREAM_API int libsmm_acc_learned_set(libsmm_acc_mlp_t* model, double min_confidence);
+
 /** profiling: per-kernel launch records timed by HIP events on the launch stream */
 LIBXSTREAM_API int libxstream_profile_enable(int enable);
 LIBXSTREAM_API int libxstream_profile_reset(void);

@@ -49,6 +49,10 @@ void* lxs_profile_begin(hipStream_t stream);
 void lxs_profile_end(void* token, hipStream_t stream, const char* kernel, double flops, double bytes);
 void lxs_profile_finalize(void);
 
+/** learned predictor slot (mlp.c): EXIT_SUCCESS and a configuration when a confident model is installed */
+int lxs_learned_predict(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config);
+void lxs_learned_finalize(void);
+
 /** LIBSMM registry lifetime (smm.c) */
 int lxs_smm_init(void);
 void lxs_smm_finalize(void);

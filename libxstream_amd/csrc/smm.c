@@ -229,6 +229,9 @@ static int lxs_smm_env(libsmm_acc_smm_config_t* config) {
   return any;
 }
 
+static int lxs_smm_register(int datatype, int m, int n, int k, const libsmm_acc_smm_config_t* config, int source,
+  int keep_best);
+
 static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config, int* source) {
   const lxs_smm_slot_t* slot;
   int src = libsmm_acc_config_default;
@@ -239,7 +242,15 @@ static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_co
     src = slot->source;
   }
   pthread_mutex_unlock(&lxs_smm_lock);
-  if (NULL == slot) lxs_smm_default(datatype, m, n, k, config);
+  if (NULL == slot) {
+    if (EXIT_SUCCESS == lxs_learned_predict(datatype, m, n, k, config)) { /* cached for the next lookup */
+      src = libsmm_acc_config_learned;
+      (void)lxs_smm_register(datatype, m, n, k, config, libsmm_acc_config_learned, 0);
+    }
+    else {
+      lxs_smm_default(datatype, m, n, k, config);
+    }
+  }
   {
     libsmm_acc_smm_config_t env = *config;
     if (0 != lxs_smm_env(&env)) {
@@ -491,6 +502,7 @@ int lxs_smm_init(void) {
 }
 
 void lxs_smm_finalize(void) {
+  lxs_learned_finalize();
   pthread_mutex_lock(&lxs_smm_lock);
   free(lxs_smm_slots);
   lxs_smm_slots = NULL;
