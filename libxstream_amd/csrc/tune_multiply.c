@@ -13,13 +13,15 @@
  *     DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH (load with OPENCL_LIBSMM_SMM_PARAMS).
  *
  * usage: tune_multiply [-t f64|f32] [-s stack] [-r reps] [-o best.csv] [-j sweep.jsonl]
- *                      [-d device] [-f triplets.txt | m n k [m n k ...]]
+ *                      [-d device] [-T seconds] [-f triplets.txt | m n k [m n k ...]]
+ * -T stops before the next shape once the time budget is spent (a resumed run continues there).
  */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 #include "../../include/libxstream_amd.h"
 
 typedef struct tune_opts_t {
@@ -287,10 +289,18 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   return result;
 }
 
+static double seconds_now(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 int main(int argc, char* argv[]) {
   tune_opts_t o;
-  int i, result, ndevices = 0;
+  int i, result, ndevices = 0, ntuned = 0;
   FILE* triplets = NULL;
+  double budget = 0;
+  const double start = seconds_now();
   o.dtype = dbcsr_type_real_8;
   o.stack_size = 30000;
   o.reps = 5;
@@ -307,6 +317,7 @@ int main(int argc, char* argv[]) {
       case 'j': o.jsonl = v; break;
       case 'd': o.device = v; break;
       case 'f': triplets = fopen(v, "r"); break;
+      case 'T': budget = atof(v); break;
       default: fprintf(stderr, "tune_multiply: unknown option %s\n", argv[i]); return EXIT_FAILURE;
     }
   }
@@ -319,7 +330,14 @@ int main(int argc, char* argv[]) {
   }
   if (NULL != triplets) {
     int m, n, k;
-    while (EXIT_SUCCESS == result && 3 == fscanf(triplets, "%i %i %i", &m, &n, &k)) result = tune_shape(&o, m, n, k);
+    while (EXIT_SUCCESS == result && 3 == fscanf(triplets, "%i %i %i", &m, &n, &k)) {
+      if (0 < budget && seconds_now() - start > budget) {
+        printf("tune_multiply: time budget of %.0f s reached after %i shapes\n", budget, ntuned);
+        break;
+      }
+      result = tune_shape(&o, m, n, k);
+      ++ntuned;
+    }
     fclose(triplets);
   }
   for (; i + 2 < argc + 0 && EXIT_SUCCESS == result; i += 3) {

@@ -126,3 +126,20 @@ def nearest_tuned(key, tuned):
         return None
     d = [(t[1] - m) ** 2 + (t[2] - n) ** 2 + (t[3] - k) ** 2 for t in cands]
     return cands[int(np.argmin(d))]
+
+
+def write_triplets(path, count=500, seed=25071975):
+    """the sweep's triplet file (one 'm n k' per line, tune_multiply -f): 400 training, then 100 held out"""
+    with open(path, "w") as f:
+        for m, n, k in triplets(count, seed):
+            f.write("%d %d %d\n" % (m, n, k))
+
+
+if __name__ == "__main__":
+    import sys
+
+    if len(sys.argv) == 3 and sys.argv[1] == "triplets":
+        write_triplets(sys.argv[2])
+    else:
+        print("usage: python -m libxstream_amd.learned triplets FILE", file=sys.stderr)
+        sys.exit(2)
