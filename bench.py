@@ -51,6 +51,7 @@ def parse():
     p.add_argument("--no-batch", action="store_true", help="skip the batched mixed-shape measurement")
     p.add_argument("--no-ozaki", action="store_true", help="skip the Ozaki-int8 DGEMM measurement")
     p.add_argument("--ozaki-sizes", default="1024,4096,8192", help="square DGEMM sizes for the Ozaki section")
+    p.add_argument("--ozaki-only", action="store_true", help="print only the Ozaki section (no stack measurement)")
     p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -400,6 +401,16 @@ def main():
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the stack path shards by C-run "
               "ranges with no exchange between GPUs)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.ozaki_only:
+        import libxstream_amd as lx
+
+        lx.init()
+        stream = lx.Stream("bench")
+        out = ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
+        stream.close()
+        lx.finalize()
+        print(json.dumps({"ozaki": out}))
+        return
     # PMC child runs first, before this process initialises the GPU
     tr, tr_detail = (None, None) if args.no_traffic else traffic(args)
 

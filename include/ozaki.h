@@ -7,7 +7,7 @@
  *
  * Scheme: every row of op(A) and column of op(B) is scaled by a power of two and cut into nslices
  * signed 7-bit digits (int8 planes); slice pairs (p, q) with p + q <= cutoff = 2(nslices-1) - oztrim
- * are multiplied with int8 MFMA, accumulated exactly in int32 per level p + q and recombined in FP64.
+ * are multiplied with int8 MFMA, summed exactly per level p + q and recombined in FP64.
  * Only Scheme 1 (mantissa slicing) is built: kind 2 (CRT) is rejected, kind 0/3 (adaptive) resolves
  * to Scheme 1.
  */
@@ -43,13 +43,14 @@ typedef struct ozaki_context_t {
   int ozflags;    /* OZAKI_TRIANGULAR | OZAKI_SYMMETRIZE: 0 and 3 compute the full sum */
   int oztrim;     /* levels dropped from the top of the pair triangle, 0..2(nslices-1) */
   int verbosity;  /* 0: quiet, 1: info, 2+: debug */
-  int tm, tn;     /* output tile of one work-group (128 x 128) */
+  int tm, tn;     /* output tile of one work-group (256 x 256) */
   int maxk;       /* K per preprocessing pass (0: all of K) */
-  int flush;      /* k-blocks of 64 accumulated in int32 before an FP64 fold (1..2080) */
+  int flush;      /* k-blocks of 128 per int8 work item, 1..1040 (env LIBXSTREAM_OZAKI_FLUSH); results
+                     do not depend on it: level sums are exact */
   void* impl;     /* device state owned by the context */
 } ozaki_context_t;
 
-/** samples/ozaki/ozaki_opencl.h:328; tm/tn: 0 or 128; ozgroups is accepted and ignored (Scheme 2) */
+/** samples/ozaki/ozaki_opencl.h:328; tm/tn: 0 or 256; ozgroups is accepted and ignored (Scheme 2) */
 LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_double, int kind, int verbosity, int ndecomp,
   int ozflags, int oztrim, int ozgroups, int maxk);
 /** samples/ozaki/ozaki_opencl.h:330 */
@@ -67,7 +68,7 @@ LIBXSTREAM_API void* ozaki_scratch_get(const ozaki_context_t* ctx, size_t* nbyte
 typedef struct ozaki_tile_t {
   int m, n;
 } ozaki_tile_t;
-/** samples/ozaki/ozaki_opencl.h:384: the output tile the GEMM kernel uses (fixed 128 x 128) */
+/** samples/ozaki/ozaki_opencl.h:384: the output tile the GEMM kernel uses (fixed 256 x 256) */
 LIBXSTREAM_API ozaki_tile_t ozaki_tile_select(const ozaki_context_t* ctx, int M, int N, int rtm, int rtn);
 /** samples/ozaki/ozaki_opencl.h:411: N-panel width; the device path is not panelled, so N */
 LIBXSTREAM_API int ozaki_npanel(const ozaki_context_t* ctx, int M, int N, int tm, int tn);

@@ -73,12 +73,16 @@ int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* dat
 int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream);
 
 /* Ozaki scheme I (ozaki.hip). Operand element (r, k) at p[r * rs + k * ks] (rs or ks is 1);
-   planes: s x Rp x Kp int8 (Rp multiple of 128, Kp multiple of 64); e: Rp ints; occ: 8 ints */
+   planes: s x Rp x Kp int8 (Rp multiple of 256, Kp multiple of 128); e: Rp ints; occ: 8 ints */
 int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int s, int Rp, int Kp, int* e,
   int* occ, signed char* planes, void* hip_stream);
-int lxs_oz_gemm(int dtype, const signed char* pa, const signed char* pb, const int* ea, const int* eb,
-  const int* occa, const int* occb, int s, int cutoff, int flush, int Mp, int Np, int Kp, int M, int N, double alpha,
-  double beta, void* c, int ldc, void* hip_stream);
+/* int8 level GEMM over work items {level, u0, u1, pad} (host array): int32 partial z at c32 + z Mp Np,
+   element (i, j) at j Mp + i; an item holds at most 1040 units (pair, k-block of 128) */
+int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, const int* occa, const int* occb,
+  const int* items, int nitems, int s, int Mp, int Np, int Kp, int* c32, void* hip_stream);
+/* FP64 recombination of the partials: zb[L] = first item of level L (cutoff + 2 entries) */
+int lxs_oz_recombine(int dtype, const int* c32, const int* zb, int cutoff, int Mp, int Np, const int* ea,
+  const int* eb, int M, int N, double alpha, double beta, void* c, int ldc, void* hip_stream);
 int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* hip_stream);
 int lxs_dgemm(const double* a, long ars, long aks, const double* b, long bks, long bcs, int M, int N, int K,
   double alpha, double beta, double* c, int ldc, void* hip_stream);

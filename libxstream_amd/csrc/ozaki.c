@@ -11,11 +11,14 @@
 #include <string.h>
 
 #define OZ_ALIGN 256
-#define OZ_TILE_MN 128
-#define OZ_TILE_K 64
-#define OZ_FLUSH_MAX 2080 /* 2080 * 64 * 127^2 < 2^31 */
+#define OZ_TILE_MN 256    /* output tile of the int8 level GEMM; row padding of the digit planes */
+#define OZ_TILE_K 128     /* k-block of the level GEMM; k padding of the digit planes */
+#define OZ_FLUSH_MAX 1040 /* k-blocks per work item: 1040 * 128 * 127^2 < 2^31 (exact int32 partials) */
+#define OZ_WG_TARGET 1024 /* work-groups the item plan aims for */
 
 typedef struct oz_impl_t {
+  int* items; /* work items {level, u0, u1, pad} of the current call */
+  int items_cap;
   void* arena; /* internal scratch, grown on demand */
   size_t arena_size;
   void* user; /* caller-owned scratch (ozaki_scratch_set) */
@@ -44,7 +47,7 @@ LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_doub
     return EXIT_FAILURE;
   }
   if ((0 != tm && OZ_TILE_MN != tm) || (0 != tn && OZ_TILE_MN != tn)) {
-    lxs_error(EXIT_FAILURE, "ozaki_init: the output tile is 128 x 128");
+    lxs_error(EXIT_FAILURE, "ozaki_init: the output tile is 256 x 256");
     return EXIT_FAILURE;
   }
   nslices = (0 < ndecomp ? ndecomp : (0 != use_double ? 8 : 4));
@@ -86,6 +89,7 @@ LIBXSTREAM_API void ozaki_destroy(ozaki_context_t* ctx) {
       if (NULL != impl->arena) (void)hipFree(impl->arena);
       if (NULL != impl->carena) (void)hipFree(impl->carena);
     }
+    free(impl->items);
     free(impl);
     ctx->impl = NULL;
   }
@@ -101,9 +105,44 @@ static int oz_trans(char t) {
   }
 }
 
+/*
+ * Work items of the level GEMM: every level L = 0 .. cutoff has npairs(L) * nkb units (pair, k-block),
+ * cut into items of at most cu units, cu chosen so that items x tiles is about OZ_WG_TARGET
+ * work-groups (at least 8 units per item, at most ctx->flush). zb[L] = first item of level L.
+ * Returns the number of items; fills items (4 ints each) when not NULL.
+ */
+static int oz_plan(const ozaki_context_t* ctx, int nkb, int tiles, int* items, int* zb) {
+  const int s = ctx->nslices, cutoff = 2 * (ctx->nslices - 1) - ctx->oztrim;
+  long total = 0;
+  int L, n = 0, cu;
+  for (L = 0; L <= cutoff; ++L) {
+    const int plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
+    total += (long)(phi - plo + 1) * nkb;
+  }
+  cu = (int)((total * tiles + OZ_WG_TARGET - 1) / OZ_WG_TARGET);
+  if (cu < 8) cu = 8;
+  if (cu > ctx->flush) cu = ctx->flush;
+  for (L = 0; L <= cutoff; ++L) {
+    const int plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
+    const int units = (phi - plo + 1) * nkb;
+    int u;
+    if (NULL != zb) zb[L] = n;
+    for (u = 0; u < units; u += cu, ++n) {
+      if (NULL != items) {
+        items[4 * n + 0] = L;
+        items[4 * n + 1] = u;
+        items[4 * n + 2] = (units - u < cu ? units : u + cu);
+        items[4 * n + 3] = 0;
+      }
+    }
+  }
+  if (NULL != zb) zb[cutoff + 1] = n;
+  return n;
+}
+
 typedef struct oz_layout_t {
-  int Mp, Np, Kp, kpass;
-  size_t planes_a, planes_b, ea, eb, occ, a, b, c, total; /* byte offsets and total */
+  int Mp, Np, Kp, kpass, nitems;
+  size_t planes_a, planes_b, ea, eb, occ, c32, a, b, c, total; /* byte offsets and total */
 } oz_layout_t;
 
 static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
@@ -115,6 +154,7 @@ static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, 
   l->Np = (int)oz_up((size_t)(0 < N ? N : 1), OZ_TILE_MN);
   l->Kp = (int)oz_up((size_t)(0 < kpass ? kpass : 1), OZ_TILE_K);
   l->kpass = kpass;
+  l->nitems = oz_plan(ctx, l->Kp / OZ_TILE_K, (l->Mp / OZ_TILE_MN) * (l->Np / OZ_TILE_MN), NULL, NULL);
   l->planes_a = off;
   off += oz_up((size_t)ctx->nslices * l->Mp * l->Kp, OZ_ALIGN);
   l->planes_b = off;
@@ -125,6 +165,8 @@ static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, 
   off += oz_up(sizeof(int) * (size_t)l->Np, OZ_ALIGN);
   l->occ = off;
   off += OZ_ALIGN; /* 2 x 8 ints */
+  l->c32 = off;
+  off += oz_up(sizeof(int) * (size_t)l->nitems * l->Mp * l->Np, OZ_ALIGN);
   l->a = l->b = l->c = 0;
   if (0 == dev) {
     l->a = off;
@@ -259,9 +301,17 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
     const long bks = (0 == tb ? 1 : ldb), bcs = (0 == tb ? ldb : 1);
     int* const occ = (int*)(base + l.occ);
     const double npairs = ozaki_count_pairs(s, cutoff, 0);
+    int zb[16];
+    if (impl->items_cap < l.nitems) {
+      int* const items = (int*)realloc(impl->items, sizeof(int) * 4 * (size_t)l.nitems);
+      if (NULL == items) return lxs_error(EXIT_FAILURE, "ozaki_gemm: out of memory");
+      impl->items = items;
+      impl->items_cap = l.nitems;
+    }
     for (k0 = 0; k0 < K && 0 == result; k0 += l.kpass) {
       const int kk = (K - k0 < l.kpass ? K - k0 : l.kpass);
       const int Kp = (int)oz_up((size_t)kk, OZ_TILE_K);
+      const int nitems = oz_plan(ctx, Kp / OZ_TILE_K, (l.Mp / OZ_TILE_MN) * (l.Np / OZ_TILE_MN), impl->items, zb);
       void* token = lxs_profile_begin(hs);
       result = lxs_oz_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, s, l.Mp, Kp, (int*)(base + l.ea), occ,
         (signed char*)(base + l.planes_a), hs);
@@ -272,11 +322,16 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
       lxs_profile_end(token, hs, "ozaki_pre", 0.0, (double)ts * ((double)M + N) * kk + (double)s * (l.Mp + l.Np) * Kp);
       if (0 == result) { /* int8 work of every pair up to the cutoff (pairs of all-zero planes are skipped) */
         token = lxs_profile_begin(hs);
-        result = lxs_oz_gemm(dtype, (const signed char*)(base + l.planes_a), (const signed char*)(base + l.planes_b),
-          (const int*)(base + l.ea), (const int*)(base + l.eb), occ, occ + 8, s, cutoff, ctx->flush, l.Mp, l.Np, Kp, M, N,
-          alpha, (0 == k0 ? beta : 1.0), dc, ldc, hs);
+        result = lxs_oz_level_gemm((const signed char*)(base + l.planes_a), (const signed char*)(base + l.planes_b), occ,
+          occ + 8, impl->items, nitems, s, l.Mp, l.Np, Kp, (int*)(base + l.c32), hs);
         lxs_profile_end(token, hs, "ozaki_i8", 2.0 * npairs * l.Mp * l.Np * Kp,
-          (double)s * (l.Mp + l.Np) * Kp + (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2));
+          (double)s * (l.Mp + l.Np) * Kp + 4.0 * nitems * l.Mp * l.Np);
+      }
+      if (0 == result) {
+        token = lxs_profile_begin(hs);
+        result = lxs_oz_recombine(dtype, (const int*)(base + l.c32), zb, cutoff, l.Mp, l.Np, (const int*)(base + l.ea),
+          (const int*)(base + l.eb), M, N, alpha, (0 == k0 ? beta : 1.0), dc, ldc, hs);
+        lxs_profile_end(token, hs, "ozaki_fp64", 0.0, 4.0 * nitems * M * N + (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2));
       }
     }
   }

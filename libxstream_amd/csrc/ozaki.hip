@@ -8,22 +8,23 @@
  * 7-bit digits by truncation,
  *   x = sum_{p=0..s-1} d_p 2^(-7(p+1)) + tail,  d_p in [-127, 127],  |tail| < 2^(-7s).
  * Then (AB)(i,j) ~ 2^(ea_i + eb_j) sum_{L=0..cutoff} 2^(-7(L+2)) sum_{p+q=L} (A_p B_q)(i,j), with
- * A_p, B_q int8 digit planes. Every inner sum is an int8 GEMM accumulated exactly in int32.
+ * A_p, B_q int8 digit planes. Every inner sum is an int8 GEMM accumulated exactly in integers.
  * cutoff = 2(s-1) - trim: trim 0 takes all s^2 slice pairs, trim s-1 the triangle p+q <= s-1.
  * Zero, subnormal and non-finite inputs count as zero (ozaki_common.cl:784-805).
  *
  * Kernels (new design for MI355X):
  *   oz_init     exponents := none, occupancy := 0
  *   oz_exp      per-row (A) / per-column (B) exponent: max over the row of (biased exponent - bias + 1)
- *   oz_slice    digits of 64x64 tiles into s planes [s][Rp][Kp] (k contiguous, rows padded to 128,
- *               k to 64, padding zero), so GEMM operand tiles are plain 16-byte vectors; sets one
+ *   oz_slice    digits of 64x64 tiles into s planes [s][Rp][Kp] (k contiguous, rows padded to 256,
+ *               k to 128, padding zero), so GEMM operand tiles are plain 16-byte vectors; sets one
  *               occupancy flag per plane that holds a nonzero digit
- *   oz_gemm     128x128 output tile per work-group (4 wavefronts of 64x64, v_mfma_i32_32x32x32_i8).
- *               Levels L = cutoff .. 0 (smallest weight first). The pairs of a level with both planes
- *               occupied accumulate in int32 over K, folded into FP64 registers as cd += 2^(-7(L+2)) acc
- *               at the level's end, or after every `flush` k-blocks of 64 (exactness bound
- *               flush * 64 * 127^2 < 2^31). Epilogue C = fma(alpha, 2^(ea+eb) cd, beta C).
- *               Operand tiles are staged through LDS, double-buffered.
+ *   oz_level_gemm  pure int8 -> int32 GEMM per work item (level L, range of its (pair, k-block) units):
+ *               256x256 output tile per work-group, 4 wavefronts of 128x128 (v_mfma_i32_32x32x32_i8),
+ *               operand tiles of 128 k staged through LDS, double-buffered, with the next tile's
+ *               global loads in flight during the MFMAs; pairs whose planes are all zero are skipped.
+ *               Every item writes int32 partial sums (exact: an item's units are bounded).
+ *   oz_recombine   per element: S_L = exact int64 sum of the level's partials, cd = sum over
+ *               L = cutoff .. 0 of 2^(-7(L+2)) S_L in FP64, C = fma(alpha, 2^(ea+eb) cd, beta C).
  *   dgemm_f64   128x128 tiles with v_mfma_f64_16x16x4f64 on FP64 operands (the comparator).
  */
 #include <hip/hip_runtime.h>
@@ -155,27 +156,42 @@ __global__ void __launch_bounds__(256) oz_scale_c(T* __restrict__ c, int ldc, in
 
 /* ---- int8 level GEMM ------------------------------------------------------------------------- */
 
-constexpr int GT = 128;      /* output tile (rows and columns) per work-group */
-constexpr int GK = 64;       /* k per LDS stage ("k-block") */
-constexpr int GLD = GK + 16; /* LDS row stride in bytes (16-byte reads of 8 consecutive rows hit distinct banks) */
+constexpr int GT = 128;        /* output tile of the FP64 comparator */
+constexpr int LT = 256;        /* int8 level GEMM: output tile (i and j) per work-group */
+constexpr int LK = 128;        /* k per LDS stage ("k-block") */
+constexpr int LLD = LK + 16;   /* LDS row stride in bytes: 16-byte reads of 8 consecutive rows hit distinct banks */
 
-/* v_mfma_i32_32x32x32_i8 lane layout: A operand lane l holds row l%32, k = 16(l/32) + byte; B operand
-   lane l holds column l%32, same k; D lane l register r holds column l%32, row 8(r/4) + 4(l/32) + r%4 */
-__device__ __forceinline__ int oz_drow(int lane, int r) { return 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3); }
+/*
+ * One work item: level L (pairs p = plo .. phi, q = L - p), units u0 .. u1 of the level (unit u is
+ * pair plo + u / nkb, k-block u % nkb). Every item writes its own int32 partial sums, exact because
+ * an item holds at most 1040 units: 1040 * 128 * 127^2 < 2^31.
+ */
+struct oz_item_t {
+  int level, u0, u1, pad;
+};
+constexpr int OZ_ITEMS = 128; /* items per launch, passed by value */
+struct oz_items_t {
+  oz_item_t it[OZ_ITEMS];
+};
 
-template <typename T>
-__global__ void __launch_bounds__(256) oz_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb,
-  const int* __restrict__ ea, const int* __restrict__ eb, const int* __restrict__ occa, const int* __restrict__ occb,
-  int s, int cutoff, int flush, int Mp, int Np, int Kp, int M, int N, double alpha, double beta, T* __restrict__ c,
-  int ldc) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[2][2][GT * GLD]; /* [buffer][A|B][128 rows x GLD] */
+/*
+ * C32[z](j, i) = sum over the item's units of sum_k A_p(i, k) B_q(j, k), stored at z * Mp * Np + j * Mp + i.
+ * v_mfma_i32_32x32x32_i8 with the B planes as the first operand (rows j) and the A planes as the
+ * second (columns i): lane l of D holds column i = l % 32 and rows 8(r/4) + 4(l/32) + r%4, so for a
+ * fixed register the 32 lanes of a half-wave store 128 contiguous bytes along i.
+ * 4 wavefronts in 2 x 2, each 128 x 128 (4 x 4 MFMA tiles, 256 accumulator registers).
+ */
+__global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb,
+  const int* __restrict__ occa, const int* __restrict__ occb, oz_items_t items, int s, int Mp, int Np, int Kp,
+  int* __restrict__ c32) {
+  __shared__ __attribute__((aligned(16))) int8_t lds[2][2][LT * LLD]; /* [buffer][A|B][256 rows x LLD] */
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int wr = wave >> 1, wc = wave & 1; /* 64x64 quadrant of the wavefront */
-  const int row0 = blockIdx.x * GT, col0 = blockIdx.y * GT;
-  const int nkb = Kp / GK;
-  /* staging: thread t copies 32 bytes of row t/2 of both tiles */
-  const int srow = tid >> 1, shalf = (tid & 1) * 32;
+  const int wj = wave >> 1, wi = wave & 1;
+  const int i0 = blockIdx.x * LT, j0 = blockIdx.y * LT;
+  const oz_item_t it = items.it[blockIdx.z];
+  const int L = it.level, nkb = Kp / LK;
+  const int plo = (L - s + 1 > 0 ? L - s + 1 : 0);
   unsigned int oa = 0, ob = 0; /* occupied planes */
   for (int p = 0; p < s; ++p) {
     oa |= (0 != occa[p] ? 1u : 0u) << p;
@@ -183,113 +199,126 @@ __global__ void __launch_bounds__(256) oz_gemm(const int8_t* __restrict__ pa, co
   }
   oa = __builtin_amdgcn_readfirstlane(oa);
   ob = __builtin_amdgcn_readfirstlane(ob);
-  d4_t cd[2][2][4];
+  i16_t acc[4][4];
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
+  for (int x = 0; x < 4; ++x) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) cd[a][b][q] = d4_t{0, 0, 0, 0};
-    }
+    for (int y = 0; y < 4; ++y) acc[x][y] = i16_t{0};
   }
-  i16_t acc[2][2];
-  i4_t ra[2], rb[2];
-  for (int L = cutoff; L >= 0; --L) {
-    /* pairs (p, L - p) of this level with both planes occupied, as a bit mask over p */
-    unsigned int mask = 0;
-    for (int p = (L - s + 1 > 0 ? L - s + 1 : 0); p <= (L < s - 1 ? L : s - 1); ++p) {
-      if ((oa >> p) & (ob >> (L - p)) & 1u) mask |= 1u << p;
-    }
-    if (0 == mask) continue;
-    const double scale = ldexp(1.0, -7 * (L + 2));
+  /* next unit to fetch: skip units of pairs whose planes hold no nonzero digit */
+  int fu = it.u0;
+  auto active = [&](int u) {
+    const int p = plo + u / nkb;
+    return 0 != ((oa >> p) & (ob >> (L - p)) & 1u);
+  };
+  auto advance = [&]() {
+    while (fu < it.u1 && !active(fu)) fu = (fu / nkb + 1) * nkb; /* skip the rest of an inactive pair */
+  };
+  advance();
+  if (fu < it.u1) {
+    /* staging: 16-byte chunk c = tid + 256 x of each operand tile (256 rows x 128 bytes): row c/8, chunk c%8 */
+    i4_t ra[8], rb[8];
+    auto fetch = [&]() {
+      const int p = plo + fu / nkb, kb = fu - (fu / nkb) * nkb;
+      const int8_t* const ga = pa + ((long)p * Mp + i0) * Kp + kb * LK;
+      const int8_t* const gb = pb + ((long)(L - p) * Np + j0) * Kp + kb * LK;
 #pragma unroll
-    for (int a = 0; a < 2; ++a) {
-#pragma unroll
-      for (int b = 0; b < 2; ++b) acc[a][b] = i16_t{0};
-    }
-    /* k-blocks in order: pairs by increasing p, k-blocks by increasing k */
-    int fp = __builtin_ctz(mask), fkb = 0; /* next k-block to fetch */
-    bool more = true;
-#define OZ_FETCH() \
-  do { \
-    const int8_t* const sa_ = pa + ((long)fp * Mp + row0 + srow) * Kp + fkb * GK + shalf; \
-    const int8_t* const sb_ = pb + ((long)(L - fp) * Np + col0 + srow) * Kp + fkb * GK + shalf; \
-    ra[0] = *(const i4_t*)sa_; \
-    ra[1] = *(const i4_t*)(sa_ + 16); \
-    rb[0] = *(const i4_t*)sb_; \
-    rb[1] = *(const i4_t*)(sb_ + 16); \
-    if (++fkb == nkb) { \
-      const unsigned int rest_ = mask & ~((2u << fp) - 1u); \
-      fkb = 0; \
-      more = (0 != rest_); \
-      fp = (more ? __builtin_ctz(rest_) : fp); \
-    } \
-  } while (0)
-#define OZ_STORE(BUF) \
-  do { \
-    *(i4_t*)(&lds[BUF][0][srow * GLD + shalf]) = ra[0]; \
-    *(i4_t*)(&lds[BUF][0][srow * GLD + shalf + 16]) = ra[1]; \
-    *(i4_t*)(&lds[BUF][1][srow * GLD + shalf]) = rb[0]; \
-    *(i4_t*)(&lds[BUF][1][srow * GLD + shalf + 16]) = rb[1]; \
-  } while (0)
-    OZ_FETCH();
-    __syncthreads(); /* the previous level's last reads of the LDS buffers are done */
-    OZ_STORE(0);
-    __syncthreads();
-    int buf = 0, cnt = 0;
-    for (;;) {
-      const bool last = !more;
-      if (!last) OZ_FETCH();
-#pragma unroll
-      for (int kk = 0; kk < GK; kk += 32) {
-        i4_t fa[2], fb[2];
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-          fa[a] = *(const i4_t*)(&lds[buf][0][(64 * wr + 32 * a + (lane & 31)) * GLD + kk + 16 * (lane >> 5)]);
-          fb[a] = *(const i4_t*)(&lds[buf][1][(64 * wc + 32 * a + (lane & 31)) * GLD + kk + 16 * (lane >> 5)]);
-        }
-#pragma unroll
-        for (int a = 0; a < 2; ++a) {
-#pragma unroll
-          for (int b = 0; b < 2; ++b) acc[a][b] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa[a], fb[b], acc[a][b], 0, 0, 0);
-        }
+      for (int x = 0; x < 8; ++x) {
+        const int c = tid + 256 * x, row = c >> 3, kc = (c & 7) * 16;
+        ra[x] = *(const i4_t*)(ga + (long)row * Kp + kc);
+        rb[x] = *(const i4_t*)(gb + (long)row * Kp + kc);
       }
-      if (++cnt == flush || last) { /* fold the exact int32 partial sums into FP64 */
+      ++fu;
+      advance();
+    };
+    auto store = [&](int buf) {
 #pragma unroll
-        for (int a = 0; a < 2; ++a) {
+      for (int x = 0; x < 8; ++x) {
+        const int c = tid + 256 * x, row = c >> 3, kc = (c & 7) * 16;
+        *(i4_t*)(&lds[buf][0][row * LLD + kc]) = ra[x];
+        *(i4_t*)(&lds[buf][1][row * LLD + kc]) = rb[x];
+      }
+    };
+    fetch();
+    store(0);
+    __syncthreads();
+    for (int buf = 0;; buf ^= 1) {
+      const bool last = !(fu < it.u1);
+      if (!last) fetch();
+      { /* two fragment sets: the LDS reads of the next 32 k overlap the 16 MFMAs of the current ones */
+        const int8_t* const la = &lds[buf][0][(128 * wi + (lane & 31)) * LLD + 16 * (lane >> 5)];
+        const int8_t* const lb = &lds[buf][1][(128 * wj + (lane & 31)) * LLD + 16 * (lane >> 5)];
+        i4_t fa0[4], fb0[4], fa1[4], fb1[4];
+        auto frag = [&](i4_t* fa, i4_t* fb, int kk) {
 #pragma unroll
-          for (int b = 0; b < 2; ++b) {
-#pragma unroll
-            for (int r = 0; r < 16; ++r) cd[a][b][r >> 2][r & 3] += scale * (double)acc[a][b][r];
-            acc[a][b] = i16_t{0};
+          for (int x = 0; x < 4; ++x) {
+            fb[x] = *(const i4_t*)(lb + 32 * x * LLD + kk);
+            fa[x] = *(const i4_t*)(la + 32 * x * LLD + kk);
           }
+        };
+        auto mma = [&](const i4_t* fa, const i4_t* fb) {
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+#pragma unroll
+            for (int y = 0; y < 4; ++y) acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[x], fa[y], acc[x][y], 0, 0, 0);
+          }
+        };
+        frag(fa0, fb0, 0);
+#pragma unroll 1
+        for (int kk = 0; kk < LK; kk += 64) {
+          frag(fa1, fb1, kk + 32);
+          mma(fa0, fb0);
+          if (kk + 64 < LK) frag(fa0, fb0, kk + 64);
+          mma(fa1, fb1);
         }
-        cnt = 0;
       }
       if (last) break;
-      OZ_STORE(buf ^ 1);
+      store(buf ^ 1);
       __syncthreads();
-      buf ^= 1;
     }
-#undef OZ_FETCH
-#undef OZ_STORE
   }
-  /* epilogue: C = fma(alpha, 2^(ea+eb) cd, beta C) */
+  int* const out = c32 + (long)blockIdx.z * Mp * Np;
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
+  for (int x = 0; x < 4; ++x) {
 #pragma unroll
-    for (int b = 0; b < 2; ++b) {
-      const int j = col0 + 64 * wc + 32 * b + (lane & 31);
+    for (int y = 0; y < 4; ++y) {
+      const int i = i0 + 128 * wi + 32 * y + (lane & 31);
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int i = row0 + 64 * wr + 32 * a + oz_drow(lane, r);
-        if (i < M && j < N) {
-          const int ex = ea[i], ey = eb[j];
-          const double v = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(cd[a][b][r >> 2][r & 3], ex + ey) : 0.0;
-          T* const pc = c + i + (long)j * ldc;
-          *pc = (T)(0 != beta ? fma(alpha, v, beta * (double)*pc) : alpha * v);
-        }
+        const int j = j0 + 128 * wj + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        out[(long)j * Mp + i] = acc[x][y][r];
       }
+    }
+  }
+}
+
+/* first item of every level (levels 0 .. 15), zb[L + 1] - zb[L] items per level */
+struct oz_levels_t {
+  int zb[17];
+};
+
+/*
+ * C(i, j) = fma(alpha, 2^(ea_i + eb_j) cd, beta C(i, j)) with cd = sum over levels L = cutoff .. 0 of
+ * 2^(-7(L+2)) S_L, S_L the exact (int64) sum of the level's int32 partials.
+ */
+template <typename T>
+__global__ void __launch_bounds__(256) oz_recombine(const int* __restrict__ c32, oz_levels_t lv, int cutoff, int Mp,
+  int Np, const int* __restrict__ ea, const int* __restrict__ eb, int M, int N, double alpha, double beta,
+  T* __restrict__ c, int ldc) {
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < (long)M * N; x += (long)gridDim.x * 256) {
+    const int i = (int)(x % M), j = (int)(x / M);
+    const long o = (long)j * Mp + i;
+    double cd = 0;
+    for (int L = cutoff; L >= 0; --L) {
+      long long sum = 0;
+      for (int z = lv.zb[L]; z < lv.zb[L + 1]; ++z) sum += c32[(long)z * Mp * Np + o];
+      cd += ldexp(1.0, -7 * (L + 2)) * (double)sum;
+    }
+    {
+      const int ex = ea[i], ey = eb[j];
+      const double v = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(cd, ex + ey) : 0.0;
+      T* const pc = c + i + (long)j * ldc;
+      *pc = (T)(0 != beta ? fma(alpha, v, beta * (double)*pc) : alpha * v);
     }
   }
 }
@@ -418,7 +447,7 @@ extern "C" int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int
   int* e, int* occ, signed char* planes, void* hip_stream) {
   const hipStream_t st = (hipStream_t)hip_stream;
   const dim3 grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
-  if (0 != Rp % GT || 0 != Kp % GK || Rp < R || Kp < K || 0 >= Kp || 1 > s || 8 < s) return LXS_ELIMIT;
+  if (0 != Rp % LT || 0 != Kp % LK || Rp < R || Kp < K || 0 >= Kp || 1 > s || 8 < s) return LXS_ELIMIT;
   if (!((1 == rs) || (1 == ks))) return LXS_ELIMIT;
   oz_init<<<(Rp + 255) / 256, 256, 0, st>>>(e, Rp, occ);
   if (3 == dtype) {
@@ -435,19 +464,42 @@ extern "C" int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int
   return (int)hipGetLastError();
 }
 
-extern "C" int lxs_oz_gemm(int dtype, const signed char* pa, const signed char* pb, const int* ea, const int* eb, const int* occa,
-  const int* occb, int s, int cutoff, int flush, int Mp, int Np, int Kp, int M, int N, double alpha, double beta,
-  void* c, int ldc, void* hip_stream) {
-  const dim3 grid((unsigned int)(Mp / GT), (unsigned int)(Np / GT));
-  const hipStream_t st = (hipStream_t)hip_stream;
-  if (0 != Mp % GT || 0 != Np % GT || 0 != Kp % GK || 0 >= Kp || Mp < M || Np < N) return LXS_ELIMIT;
-  if (1 > s || 8 < s || 0 > cutoff || 2 * (s - 1) < cutoff || 1 > flush || 2080 < flush) return LXS_ELIMIT;
+/* items: host array of nitems x {level, u0, u1, pad}; partial z of item z at c32 + z * Mp * Np;
+   planes row-padded to a multiple of 256 and k-padded to a multiple of 128 */
+extern "C" int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, const int* occa, const int* occb,
+  const int* items, int nitems, int s, int Mp, int Np, int Kp, int* c32, void* hip_stream) {
+  if (0 != Mp % LT || 0 != Np % LT || 0 != Kp % LK || 0 >= Kp || 1 > s || 8 < s || 0 >= nitems) return LXS_ELIMIT;
+  for (int z0 = 0; z0 < nitems; z0 += OZ_ITEMS) {
+    const int nz = (nitems - z0 < OZ_ITEMS ? nitems - z0 : OZ_ITEMS);
+    oz_items_t arg;
+    for (int z = 0; z < OZ_ITEMS; ++z) {
+      const int* const src = items + 4 * (z0 + (z < nz ? z : 0));
+      arg.it[z].level = src[0];
+      arg.it[z].u0 = src[1];
+      arg.it[z].u1 = src[2];
+      arg.it[z].pad = 0;
+    }
+    oz_level_gemm<<<dim3((unsigned int)(Mp / LT), (unsigned int)(Np / LT), (unsigned int)nz), 256, 0,
+      (hipStream_t)hip_stream>>>(pa, pb, occa, occb, arg, s, Mp, Np, Kp, c32 + (long)z0 * Mp * Np);
+  }
+  return (int)hipGetLastError();
+}
+
+/* zb: first item per level, cutoff + 2 entries */
+extern "C" int lxs_oz_recombine(int dtype, const int* c32, const int* zb, int cutoff, int Mp, int Np, const int* ea,
+  const int* eb, int M, int N, double alpha, double beta, void* c, int ldc, void* hip_stream) {
+  oz_levels_t lv;
+  const long blocks = ((long)M * N + 255) / 256;
+  const unsigned int grid = (unsigned int)(blocks < 4096 ? blocks : 4096);
+  if (0 > cutoff || 14 < cutoff || Mp < M || Np < N) return LXS_ELIMIT;
+  if (0 >= M || 0 >= N) return 0;
+  for (int L = 0; L < 17; ++L) lv.zb[L] = (L <= cutoff + 1 ? zb[L] : zb[cutoff + 1]);
   if (3 == dtype) {
-    oz_gemm<double><<<grid, 256, 0, st>>>(pa, pb, ea, eb, occa, occb, s, cutoff, flush, Mp, Np, Kp, M, N, alpha, beta,
+    oz_recombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>(c32, lv, cutoff, Mp, Np, ea, eb, M, N, alpha, beta,
       (double*)c, ldc);
   }
   else if (1 == dtype) {
-    oz_gemm<float><<<grid, 256, 0, st>>>(pa, pb, ea, eb, occa, occb, s, cutoff, flush, Mp, Np, Kp, M, N, alpha, beta,
+    oz_recombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>(c32, lv, cutoff, Mp, Np, ea, eb, M, N, alpha, beta,
       (float*)c, ldc);
   }
   else {

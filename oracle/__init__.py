@@ -50,7 +50,7 @@ def lib():
         L.oracle_oz_digits.restype = i
         L.oracle_oz_digits_ref.argtypes = [i, vp, i, vp, vp]
         L.oracle_oz_digits_ref.restype = i
-        L.oracle_ozaki_gemm.argtypes = [i, ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, i, i, i, i, i]
+        L.oracle_ozaki_gemm.argtypes = [i, ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, i, i, i, i]
         L.oracle_ozaki_gemm.restype = i
         L.oracle_mlp_loss_grad.argtypes = [i, vp, vp, vp, vp, vp, vp, i, i, vp, vp, i]
         L.oracle_mlp_loss_grad.restype = i
@@ -177,8 +177,7 @@ def oz_digits_ref(v, s):
     return ex.value, out
 
 
-def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, trim=0, flush=2080, maxk=0,
-               plant=OZ_PLANT_NONE):
+def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, trim=0, maxk=0, plant=OZ_PLANT_NONE):
     """Ozaki scheme I emulated with the device's operation order; returns the updated copy of c
     (column-major flat, float64 or float32 like a)."""
     a = np.ascontiguousarray(a)
@@ -186,7 +185,7 @@ def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, 
     out = np.array(c, dtype=a.dtype, copy=True, order="C").reshape(-1)
     use_double = 1 if a.dtype == np.float64 else 0
     rc = lib().oracle_ozaki_gemm(use_double, transa.encode(), transb.encode(), m, n, k, alpha, a.ctypes.data, lda,
-                                 b.ctypes.data, ldb, beta, out.ctypes.data, ldc, s, trim, flush, maxk, plant)
+                                 b.ctypes.data, ldb, beta, out.ctypes.data, ldc, s, trim, maxk, plant)
     assert rc == 0
     return out
 

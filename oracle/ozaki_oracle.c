@@ -15,10 +15,9 @@
  *                         remain and this formulation keeps 7 bits (the next 3 come from elements
  *                         below the row maximum), so |d_7| >> 3 equals the reference's digit.
  *   oracle_ozaki_gemm     the whole emulated GEMM with the device's operation order: for every
- *                         level L = cutoff..0 the pairs (p, L-p) whose planes hold a nonzero digit,
- *                         exact integer sums over k-blocks of 64 folded into FP64 as
- *                         cd += 2^(-7(L+2)) * sum after every `flush` k-blocks and at the level's
- *                         end, then C = fma(alpha, ldexp(cd, ea + eb), beta C) (alpha * v if beta = 0).
+ *                         level L = cutoff..0 the exact integer sum S_L over the pairs (p, L-p) whose
+ *                         planes hold a nonzero digit, cd += 2^(-7(L+2)) * S_L in FP64, then
+ *                         C = fma(alpha, ldexp(cd, ea + eb), beta C) (alpha * v if beta = 0).
  *                         Its results are therefore bit-identical to the HIP path's.
  * Zero, subnormal and non-finite inputs count as zero (ozaki_common.cl:784-805).
  *
@@ -127,17 +126,17 @@ int oracle_oz_digits_ref(int n, const double* v, int s, signed char* out, int* m
 
 /*
  * C = alpha op(A) op(B) + beta C emulated with s slices (column-major; use_double selects double
- * or float for A, B and C). trim: levels dropped (cutoff 2(s-1) - trim). flush: k-blocks per int
- * fold (1..2080). maxk: K per preprocessing pass (0: all). c is updated in place.
+ * or float for A, B and C). trim: levels dropped (cutoff 2(s-1) - trim). maxk: K per preprocessing
+ * pass (0: all). c is updated in place.
  */
 int oracle_ozaki_gemm(int use_double, char transa, char transb, int M, int N, int K, double alpha, const void* a, int lda,
-  const void* b, int ldb, double beta, void* c, int ldc, int s, int trim, int flush, int maxk, int plant) {
+  const void* b, int ldb, double beta, void* c, int ldc, int s, int trim, int maxk, int plant) {
   const int ta = ('N' == transa || 'n' == transa) ? 0 : 1;
   const int tb = (OZ_PLANT_TRANSB == plant || 'N' == transb || 'n' == transb) ? 0 : 1;
   const int cutoff = 2 * (s - 1) - trim;
   const int kpass = (0 < maxk && maxk < K) ? maxk : K;
   int k0;
-  if (0 > M || 0 > N || 0 > K || 1 > s || 8 < s || 0 > cutoff || 1 > flush) return 1;
+  if (0 > M || 0 > N || 0 > K || 1 > s || 8 < s || 0 > cutoff) return 1;
   if (0 == M || 0 == N) return 0;
   if (0 == K || 0 == alpha) {
     long x;
@@ -205,14 +204,13 @@ int oracle_ozaki_gemm(int use_double, char transa, char transb, int M, int N, in
         for (L = cutoff; L >= 0; --L) {
           const double scale = ldexp(1.0, -7 * (L + (OZ_PLANT_WEIGHT == plant ? 1 : 2)));
           int64_t acc = 0;
-          int p, cnt = 0, any = 0;
+          int p;
           if (OZ_PLANT_DROPLEVEL == plant && 1 == L) continue;
           for (p = (L - s + 1 > 0 ? L - s + 1 : 0); p <= (L < s - 1 ? L : s - 1); ++p) {
             const signed char* ra;
             const signed char* rb;
             int kb;
             if (!((oa >> p) & (ob >> (L - p)) & 1u)) continue;
-            any = 1;
             ra = da + ((long)p * M + i) * Kp;
             rb = db + ((long)(L - p) * N + j) * Kp;
             for (kb = 0; kb < nkb; ++kb) {
@@ -220,14 +218,9 @@ int oracle_ozaki_gemm(int use_double, char transa, char transb, int M, int N, in
               int k;
               for (k = kb * OZ_KB; k < kb * OZ_KB + OZ_KB; ++k) blk += (int32_t)ra[k] * (int32_t)rb[k];
               acc += blk;
-              if (++cnt == flush) {
-                cd += scale * (double)acc;
-                acc = 0;
-                cnt = 0;
-              }
             }
           }
-          if (any && 0 != cnt) cd += scale * (double)acc;
+          cd += scale * (double)acc;
         }
         v = (OZ_NONE != ea[i] && OZ_NONE != eb[j]) ? ldexp(cd, ea[i] + eb[j]) : 0.0;
         if (use_double) {

@@ -2,7 +2,8 @@
 
 Bar: the HIP path equals the oracle's restatement (oracle/ozaki_oracle.c, same operation order) bit
 for bit, for every trans combination, ragged and tile-aligned shapes, slice counts, trims, alpha/beta,
-FP32, K passes, small int32 fold intervals, host operands, and zero/subnormal/non-finite inputs.
+FP32, K passes, small work items (many int32 partials per level), host operands, and
+zero/subnormal/non-finite inputs.
 Accuracy against the double-double GEMM is stated with the slice count: per-row/column exponents
 (mantissas uniform [1,2), exponents uniform in [-e, e]) give a normwise error <= 1e-14 with 7 slices
 over the triangle and <= 1e-15 with 8. The native FP64 matrix-core GEMM (the comparator) and the
@@ -53,9 +54,9 @@ def _run(oz, stream, ta, tb, m, n, k, alpha, A, B, beta, C0, dtype=np.float64, d
     return out, (a, lda, b, ldb, c)
 
 
-def _expect(ta, tb, m, n, k, alpha, args, beta, s, trim=0, flush=2080, maxk=0):
+def _expect(ta, tb, m, n, k, alpha, args, beta, s, trim=0, maxk=0):
     a, lda, b, ldb, c = args
-    return oracle.ozaki_gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, m, s, trim=trim, flush=flush, maxk=maxk)
+    return oracle.ozaki_gemm(ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, m, s, trim=trim, maxk=maxk)
 
 
 def _mixed(rng, m, k):
@@ -106,7 +107,8 @@ def test_bitexact_fp32(stream):
     assert out.dtype == np.float32 and np.array_equal(out, exp)
 
 
-def test_bitexact_k_passes_and_small_fold_interval(stream, monkeypatch):
+def test_bitexact_k_passes_and_small_work_items(stream, monkeypatch):
+    """K in passes of 300, work items of 3 k-blocks: level sums stay exact, so the result is the same"""
     m, n, k = 130, 70, 700
     rng = np.random.default_rng(77)
     A, B, C0 = _mixed(rng, m, k), _mixed(rng, k, n), rng.uniform(-1, 1, (m, n))
@@ -117,7 +119,7 @@ def test_bitexact_k_passes_and_small_fold_interval(stream, monkeypatch):
         out, args = _run(oz, stream, "T", "N", m, n, k, -1.0, A, B, 2.0, C0)
     finally:
         oz.close()
-    assert np.array_equal(out, _expect("T", "N", m, n, k, -1.0, args, 2.0, 6, trim=2, flush=3, maxk=300))
+    assert np.array_equal(out, _expect("T", "N", m, n, k, -1.0, args, 2.0, 6, trim=2, maxk=300))
 
 
 def test_bitexact_host_operands(stream):
