@@ -46,6 +46,9 @@ typedef enum libsmm_acc_config_source_t {
 /** configuration libsmm_acc_process would use for this shape; *source may be NULL */
 LIBXSTREAM_API int libsmm_acc_config_get(libsmm_acc_data_t datatype, int m, int n, int k, libsmm_acc_smm_config_t* config,
   int* source);
+/** the built-in heuristic's configuration for this shape (ignores the registry and the predictor) */
+LIBXSTREAM_API int libsmm_acc_config_heuristic(libsmm_acc_data_t datatype, int m, int n, int k,
+  libsmm_acc_smm_config_t* config);
 /** register a configuration for this shape (replaces any existing one) */
 LIBXSTREAM_API int libsmm_acc_config_set(libsmm_acc_data_t datatype, int m, int n, int k,
   const libsmm_acc_smm_config_t* config);

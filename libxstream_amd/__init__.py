@@ -109,6 +109,7 @@ def _declare(L):
         "libsmm_acc_transpose": (i, [vp, i, i, vp, i, i, i, i, vp]),
         "libsmm_acc_process_config": (i, [ctypes.POINTER(SmmConfig), vp, i, i, vp, vp, vp, i, i, i, vp]),
         "libsmm_acc_config_get": (i, [i, i, i, i, ctypes.POINTER(SmmConfig), pi]),
+        "libsmm_acc_config_heuristic": (i, [i, i, i, i, ctypes.POINTER(SmmConfig)]),
         "libsmm_acc_config_set": (i, [i, i, i, i, ctypes.POINTER(SmmConfig)]),
         "libsmm_acc_config_valid": (i, [i, i, i, i, ctypes.POINTER(SmmConfig)]),
         "libsmm_acc_kernel_variants": (i, [i, pi, i]),
@@ -327,6 +328,13 @@ def config_get(tid, m, n, k):
 
 def config_set(tid, m, n, k, cfg):
     check(lib().libsmm_acc_config_set(tid, m, n, k, ctypes.byref(cfg)), "config_set")
+
+
+def default_config(tid, m, n, k):
+    """the built-in heuristic's configuration (no registry, no predictor)"""
+    cfg = SmmConfig()
+    check(lib().libsmm_acc_config_heuristic(tid, m, n, k, ctypes.byref(cfg)), "config_heuristic")
+    return cfg
 
 
 def config_valid(tid, m, n, k, cfg):

@@ -52,6 +52,8 @@ void lxs_profile_finalize(void);
 /** learned predictor slot (mlp.c): EXIT_SUCCESS and a configuration when a confident model is installed */
 int lxs_learned_predict(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config);
 void lxs_learned_finalize(void);
+/** load a model file and install it as the predictor, owned by the library until finalize */
+void lxs_learned_load_default(const char* path);
 
 /** LIBSMM registry lifetime (smm.c) */
 int lxs_smm_init(void);

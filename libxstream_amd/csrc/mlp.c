@@ -41,7 +41,9 @@ struct libsmm_acc_mlp_t {
 };
 
 static libsmm_acc_mlp_t* lxs_learned = NULL;
+static libsmm_acc_mlp_t* lxs_learned_owned = NULL; /* the shipped model, loaded at init */
 static double lxs_learned_conf = 0.0;
+#define MLP_DEFAULT_CONF 0.0 /* confidence threshold of the shipped model (LIBXSTREAM_LEARNED_CONF) */
 static pthread_mutex_t lxs_learned_lock = PTHREAD_MUTEX_INITIALIZER;
 
 static int mlp_nout(void) {
@@ -133,6 +135,7 @@ LIBXSTREAM_API int libsmm_acc_mlp_destroy(libsmm_acc_mlp_t* model) {
   if (NULL != model) {
     pthread_mutex_lock(&lxs_learned_lock);
     if (lxs_learned == model) lxs_learned = NULL;
+    if (lxs_learned_owned == model) lxs_learned_owned = NULL;
     pthread_mutex_unlock(&lxs_learned_lock);
     mlp_free_device(model);
     free(model->params);
@@ -507,8 +510,26 @@ int lxs_learned_predict(int datatype, int m, int n, int k, libsmm_acc_smm_config
   return result;
 }
 
+void lxs_learned_load_default(const char* path) {
+  libsmm_acc_mlp_t* model = NULL;
+  if (EXIT_SUCCESS == libsmm_acc_mlp_load(&model, path)) {
+    const char* const env = getenv("LIBXSTREAM_LEARNED_CONF");
+    pthread_mutex_lock(&lxs_learned_lock);
+    lxs_learned = lxs_learned_owned = model;
+    lxs_learned_conf = (NULL != env && '\0' != *env) ? atof(env) : MLP_DEFAULT_CONF;
+    pthread_mutex_unlock(&lxs_learned_lock);
+  }
+  else {
+    (void)libxstream_error_consume(NULL, 0);
+  }
+}
+
 void lxs_learned_finalize(void) {
+  libsmm_acc_mlp_t* owned;
   pthread_mutex_lock(&lxs_learned_lock);
   lxs_learned = NULL;
+  owned = lxs_learned_owned;
+  lxs_learned_owned = NULL;
   pthread_mutex_unlock(&lxs_learned_lock);
+  if (NULL != owned) libsmm_acc_mlp_destroy(owned);
 }

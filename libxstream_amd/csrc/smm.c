@@ -266,6 +266,14 @@ static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_co
   if (NULL != source) *source = src;
 }
 
+LIBXSTREAM_API int libsmm_acc_config_heuristic(libsmm_acc_data_t datatype, int m, int n, int k,
+  libsmm_acc_smm_config_t* config) {
+  if (NULL == config || 0 == lxs_typesize(datatype) || 0 >= m || 0 >= n || 0 >= k) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_config_heuristic: invalid argument");
+  }
+  return lxs_smm_default((int)datatype, m, n, k, config);
+}
+
 LIBXSTREAM_API int libsmm_acc_config_get(libsmm_acc_data_t datatype, int m, int n, int k, libsmm_acc_smm_config_t* config,
   int* source) {
   if (NULL == config || 0 == lxs_typesize(datatype) || 0 >= m || 0 >= n || 0 >= k) {
@@ -464,12 +472,13 @@ LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* devi
 }
 
 /* directory of this library (tuned parameters ship next to it) */
-static int lxs_smm_default_params(char* path, size_t size) {
+/* path of a file shipped in params/ next to the loaded library */
+static int lxs_smm_default_params(char* path, size_t size, const char* file) {
   Dl_info info;
   if (0 != dladdr((void*)&lxs_smm_default_params, &info) && NULL != info.dli_fname) {
     const char* const slash = strrchr(info.dli_fname, '/');
     const int len = (NULL != slash ? (int)(slash - info.dli_fname) : 0);
-    snprintf(path, size, "%.*s%sparams/tune_multiply_MI355X.csv", len, info.dli_fname, 0 < len ? "/" : "");
+    snprintf(path, size, "%.*s%sparams/%s", len, info.dli_fname, 0 < len ? "/" : "", file);
     return EXIT_SUCCESS;
   }
   return EXIT_FAILURE;
@@ -479,7 +488,9 @@ int lxs_smm_init(void) {
   const char* const env = getenv("OPENCL_LIBSMM_SMM_PARAMS");
   char path[4096];
   if (NULL == env) {
-    if (EXIT_SUCCESS == lxs_smm_default_params(path, sizeof(path))) (void)libsmm_acc_params_load(path);
+    if (EXIT_SUCCESS == lxs_smm_default_params(path, sizeof(path), "tune_multiply_MI355X.csv")) {
+      (void)libsmm_acc_params_load(path);
+    }
   }
   else if ('0' != *env && '\0' != *env) {
     if (0 > libsmm_acc_params_load(env)) {
@@ -495,6 +506,17 @@ int lxs_smm_init(void) {
       }
       else if (0 != lxs_cfg.verbosity) {
         fprintf(stderr, "WARN LIBSMM: failed to open parameter file!\n");
+      }
+    }
+  }
+  { /* the shipped predictor serves shapes without a tuned configuration (LIBXSTREAM_LEARNED=0: off) */
+    const char* const learned = getenv("LIBXSTREAM_LEARNED");
+    if ((NULL == learned || '0' != *learned) &&
+        EXIT_SUCCESS == lxs_smm_default_params(path, sizeof(path), "mlp_MI355X.txt")) {
+      FILE* const f = fopen(path, "r");
+      if (NULL != f) {
+        fclose(f);
+        lxs_learned_load_default(path);
       }
     }
   }

@@ -187,9 +187,9 @@ def test_params_load_reference_and_roundtrip(tmp_path, monkeypatch):
                                    (80, 80, 500), (7, 11, 3), (1, 1, 1), (17, 33, 2)])
 @pytest.mark.parametrize("tid", [lx.REAL_8, lx.REAL_4])
 def test_default_config_valid(shape, tid):
-    cfg, src = lx.config_get(tid, *shape)
-    assert src == 0
-    assert lx.config_valid(tid, *shape, cfg)
+    assert lx.config_valid(tid, *shape, lx.default_config(tid, *shape))
+    cfg, src = lx.config_get(tid, *shape)  # tuned row, learned prediction (with a device) or the heuristic
+    assert src in (0, 1, 3) and lx.config_valid(tid, *shape, cfg)
 
 
 def test_config_validation():
