@@ -107,37 +107,43 @@ LIBXSTREAM_API int libsmm_acc_batch_info(const libsmm_acc_batch_t* batch, int* n
 LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch);
 
 /**
- * Learned predictor (FP32 MLP trained and evaluated on the GPU by the library's own kernels).
- * Where the reference maps an untuned shape to the parameters of the nearest tuned triplet
- * (samples/smm/smm_kernel.c:116-148, :206-213), the network scores every valid compiled
- * configuration. Heads (libsmm_acc_mlp_heads): softmax classifiers over variant, tm, tn, nt, kt,
- * bs, flush and a regression of log2 GFLOP/s. Features: libsmm_acc_mlp_features (8 floats).
- * Targets for training: one row of nout floats per sample, soft class distributions per head
- * (summing to 1, or all zero to skip the head) and the log2 GFLOP/s value.
+ * Learned predictor: an ensemble of FP32 MLPs trained and evaluated on the GPU by the library's own
+ * kernels. Where the reference maps an untuned shape to the parameters of the nearest tuned triplet
+ * (samples/smm/smm_kernel.c:116-148, :206-213), every network maps (8 shape features: log m, n, k,
+ * type, log bytes of the A/B/C blocks, log flops; one-hot encoding of a configuration's variant, tm,
+ * tn, nt, kt, bs, flush) through hidden ReLU layers to two outputs: the configuration's throughput
+ * relative to the shape's best and log2 of its GFLOP/s. Prediction scores every valid compiled
+ * configuration with the ensemble mean and takes the best.
  */
 typedef struct libsmm_acc_mlp_t libsmm_acc_mlp_t;
-/** hidden: width of the hidden layers, nhidden: their number; weights seeded deterministically */
-LIBXSTREAM_API int libsmm_acc_mlp_create(libsmm_acc_mlp_t** model, int hidden, int nhidden, unsigned long long seed);
+/** nnets members of nhidden layers of width hidden; weights seeded deterministically */
+LIBXSTREAM_API int libsmm_acc_mlp_create(libsmm_acc_mlp_t** model, int nnets, int hidden, int nhidden,
+  unsigned long long seed);
 LIBXSTREAM_API int libsmm_acc_mlp_destroy(libsmm_acc_mlp_t* model);
-/** head layout: seg[h] = first output of head h (seg[nheads] = outputs), values[] = class values
- *  (-1 for the regression head); returns the number of heads */
-LIBXSTREAM_API int libsmm_acc_mlp_heads(int* seg, int* values, int cap);
-LIBXSTREAM_API int libsmm_acc_mlp_features(libsmm_acc_data_t datatype, int m, int n, int k, float* features);
-/** number of parameters; copies up to cap of them to params when not NULL */
-LIBXSTREAM_API int libsmm_acc_mlp_params(libsmm_acc_mlp_t* model, float* params, int cap);
-LIBXSTREAM_API int libsmm_acc_mlp_set_params(libsmm_acc_mlp_t* model, const float* params, int n);
-/** one forward and backward pass on the GPU: gradient of the mean loss (nparams floats) and the loss;
- *  head_weights (one per head) may be NULL (all 1) */
-LIBXSTREAM_API int libsmm_acc_mlp_gradient(libsmm_acc_mlp_t* model, const float* x, const float* targets,
-  const float* head_weights, int batch, float* grad, double* loss);
-/** full-batch Adam on the GPU for epochs steps; loss of the final parameters when loss != NULL */
+/** layout of the configuration encoding: seg[p] = first one-hot input of parameter p (after the 8 shape
+ *  features; seg[7] = end), values[] = class values; returns the number of inputs */
+LIBXSTREAM_API int libsmm_acc_mlp_encoding(int* seg, int* values, int cap);
+/** inputs of one (shape, configuration) row; config NULL encodes the shape only */
+LIBXSTREAM_API int libsmm_acc_mlp_features(libsmm_acc_data_t datatype, int m, int n, int k,
+  const libsmm_acc_smm_config_t* config, float* features);
+/** members, layers and widths (width[0] = inputs); returns the parameters per member */
+LIBXSTREAM_API int libsmm_acc_mlp_info(const libsmm_acc_mlp_t* model, int* nnets, int* nlayers, int* width, int cap);
+/** parameters of member net (copies up to cap when params is not NULL); returns their number */
+LIBXSTREAM_API int libsmm_acc_mlp_params(libsmm_acc_mlp_t* model, int net, float* params, int cap);
+LIBXSTREAM_API int libsmm_acc_mlp_set_params(libsmm_acc_mlp_t* model, int net, const float* params, int n);
+/** one forward and backward pass of member net on the GPU: gradient of the mean weighted squared error
+ *  (out_weights per output, NULL = 1) and the loss */
+LIBXSTREAM_API int libsmm_acc_mlp_gradient(libsmm_acc_mlp_t* model, int net, const float* x, const float* targets,
+  const float* out_weights, int batch, float* grad, double* loss);
+/** full-batch Adam on the GPU for epochs steps, every member in turn; mean final loss when loss != NULL */
 LIBXSTREAM_API int libsmm_acc_mlp_train(libsmm_acc_mlp_t* model, const float* x, const float* targets,
-  const float* head_weights, int batch, int epochs, double lr, double weight_decay, double* loss);
-/** inference on the GPU: out[batch][outputs] */
+  const float* out_weights, int batch, int epochs, double lr, double weight_decay, double* loss);
+/** inference on the GPU: ensemble mean out[batch][2] */
 LIBXSTREAM_API int libsmm_acc_mlp_forward(libsmm_acc_mlp_t* model, const float* x, int batch, float* out);
 LIBXSTREAM_API int libsmm_acc_mlp_save(const libsmm_acc_mlp_t* model, const char* filename);
 LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* filename);
-/** best valid configuration for the shape by the network, with its share of the candidates' probability */
+/** best valid configuration for the shape by the ensemble; confidence = its predicted relative
+ *  throughput (0..1); config->gflops = predicted GFLOP/s */
 LIBXSTREAM_API int libsmm_acc_mlp_predict_config(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
   int k, libsmm_acc_smm_config_t* config, double* confidence);
 /** install (or remove: NULL) the model as predictor for shapes without a tuned configuration; a

@@ -126,13 +126,14 @@ def _declare(L):
         "c_dbcsr_acc_get_ndevices": (i, [pi]),
         "c_dbcsr_acc_set_active_device": (i, [i]),
         "c_calculate_norms": (i, [vp, i, vp, vp, vp, vp]),
-        "libsmm_acc_mlp_create": (i, [pvp, i, i, ctypes.c_ulonglong]),
+        "libsmm_acc_mlp_create": (i, [pvp, i, i, i, ctypes.c_ulonglong]),
         "libsmm_acc_mlp_destroy": (i, [vp]),
-        "libsmm_acc_mlp_heads": (i, [vp, vp, i]),
-        "libsmm_acc_mlp_features": (i, [i, i, i, i, vp]),
-        "libsmm_acc_mlp_params": (i, [vp, vp, i]),
-        "libsmm_acc_mlp_set_params": (i, [vp, vp, i]),
-        "libsmm_acc_mlp_gradient": (i, [vp, vp, vp, vp, i, vp, ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_mlp_encoding": (i, [vp, vp, i]),
+        "libsmm_acc_mlp_features": (i, [i, i, i, i, ctypes.POINTER(SmmConfig), vp]),
+        "libsmm_acc_mlp_info": (i, [vp, pi, pi, vp, i]),
+        "libsmm_acc_mlp_params": (i, [vp, i, vp, i]),
+        "libsmm_acc_mlp_set_params": (i, [vp, i, vp, i]),
+        "libsmm_acc_mlp_gradient": (i, [vp, i, vp, vp, vp, i, vp, ctypes.POINTER(ctypes.c_double)]),
         "libsmm_acc_mlp_train": (i, [vp, vp, vp, vp, i, i, ctypes.c_double, ctypes.c_double,
                                      ctypes.POINTER(ctypes.c_double)]),
         "libsmm_acc_mlp_forward": (i, [vp, vp, i, vp]),
@@ -438,74 +439,79 @@ def dgemm_native(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s
                                    a.ptr, lda, b.ptr, ldb, beta, c.ptr, ldc), "ozaki_dgemm_native")
 
 
-def mlp_heads():
-    """(segment starts incl. the end, class values per output: -1 for the regression output)"""
+def mlp_encoding():
+    """(segment starts of the configuration one-hot inputs incl. the end, class values, number of inputs)"""
     seg = np.zeros(64, dtype=np.int32)
     vals = np.zeros(64, dtype=np.int32)
-    nh = lib().libsmm_acc_mlp_heads(seg.ctypes.data, vals.ctypes.data, 64)
-    seg = seg[:nh + 1].copy()
-    return seg, vals[:seg[-1]].copy()
+    nin = lib().libsmm_acc_mlp_encoding(seg.ctypes.data, vals.ctypes.data, 64)
+    seg = seg[:8].copy()
+    return seg, vals[:seg[-1]].copy(), nin
 
 
-def mlp_features(tid, m, n, k):
-    f = np.zeros(8, dtype=np.float32)
-    check(lib().libsmm_acc_mlp_features(tid, m, n, k, f.ctypes.data), "libsmm_acc_mlp_features")
+def mlp_features(tid, m, n, k, cfg=None):
+    """network inputs of one (shape, configuration) row (cfg None: the shape only)"""
+    nin = lib().libsmm_acc_mlp_encoding(None, None, 0)
+    f = np.zeros(nin, dtype=np.float32)
+    check(lib().libsmm_acc_mlp_features(tid, m, n, k, ctypes.byref(cfg) if cfg is not None else None, f.ctypes.data),
+          "libsmm_acc_mlp_features")
     return f
 
 
 class Mlp:
-    """The learned predictor's network (libsmm_acc_mlp_*): training and inference on the GPU."""
+    """The learned predictor's network ensemble (libsmm_acc_mlp_*): training and inference on the GPU."""
 
-    def __init__(self, hidden=64, nhidden=3, seed=1, path=None):
+    def __init__(self, nnets=4, hidden=64, nhidden=3, seed=1, path=None):
         h = ctypes.c_void_p()
         if path is not None:
             check(lib().libsmm_acc_mlp_load(ctypes.byref(h), os.fsencode(path)), "libsmm_acc_mlp_load")
         else:
-            check(lib().libsmm_acc_mlp_create(ctypes.byref(h), hidden, nhidden, seed), "libsmm_acc_mlp_create")
+            check(lib().libsmm_acc_mlp_create(ctypes.byref(h), nnets, hidden, nhidden, seed), "libsmm_acc_mlp_create")
         self.handle = h
 
-    @property
-    def nparams(self):
-        return lib().libsmm_acc_mlp_params(self.handle, None, 0)
+    def info(self):
+        nn, nl = ctypes.c_int(0), ctypes.c_int(0)
+        w = np.zeros(16, dtype=np.int32)
+        npar = lib().libsmm_acc_mlp_info(self.handle, ctypes.byref(nn), ctypes.byref(nl), w.ctypes.data, 16)
+        return {"nnets": nn.value, "nlayers": nl.value, "width": w[:nl.value + 1].tolist(), "nparams": npar}
 
-    def params(self):
-        p = np.zeros(self.nparams, dtype=np.float32)
-        lib().libsmm_acc_mlp_params(self.handle, p.ctypes.data, p.size)
+    def params(self, net=0):
+        n = lib().libsmm_acc_mlp_params(self.handle, net, None, 0)
+        p = np.zeros(n, dtype=np.float32)
+        lib().libsmm_acc_mlp_params(self.handle, net, p.ctypes.data, n)
         return p
 
-    def set_params(self, p):
+    def set_params(self, p, net=0):
         p = np.ascontiguousarray(p, dtype=np.float32)
-        check(lib().libsmm_acc_mlp_set_params(self.handle, p.ctypes.data, p.size), "libsmm_acc_mlp_set_params")
+        check(lib().libsmm_acc_mlp_set_params(self.handle, net, p.ctypes.data, p.size), "libsmm_acc_mlp_set_params")
 
     @staticmethod
-    def _hw(head_weights):
-        return None if head_weights is None else np.ascontiguousarray(head_weights, dtype=np.float32)
+    def _w(out_weights):
+        return None if out_weights is None else np.ascontiguousarray(out_weights, dtype=np.float32)
 
-    def gradient(self, x, t, head_weights=None):
+    def gradient(self, x, t, out_weights=None, net=0):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
-        hw = self._hw(head_weights)
-        g = np.zeros(self.nparams, dtype=np.float32)
+        w = self._w(out_weights)
+        g = np.zeros(lib().libsmm_acc_mlp_params(self.handle, net, None, 0), dtype=np.float32)
         loss = ctypes.c_double(0)
-        check(lib().libsmm_acc_mlp_gradient(self.handle, x.ctypes.data, t.ctypes.data,
-                                            None if hw is None else hw.ctypes.data, x.shape[0], g.ctypes.data,
+        check(lib().libsmm_acc_mlp_gradient(self.handle, net, x.ctypes.data, t.ctypes.data,
+                                            None if w is None else w.ctypes.data, x.shape[0], g.ctypes.data,
                                             ctypes.byref(loss)), "libsmm_acc_mlp_gradient")
         return loss.value, g
 
-    def train(self, x, t, epochs, lr=1e-3, weight_decay=0.0, head_weights=None):
+    def train(self, x, t, epochs, lr=3e-3, weight_decay=1e-5, out_weights=None):
         x = np.ascontiguousarray(x, dtype=np.float32)
         t = np.ascontiguousarray(t, dtype=np.float32)
-        hw = self._hw(head_weights)
+        w = self._w(out_weights)
         loss = ctypes.c_double(0)
-        check(lib().libsmm_acc_mlp_train(self.handle, x.ctypes.data, t.ctypes.data,
-                                         None if hw is None else hw.ctypes.data, x.shape[0], epochs, lr, weight_decay,
-                                         ctypes.byref(loss)), "libsmm_acc_mlp_train")
+        check(lib().libsmm_acc_mlp_train(self.handle, x.ctypes.data, t.ctypes.data, None if w is None else w.ctypes.data,
+                                         x.shape[0], epochs, lr, weight_decay, ctypes.byref(loss)),
+              "libsmm_acc_mlp_train")
         return loss.value
 
     def forward(self, x):
         x = np.ascontiguousarray(x, dtype=np.float32)
-        seg, _ = mlp_heads()
-        out = np.zeros((x.shape[0], int(seg[-1])), dtype=np.float32)
+        out = np.zeros((x.shape[0], 2), dtype=np.float32)
         check(lib().libsmm_acc_mlp_forward(self.handle, x.ctypes.data, x.shape[0], out.ctypes.data),
               "libsmm_acc_mlp_forward")
         return out

@@ -95,10 +95,12 @@ int lxs_oz_ccombine(int dtype, const void* t1, const void* t2, const void* t3, i
 
 /* learned predictor MLP (mlp.hip): activations [batch][features], weights [out][in] then bias [out] */
 int lxs_mlp_forward(const float* x, const float* w, int batch, int nin, int nout, int relu, float* y, void* hip_stream);
-int lxs_mlp_loss(const float* z, const float* tgt, const int* seg, const float* hw, int nseg, int batch, int nh,
-  float* dz, float* loss, void* hip_stream);
+int lxs_mlp_loss(const float* z, const float* tgt, const float* hw, int batch, int nh, float* dz, float* loss,
+  void* hip_stream);
+/* scratch (floats) of lxs_mlp_backward; layers up to 64 outputs x 64 inputs */
+long lxs_mlp_scratch(int batch, int nin, int nout);
 int lxs_mlp_backward(const float* x, const float* y, const float* w, float* dz, int batch, int nin, int nout, int relu,
-  float* dw, float* dx, void* hip_stream);
+  float* dw, float* dx, float* scratch, void* hip_stream);
 int lxs_mlp_adam(float* p, const float* g, float* m, float* v, int n, float lr, float beta1, float beta2, float eps,
   int step, float wd, void* hip_stream);
 

@@ -1,13 +1,11 @@
 /*
- * FP32 multi-layer perceptron for the learned stack-kernel predictor: forward, loss gradient,
- * backward and Adam update as HIP kernels (gfx950). The network is small (8 -> 64 -> 64 -> 64 -> H),
- * so every kernel maps one thread to one output element and loops over the reduction dimension
- * in a fixed order: results are deterministic run to run.
+ * FP32 multi-layer perceptron of the learned stack-kernel predictor: forward, loss gradient,
+ * backward and Adam update as HIP kernels (gfx950).
  *
  * Layouts: activations row-major [batch][features]; layer weights row-major [out][in] followed by
- * the bias [out]; all layers of a network are packed in one parameter vector.
- * Heads: the output vector is cut into segments; a segment of length 1 is a regression output
- * (squared error), a longer one a softmax classifier (cross-entropy against soft targets).
+ * the bias [out]; the layers of one network are packed in one parameter vector.
+ * Every reduction runs in a fixed order (rows in order within a chunk, chunks in order), so a
+ * training run gives the same parameters every time.
  */
 #include <hip/hip_runtime.h>
 #include "lxs_kernels.h"
@@ -17,9 +15,9 @@ namespace {
 /* y[b][o] = act(sum_i x[b][i] w[o][i] + bias[o]) */
 __global__ void __launch_bounds__(256) mlp_forward(const float* __restrict__ x, const float* __restrict__ w,
   int batch, int nin, int nout, int relu, float* __restrict__ y) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t < batch * nout) {
-    const int b = t / nout, o = t - b * nout;
+  const long t = blockIdx.x * 256L + threadIdx.x;
+  if (t < (long)batch * nout) {
+    const int b = (int)(t / nout), o = (int)(t - (long)b * nout);
     const float* const xr = x + (long)b * nin;
     const float* const wr = w + (long)o * nin;
     float acc = w[(long)nout * nin + o];
@@ -28,79 +26,84 @@ __global__ void __launch_bounds__(256) mlp_forward(const float* __restrict__ x, 
   }
 }
 
-/* loss gradient on the output z[b][h] against targets tgt[b][h]; per-sample loss to loss[b].
-   seg[s] = first column of segment s (nseg + 1 entries), hw[s] = weight of segment s. */
+/* weighted squared error per output column: loss[b] = sum_c hw[c] (z - t)^2, dz = 2 hw (z - t) / batch */
 __global__ void __launch_bounds__(256) mlp_loss(const float* __restrict__ z, const float* __restrict__ tgt,
-  const int* __restrict__ seg, const float* __restrict__ hw, int nseg, int batch, int nh, float* __restrict__ dz,
-  float* __restrict__ loss) {
+  const float* __restrict__ hw, int batch, int nh, float* __restrict__ dz, float* __restrict__ loss) {
   const int b = blockIdx.x * 256 + threadIdx.x;
   if (b < batch) {
-    const float* const zr = z + (long)b * nh;
-    const float* const tr = tgt + (long)b * nh;
-    float* const dr = dz + (long)b * nh;
     const float inv = 1.f / (float)batch;
     float l = 0.f;
-    for (int s = 0; s < nseg; ++s) {
-      const int c0 = seg[s], c1 = seg[s + 1];
-      if (1 == c1 - c0) { /* regression */
-        const float d = zr[c0] - tr[c0];
-        l += hw[s] * d * d;
-        dr[c0] = 2.f * hw[s] * d * inv;
-      }
-      else { /* softmax cross-entropy with soft targets (summing to 1, or all 0 = no label) */
-        float mx = zr[c0], sum = 0.f, tsum = 0.f;
-        for (int c = c0 + 1; c < c1; ++c) mx = fmaxf(mx, zr[c]);
-        for (int c = c0; c < c1; ++c) sum += expf(zr[c] - mx);
-        for (int c = c0; c < c1; ++c) tsum += tr[c];
-        const float lse = mx + logf(sum);
-        for (int c = c0; c < c1; ++c) {
-          const float p = expf(zr[c] - mx) / sum;
-          l -= hw[s] * tr[c] * (zr[c] - lse);
-          dr[c] = hw[s] * (tsum * p - tr[c]) * inv;
-        }
-      }
+    for (int c = 0; c < nh; ++c) {
+      const long o = (long)b * nh + c;
+      const float d = z[o] - tgt[o];
+      l += hw[c] * d * d;
+      dz[o] = 2.f * hw[c] * d * inv;
     }
     loss[b] = l;
   }
 }
 
 /* dz = dy * (y > 0) in place (ReLU layers) */
-__global__ void __launch_bounds__(256) mlp_relu_grad(const float* __restrict__ y, float* __restrict__ dy, int n) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
+__global__ void __launch_bounds__(256) mlp_relu_grad(const float* __restrict__ y, float* __restrict__ dy, long n) {
+  const long t = blockIdx.x * 256L + threadIdx.x;
   if (t < n && !(y[t] > 0.f)) dy[t] = 0.f;
 }
 
-/* dw[o][i] = sum_b dz[b][o] x[b][i]; db[o] = sum_b dz[b][o] */
-__global__ void __launch_bounds__(256) mlp_wgrad(const float* __restrict__ x, const float* __restrict__ dz, int batch,
-  int nin, int nout, float* __restrict__ dw) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t < nout * (nin + 1)) {
+/* partial weight gradients of one chunk of rows, in the parameter layout (weights [out][in], then the
+   bias [out]): part[c][o nin + i] = sum over the chunk of dz[b][o] x[b][i], part[c][nout nin + o] =
+   sum of dz[b][o]; rows walked in order, each thread owns a fixed set of outputs */
+__global__ void __launch_bounds__(256) mlp_wgrad_part(const float* __restrict__ x, const float* __restrict__ dz,
+  int batch, int nin, int nout, int rows_per_chunk, float* __restrict__ part) {
+  constexpr int PER = 17; /* outputs per thread: 256 * 17 >= 64 * 65 */
+  const int np = nout * (nin + 1), b0 = blockIdx.x * rows_per_chunk;
+  const int b1 = (b0 + rows_per_chunk < batch ? b0 + rows_per_chunk : batch);
+  float acc[PER];
+  int oo[PER], ii[PER];
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int p = threadIdx.x + 256 * q;
+    const bool bias = (p >= nout * nin);
+    oo[q] = (p < np ? (bias ? p - nout * nin : p / nin) : 0);
+    ii[q] = (p < np && !bias ? p - (p / nin) * nin : nin); /* nin: the bias (x = 1) */
+    acc[q] = 0.f;
+  }
+  for (int b = b0; b < b1; ++b) {
+    const float* const xr = x + (long)b * nin;
+    const float* const dr = dz + (long)b * nout;
+#pragma unroll
+    for (int q = 0; q < PER; ++q) acc[q] = fmaf(dr[oo[q]], (ii[q] < nin ? xr[ii[q]] : 1.f), acc[q]);
+  }
+#pragma unroll
+  for (int q = 0; q < PER; ++q) {
+    const int p = threadIdx.x + 256 * q;
+    if (p < np) part[(long)blockIdx.x * np + p] = acc[q];
+  }
+}
+
+/* dw[p] = sum over chunks of part[c][p], chunks in order */
+__global__ void __launch_bounds__(256) mlp_wgrad_sum(const float* __restrict__ part, int nchunks, int np,
+  float* __restrict__ dw) {
+  const int p = blockIdx.x * 256 + threadIdx.x;
+  if (p < np) {
     float acc = 0.f;
-    if (t < nout * nin) {
-      const int o = t / nin, i = t - o * nin;
-      for (int b = 0; b < batch; ++b) acc = fmaf(dz[(long)b * nout + o], x[(long)b * nin + i], acc);
-    }
-    else {
-      const int o = t - nout * nin;
-      for (int b = 0; b < batch; ++b) acc += dz[(long)b * nout + o];
-    }
-    dw[t] = acc;
+    for (int c = 0; c < nchunks; ++c) acc += part[(long)c * np + p];
+    dw[p] = acc;
   }
 }
 
 /* dx[b][i] = sum_o dz[b][o] w[o][i] */
 __global__ void __launch_bounds__(256) mlp_xgrad(const float* __restrict__ dz, const float* __restrict__ w, int batch,
   int nin, int nout, float* __restrict__ dx) {
-  const int t = blockIdx.x * 256 + threadIdx.x;
-  if (t < batch * nin) {
-    const int b = t / nin, i = t - b * nin;
+  const long t = blockIdx.x * 256L + threadIdx.x;
+  if (t < (long)batch * nin) {
+    const int b = (int)(t / nin), i = (int)(t - (long)b * nin);
     float acc = 0.f;
     for (int o = 0; o < nout; ++o) acc = fmaf(dz[(long)b * nout + o], w[(long)o * nin + i], acc);
     dx[t] = acc;
   }
 }
 
-/* Adam (Kingma and Ba, 2015) with bias correction; step counts from 1 */
+/* Adam (Kingma and Ba, 2015) with bias correction and L2 weight decay; step counts from 1 */
 __global__ void __launch_bounds__(256) mlp_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m,
   float* __restrict__ v, int n, float lr, float beta1, float beta2, float eps, float c1, float c2, float wd) {
   const int t = blockIdx.x * 256 + threadIdx.x;
@@ -125,19 +128,29 @@ extern "C" int lxs_mlp_forward(const float* x, const float* w, int batch, int ni
   return (int)hipGetLastError();
 }
 
-extern "C" int lxs_mlp_loss(const float* z, const float* tgt, const int* seg, const float* hw, int nseg, int batch,
-  int nh, float* dz, float* loss, void* hip_stream) {
+extern "C" int lxs_mlp_loss(const float* z, const float* tgt, const float* hw, int batch, int nh, float* dz,
+  float* loss, void* hip_stream) {
   if (0 >= batch) return 0;
-  mlp_loss<<<blocks(batch), 256, 0, (hipStream_t)hip_stream>>>(z, tgt, seg, hw, nseg, batch, nh, dz, loss);
+  mlp_loss<<<blocks(batch), 256, 0, (hipStream_t)hip_stream>>>(z, tgt, hw, batch, nh, dz, loss);
   return (int)hipGetLastError();
 }
 
+/* scratch floats lxs_mlp_backward needs for its partial weight gradients */
+extern "C" long lxs_mlp_scratch(int batch, int nin, int nout) {
+  const int nchunks = (batch + 399) / 400 < 256 ? (batch + 399) / 400 : 256;
+  return (long)(nchunks > 0 ? nchunks : 1) * nout * (nin + 1);
+}
+
 extern "C" int lxs_mlp_backward(const float* x, const float* y, const float* w, float* dz, int batch, int nin, int nout,
-  int relu, float* dw, float* dx, void* hip_stream) {
+  int relu, float* dw, float* dx, float* scratch, void* hip_stream) {
   const hipStream_t st = (hipStream_t)hip_stream;
+  const int np = nout * (nin + 1);
+  const int nchunks = (int)(lxs_mlp_scratch(batch, nin, nout) / np), rows = (batch + nchunks - 1) / nchunks;
   if (0 >= batch) return 0;
-  if (0 != relu) mlp_relu_grad<<<blocks((long)batch * nout), 256, 0, st>>>(y, dz, batch * nout);
-  mlp_wgrad<<<blocks((long)nout * (nin + 1)), 256, 0, st>>>(x, dz, batch, nin, nout, dw);
+  if (64 * 65 < np) return LXS_ELIMIT;
+  if (0 != relu) mlp_relu_grad<<<blocks((long)batch * nout), 256, 0, st>>>(y, dz, (long)batch * nout);
+  mlp_wgrad_part<<<(unsigned int)nchunks, 256, 0, st>>>(x, dz, batch, nin, nout, rows, scratch);
+  mlp_wgrad_sum<<<blocks(np), 256, 0, st>>>(scratch, nchunks, np, dw);
   if (NULL != dx) mlp_xgrad<<<blocks((long)batch * nin), 256, 0, st>>>(dz, w, batch, nin, nout, dx);
   return (int)hipGetLastError();
 }

@@ -1,15 +1,16 @@
 """Learned stack-kernel predictor: training data from exhaustive sweeps, training, evaluation.
 
 The sweeps are the JSON lines tune_multiply writes (one per measured configuration, plus a "done"
-line per shape). For every (dtype, m, n, k) the targets of the network's heads are soft class
-distributions over the configurations within `within` of the best throughput, weighted by
-exp((gflops/best - 1) / temperature), and log2 of the best GFLOP/s. Training and inference run on
-the GPU through libsmm_acc_mlp_* (hand-written HIP kernels); this module only prepares arrays.
+line per shape). Every measured (shape, configuration) is one training row: the network's inputs are
+libsmm_acc_mlp_features (8 shape features and the configuration's one-hot parameters), its targets
+the configuration's throughput relative to the shape's best and log2 of its GFLOP/s. Training and
+inference run on the GPU through libsmm_acc_mlp_* (hand-written HIP kernels); this module only
+prepares arrays.
 
-Evaluation follows the exhaustive autotuner as the judge: the network's choice for a held-out
-shape is looked up in that shape's sweep (every configuration was measured there), and compared
-with the best measured one; the method without the network (the built-in default heuristic, or
-the reference's nearest tuned triplet, samples/smm/smm_kernel.c:116-148) is scored the same way.
+Evaluation follows the exhaustive autotuner as the judge: the network's choice for a held-out shape
+is looked up in that shape's sweep (every configuration was measured there) and compared with the
+best measured one. The methods without the network are scored the same way: the built-in default
+heuristic, and the reference's nearest tuned triplet (samples/smm/smm_kernel.c:116-148).
 """
 import gzip
 import json
@@ -19,7 +20,8 @@ import numpy as np
 
 import libxstream_amd as lx
 
-HEAD_KEYS = ("variant", "tm", "tn", "nt", "kt", "bs", "flush")
+PARAM_KEYS = ("variant", "tm", "tn", "nt", "kt", "bs", "flush")
+OUT_WEIGHTS = (1.0, 0.01)  # loss weights of the outputs: relative throughput, log2 GFLOP/s
 
 
 def triplets(count=500, seed=25071975, lo=4, hi=80):
@@ -35,12 +37,13 @@ def triplets(count=500, seed=25071975, lo=4, hi=80):
 
 
 def split(trips, ntrain=400):
+    """the first ntrain triplets train, the rest are held out"""
     return trips[:ntrain], trips[ntrain:]
 
 
 def load_sweep(paths):
-    """{(tid, m, n, k): [row, ...]} of measured configurations (rows are dicts)"""
-    rows = defaultdict(list)
+    """{(tid, m, n, k): [row, ...]} of measured configurations (rows are dicts); .gz files are read too"""
+    rows_by_key = defaultdict(list)
     for path in ([paths] if isinstance(paths, str) else paths):
         with (gzip.open(path, "rt") if str(path).endswith(".gz") else open(path)) as f:
             for line in f:
@@ -50,55 +53,36 @@ def load_sweep(paths):
                 r = json.loads(line)
                 if r.get("done") or r.get("gflops", 0) <= 0:
                     continue
-                rows[(r["dtype"], r["m"], r["n"], r["k"])].append(r)
-    return dict(rows)
+                rows_by_key[(r["dtype"], r["m"], r["n"], r["k"])].append(r)
+    return dict(rows_by_key)
 
 
-def head_value(row, key):
-    """class value of a sweep row for a head: the vector kernel (variant 0) has no compiled kt, and
-    flush 0 (automatic) launches the atomic flush (2)"""
-    if key == "kt" and row["variant"] == 0:
+def param_value(cfg, key):
+    """a configuration parameter as the network encodes it: the vector kernel (variant 0) has no
+    compiled kt, and flush 0 (automatic) launches the atomic flush (2)"""
+    get = cfg.get if isinstance(cfg, dict) else (lambda k: getattr(cfg, k))
+    if key == "kt" and get("variant") == 0:
         return 0
-    if key == "flush" and row[key] == 0:
+    if key == "flush" and get("flush") == 0:
         return 2
-    return row[key]
-
-
-def targets(rows, seg, values, within=0.05, temperature=0.02):
-    """soft per-head targets and log2 of the best GFLOP/s for one shape"""
-    nout = int(seg[-1])
-    t = np.zeros(nout, dtype=np.float32)
-    best = max(r["gflops"] for r in rows)
-    good = [r for r in rows if r["gflops"] >= (1.0 - within) * best]
-    wts = np.array([np.exp((r["gflops"] / best - 1.0) / temperature) for r in good])
-    wts /= wts.sum()
-    for h, key in enumerate(HEAD_KEYS):
-        c0, c1 = int(seg[h]), int(seg[h + 1])
-        cls = {int(v): c0 + i for i, v in enumerate(values[c0:c1])}
-        for r, w in zip(good, wts):
-            c = cls.get(int(head_value(r, key)))
-            if c is not None:
-                t[c] += w
-        s = t[c0:c1].sum()
-        if s > 0:
-            t[c0:c1] /= s
-    t[int(seg[len(HEAD_KEYS)])] = np.log2(best)
-    return t
-
-
-def dataset(sweep, shapes, within=0.05, temperature=0.02):
-    """features [N][8] and targets [N][nout] for the (tid, m, n, k) keys of `shapes` present in the sweep"""
-    seg, values = lx.mlp_heads()
-    keys = [s for s in shapes if s in sweep]
-    x = np.stack([lx.mlp_features(*key) for key in keys]) if keys else np.zeros((0, 8), np.float32)
-    t = np.stack([targets(sweep[key], seg, values, within, temperature) for key in keys]) if keys else None
-    return keys, x, t
+    return int(get(key))
 
 
 def config_key(cfg):
-    """(variant, tm, tn, nt, kt class, bs, flush) of a sweep row (dict) or an SmmConfig"""
-    get = cfg.get if isinstance(cfg, dict) else (lambda k: getattr(cfg, k))
-    return tuple(int(head_value({"variant": get("variant"), key: get(key)}, key)) for key in HEAD_KEYS)
+    """(variant, tm, tn, nt, kt, bs, flush) of a sweep row (dict) or an SmmConfig, as encoded"""
+    return tuple(param_value(cfg, key) for key in PARAM_KEYS)
+
+
+def to_config(row):
+    """SmmConfig of a sweep row"""
+    cfg = lx.SmmConfig()
+    for key in ("variant", "tm", "tn", "nt", "bs", "kt", "flush"):
+        setattr(cfg, key, int(row[key]))
+    return cfg
+
+
+def best_row(rows):
+    return max(rows, key=lambda r: r["gflops"])
 
 
 def lookup(rows, cfg):
@@ -110,8 +94,24 @@ def lookup(rows, cfg):
     return None
 
 
-def best_row(rows):
-    return max(rows, key=lambda r: r["gflops"])
+def dataset(sweep, keys):
+    """inputs [R][nin], targets [R][2] and the shape index of every measured configuration of `keys`"""
+    xs, ts, gs = [], [], []
+    for i, key in enumerate(keys):
+        best = best_row(sweep[key])["gflops"]
+        for r in sweep[key]:
+            xs.append(lx.mlp_features(*key, to_config(r)))
+            ts.append((r["gflops"] / best, np.log2(r["gflops"])))
+            gs.append(i)
+    return np.array(xs, dtype=np.float32), np.array(ts, dtype=np.float32), np.array(gs)
+
+
+def train(sweep, train_keys, nnets=4, hidden=64, nhidden=3, seed=1, epochs=2000, lr=3e-3, weight_decay=1e-5):
+    """an ensemble trained on every measured configuration of the training shapes: (model, mean loss)"""
+    x, t, _ = dataset(sweep, train_keys)
+    model = lx.Mlp(nnets, hidden, nhidden, seed=seed)
+    loss = model.train(x, t, epochs, lr=lr, weight_decay=weight_decay, out_weights=OUT_WEIGHTS)
+    return model, loss
 
 
 def nearest_tuned(key, tuned_keys):
@@ -125,55 +125,45 @@ def nearest_tuned(key, tuned_keys):
     return cands[int(np.argmin(d))]
 
 
-def _config(row_or_cfg):
-    cfg = lx.SmmConfig()
-    get = row_or_cfg.get if isinstance(row_or_cfg, dict) else (lambda k: getattr(row_or_cfg, k))
-    for key in ("variant", "tm", "tn", "nt", "bs", "kt", "flush"):
-        setattr(cfg, key, int(get(key)))
-    return cfg
-
-
-def evaluate(model, sweep, keys, train_keys):
+def evaluate(model, sweep, keys, tuned_keys):
     """per held-out shape: measured GFLOP/s of the network's choice, of the default heuristic, of the
-    nearest tuned triplet's configuration, and the best measured (the exhaustive autotuner's)"""
+    nearest tuned triplet's configuration, and the best measured one (the exhaustive autotuner's)"""
     out = []
     for key in keys:
         rows = sweep[key]
         best = best_row(rows)["gflops"]
         cfg, conf = model.predict_config(*key)
-        learned = lookup(rows, cfg)
         default = lookup(rows, lx.default_config(*key))
-        near = nearest_tuned(key, train_keys)
+        near = nearest_tuned(key, tuned_keys)
         nearest = None
         if near is not None:
-            ncfg = _config(best_row(sweep[near]))
+            ncfg = to_config(best_row(sweep[near]))
             if lx.config_valid(key[0], key[1], key[2], key[3], ncfg):
                 nearest = lookup(rows, ncfg)
-            if nearest is None:  # not valid for this shape: the reference falls back to its default
-                nearest = default
-        out.append({"key": list(key), "best": best, "learned": learned, "confidence": conf, "default": default,
-                    "nearest": nearest})
+        if nearest is None:  # not valid for this shape: the reference falls back to its default
+            nearest = default
+        out.append({"key": list(key), "best": best, "learned": lookup(rows, cfg), "confidence": conf,
+                    "predicted_gflops": cfg.gflops, "default": default, "nearest": nearest})
     return out
 
 
 def summary(results, field):
+    """fraction of the exhaustive best reached by `field` over the results"""
     ratios = np.array([(r[field] or 0.0) / r["best"] for r in results])
     return {"mean": round(float(ratios.mean()), 4), "median": round(float(np.median(ratios)), 4),
             "min": round(float(ratios.min()), 4), "within_10pct": round(float((ratios >= 0.9).mean()), 4),
             "geomean": round(float(np.exp(np.log(np.maximum(ratios, 1e-6)).mean())), 4)}
 
 
-def train(sweep, train_keys, hidden=64, nhidden=3, seed=1, epochs=(3000, 3000), lrs=(3e-3, 1e-3), weight_decay=1e-4,
-          within=0.05, temperature=0.02):
-    """train a network on the training shapes; returns (model, final loss)"""
-    keys, x, t = dataset(sweep, train_keys, within, temperature)
-    hw = np.ones(len(HEAD_KEYS) + 1, dtype=np.float32)
-    hw[-1] = 0.5  # log2 GFLOP/s regression
-    model = lx.Mlp(hidden, nhidden, seed=seed)
-    loss = None
-    for ep, lr in zip(epochs, lrs):
-        loss = model.train(x, t, ep, lr=lr, weight_decay=weight_decay, head_weights=hw)
-    return model, loss
+def report(results, tids):
+    rep = {"heldout_shapes": len(results)}
+    for f in ("learned", "default", "nearest"):
+        rep[f] = summary(results, f)
+    for tid in tids:
+        sub = [x for x in results if x["key"][0] == tid]
+        if sub:
+            rep["type%d" % tid] = {f: summary(sub, f) for f in ("learned", "default", "nearest")}
+    return rep
 
 
 def write_triplets(path, count=500, seed=25071975):
@@ -194,47 +184,35 @@ def main(argv):
     r.add_argument("sweeps", nargs="+")
     r.add_argument("--model", required=True, help="output model file")
     r.add_argument("--report", help="output JSON report")
-    r.add_argument("--seeds", type=int, default=4, help="networks trained (lowest training loss kept)")
+    r.add_argument("--epochs", type=int, default=2000)
+    r.add_argument("--nets", type=int, default=4)
+    e = sub.add_parser("evaluate", help="evaluate a model on the held-out shapes of the sweeps")
+    e.add_argument("sweeps", nargs="+")
+    e.add_argument("--model", required=True)
     args = p.parse_args(argv)
     if args.cmd == "triplets":
         write_triplets(args.path)
         return 0
     lx.init()
     sweep = load_sweep(args.sweeps)
-    trips = triplets()
-    train_t, held_t = split(trips)
+    train_t, held_t = split(triplets())
     tids = sorted({key[0] for key in sweep})
     train_keys = [(tid,) + t for tid in tids for t in train_t if (tid,) + t in sweep]
     held_keys = [(tid,) + t for tid in tids for t in held_t if (tid,) + t in sweep]
-    best = None
-    for seed in range(1, args.seeds + 1):
-        model, loss = train(sweep, train_keys, seed=seed)
-        print("seed %d: training loss %.4f" % (seed, loss), flush=True)
-        if best is None or loss < best[1]:
-            if best is not None:
-                best[0].close()
-            best = (model, loss, seed)
-        else:
-            model.close()
-    model, loss, seed = best
-    model.save(args.model)
+    if args.cmd == "train":
+        model, loss = train(sweep, train_keys, nnets=args.nets, epochs=args.epochs)
+        model.save(args.model)
+        print("trained %d networks on %d shapes: loss %.5f" % (args.nets, len(train_keys), loss), flush=True)
+    else:
+        model, loss = lx.Mlp(path=args.model), None
     results = evaluate(model, sweep, held_keys, train_keys)
-    report = {"train_shapes": len(train_keys), "heldout_shapes": len(held_keys), "seed": seed, "loss": loss,
-              "learned": summary(results, "learned"), "default": summary(results, "default"),
-              "nearest_tuned": summary(results, "nearest")}
-    for tid in tids:
-        sub_r = [x for x in results if x["key"][0] == tid]
-        report["type%d" % tid] = {f: summary(sub_r, f) for f in ("learned", "default", "nearest")}
-    conf = np.array([x["confidence"] for x in results])
-    ok = np.array([(x["learned"] or 0) >= 0.9 * x["best"] for x in results])
-    report["confidence_quantiles"] = [round(float(q), 4) for q in np.quantile(conf, [0.1, 0.25, 0.5, 0.75, 0.9])]
-    report["within_10pct_by_confidence"] = {str(q): round(float(ok[conf >= q].mean()), 4) if (conf >= q).any() else None
-                                           for q in (0.0, 0.05, 0.1, 0.2, 0.3, 0.5)}
-    report["results"] = results
-    print(json.dumps({k: v for k, v in report.items() if k != "results"}, indent=1))
-    if args.report:
+    rep = report(results, tids)
+    rep["train_shapes"], rep["loss"] = len(train_keys), loss
+    print(json.dumps(rep, indent=1))
+    if getattr(args, "report", None):
+        rep["results"] = results
         with open(args.report, "w") as f:
-            json.dump(report, f)
+            json.dump(rep, f)
     model.close()
     lx.finalize()
     return 0
