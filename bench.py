@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--no-ozaki", action="store_true", help="skip the Ozaki-int8 DGEMM measurement")
     p.add_argument("--ozaki-sizes", default="1024,4096,8192", help="square DGEMM sizes for the Ozaki section")
     p.add_argument("--ozaki-only", action="store_true", help="print only the Ozaki section (no stack measurement)")
+    p.add_argument("--no-learned", action="store_true", help="skip the learned-predictor evaluation")
     p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -299,6 +300,33 @@ def ozaki_section(lx, stream, sizes, reps=3):
     }
 
 
+def learned_section(lx):
+    """The shipped predictor judged on the 100 held-out triplets per type by the exhaustive sweeps shipped
+    with it (each method's choice looked up in the shape's measured configurations)."""
+    import libxstream_amd.learned as learned
+
+    params = os.path.join(os.path.dirname(lx.LIB_PATH), "params")
+    model_path = os.path.join(params, "mlp_MI355X.txt")
+    sweeps = [os.path.join(params, "sweep_%s_MI355X.jsonl.gz" % t) for t in ("f64", "f32")]
+    if not os.path.exists(model_path) or not all(os.path.exists(x) for x in sweeps):
+        return None
+    sweep = learned.load_sweep(sweeps)
+    train_t, held_t = learned.split(learned.triplets())
+    tids = (lx.REAL_8, lx.REAL_4)
+    train_keys = [(tid,) + t for tid in tids for t in train_t if (tid,) + t in sweep]
+    held_keys = [(tid,) + t for tid in tids for t in held_t if (tid,) + t in sweep]
+    model = lx.Mlp(path=model_path)
+    t0 = time.perf_counter()
+    results = learned.evaluate(model, sweep, held_keys, train_keys)
+    t1 = time.perf_counter()
+    model.close()
+    rep = learned.report(results, tids)
+    rep["workload"] = ("exhaustive sweeps (tune_multiply, S=30000) over 500 seeded triplets in 4..80; "
+                       "400 train, 100 held out per type; fraction of the best measured configuration")
+    rep["prediction_ms_per_shape"] = round((t1 - t0) * 1e3 / max(len(held_keys), 1), 3)
+    return rep
+
+
 def traffic_probe(args):
     """Child of rocprofv3 --pmc: one calibration copy, then three stack launches."""
     import libxstream_amd as lx
@@ -436,6 +464,7 @@ def main():
                  "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
     batch = None if args.no_batch else batched(lx, stream, args.steps, args.warmup)
     ozaki = None if args.no_ozaki else ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
+    learned_rep = None if args.no_learned else learned_section(lx)
     ncus, arch = lx.device_info()
     stream.close()
     lx.finalize()
@@ -489,6 +518,7 @@ def main():
         "stack_30000": small,
         "batched_mixed": batch,
         "ozaki": ozaki,
+        "learned": learned_rep,
         "reference_published_gflops_23cubed_fp64_s30000": {"V100": 2459.7, "P100": 1657.8},
     }
     print(json.dumps(line))
