@@ -45,7 +45,7 @@ typedef struct ozaki_context_t {
   int verbosity;  /* 0: quiet, 1: info, 2+: debug */
   int tm, tn;     /* output tile of one work-group (256 x 256) */
   int maxk;       /* K per preprocessing pass (0: all of K) */
-  int flush;      /* k-blocks of 128 per int8 work item, 1..1040 (env LIBXSTREAM_OZAKI_FLUSH); results
+  int flush;      /* k-blocks of 64 per int8 work item, 1..2080 (env LIBXSTREAM_OZAKI_FLUSH); results
                      do not depend on it: level sums are exact */
   void* impl;     /* device state owned by the context */
 } ozaki_context_t;

@@ -77,7 +77,7 @@ int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream);
 int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int s, int Rp, int Kp, int* e,
   int* occ, signed char* planes, void* hip_stream);
 /* int8 level GEMM over work items {level, u0, u1, pad} (host array): int32 partial z at c32 + z Mp Np,
-   element (i, j) at j Mp + i; an item holds at most 1040 units (pair, k-block of 128) */
+   element (i, j) at j Mp + i; an item holds at most 2080 units (pair, k-block of 64) */
 int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, const int* occa, const int* occb,
   const int* items, int nitems, int s, int Mp, int Np, int Kp, int* c32, void* hip_stream);
 /* FP64 recombination of the partials: zb[L] = first item of level L (cutoff + 2 entries) */

@@ -12,8 +12,9 @@
 
 #define OZ_ALIGN 256
 #define OZ_TILE_MN 256    /* output tile of the int8 level GEMM; row padding of the digit planes */
-#define OZ_TILE_K 128     /* k-block of the level GEMM; k padding of the digit planes */
-#define OZ_FLUSH_MAX 1040 /* k-blocks per work item: 1040 * 128 * 127^2 < 2^31 (exact int32 partials) */
+#define OZ_TILE_K 128     /* k padding of the digit planes */
+#define OZ_UNIT_K 64      /* k per unit of the level GEMM */
+#define OZ_FLUSH_MAX 2080 /* units per work item: 2080 * 64 * 127^2 < 2^31 (exact int32 partials) */
 #define OZ_WG_TARGET 1024 /* work-groups the item plan aims for */
 
 typedef struct oz_impl_t {
@@ -106,9 +107,9 @@ static int oz_trans(char t) {
 }
 
 /*
- * Work items of the level GEMM: every level L = 0 .. cutoff has npairs(L) * nkb units (pair, k-block),
+ * Work items of the level GEMM: every level L = 0 .. cutoff has npairs(L) * nkb units (pair, k-block of 64),
  * cut into items of at most cu units, cu chosen so that items x tiles is about OZ_WG_TARGET
- * work-groups (at least 8 units per item, at most ctx->flush). zb[L] = first item of level L.
+ * work-groups (at least 16 units per item, at most ctx->flush). zb[L] = first item of level L.
  * Returns the number of items; fills items (4 ints each) when not NULL.
  */
 static int oz_plan(const ozaki_context_t* ctx, int nkb, int tiles, int* items, int* zb) {
@@ -120,7 +121,7 @@ static int oz_plan(const ozaki_context_t* ctx, int nkb, int tiles, int* items, i
     total += (long)(phi - plo + 1) * nkb;
   }
   cu = (int)((total * tiles + OZ_WG_TARGET - 1) / OZ_WG_TARGET);
-  if (cu < 8) cu = 8;
+  if (cu < 16) cu = 16;
   if (cu > ctx->flush) cu = ctx->flush;
   for (L = 0; L <= cutoff; ++L) {
     const int plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
@@ -154,7 +155,7 @@ static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, 
   l->Np = (int)oz_up((size_t)(0 < N ? N : 1), OZ_TILE_MN);
   l->Kp = (int)oz_up((size_t)(0 < kpass ? kpass : 1), OZ_TILE_K);
   l->kpass = kpass;
-  l->nitems = oz_plan(ctx, l->Kp / OZ_TILE_K, (l->Mp / OZ_TILE_MN) * (l->Np / OZ_TILE_MN), NULL, NULL);
+  l->nitems = oz_plan(ctx, l->Kp / OZ_UNIT_K, (l->Mp / OZ_TILE_MN) * (l->Np / OZ_TILE_MN), NULL, NULL);
   l->planes_a = off;
   off += oz_up((size_t)ctx->nslices * l->Mp * l->Kp, OZ_ALIGN);
   l->planes_b = off;
@@ -311,7 +312,7 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
     for (k0 = 0; k0 < K && 0 == result; k0 += l.kpass) {
       const int kk = (K - k0 < l.kpass ? K - k0 : l.kpass);
       const int Kp = (int)oz_up((size_t)kk, OZ_TILE_K);
-      const int nitems = oz_plan(ctx, Kp / OZ_TILE_K, (l.Mp / OZ_TILE_MN) * (l.Np / OZ_TILE_MN), impl->items, zb);
+      const int nitems = oz_plan(ctx, Kp / OZ_UNIT_K, (l.Mp / OZ_TILE_MN) * (l.Np / OZ_TILE_MN), impl->items, zb);
       void* token = lxs_profile_begin(hs);
       result = lxs_oz_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, s, l.Mp, Kp, (int*)(base + l.ea), occ,
         (signed char*)(base + l.planes_a), hs);

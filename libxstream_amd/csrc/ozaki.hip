@@ -20,9 +20,9 @@
  *               occupancy flag per plane that holds a nonzero digit
  *   oz_level_gemm  pure int8 -> int32 GEMM per work item (level L, range of its (pair, k-block) units):
  *               256x256 output tile per work-group, 4 wavefronts of 128x128 (v_mfma_i32_32x32x32_i8),
- *               operand tiles of 128 k staged through LDS, double-buffered, with the next tile's
- *               global loads in flight during the MFMAs; pairs whose planes are all zero are skipped.
- *               Every item writes int32 partial sums (exact: an item's units are bounded).
+ *               operand tiles of 64 k moved into LDS by DMA, 4 stages deep; pairs whose planes are all
+ *               zero are skipped. Every item writes int32 partial sums (exact: an item's units are
+ *               bounded).
  *   oz_recombine   per element: S_L = exact int64 sum of the level's partials, cd = sum over
  *               L = cutoff .. 0 of 2^(-7(L+2)) S_L in FP64, C = fma(alpha, 2^(ea+eb) cd, beta C).
  *   dgemm_f64   128x128 tiles with v_mfma_f64_16x16x4f64 on FP64 operands (the comparator).
@@ -158,13 +158,12 @@ __global__ void __launch_bounds__(256) oz_scale_c(T* __restrict__ c, int ldc, in
 
 constexpr int GT = 128;        /* output tile of the FP64 comparator */
 constexpr int LT = 256;        /* int8 level GEMM: output tile (i and j) per work-group */
-constexpr int LK = 128;        /* k per LDS stage ("k-block") */
-constexpr int LLD = LK + 16;   /* LDS row stride in bytes: 16-byte reads of 8 consecutive rows hit distinct banks */
+constexpr int LK = 128;        /* k padding of the digit planes */
 
 /*
  * One work item: level L (pairs p = plo .. phi, q = L - p), units u0 .. u1 of the level (unit u is
- * pair plo + u / nkb, k-block u % nkb). Every item writes its own int32 partial sums, exact because
- * an item holds at most 1040 units: 1040 * 128 * 127^2 < 2^31.
+ * pair plo + u / nkb, k-block u % nkb of 64 k). Every item writes its own int32 partial sums, exact
+ * because an item holds at most 2080 units: 2080 * 64 * 127^2 < 2^31.
  */
 struct oz_item_t {
   int level, u0, u1, pad;
@@ -180,19 +179,38 @@ struct oz_items_t {
  * second (columns i): lane l of D holds column i = l % 32 and rows 8(r/4) + 4(l/32) + r%4, so for a
  * fixed register the 32 lanes of a half-wave store 128 contiguous bytes along i.
  * 4 wavefronts in 2 x 2, each 128 x 128 (4 x 4 MFMA tiles, 256 accumulator registers).
+ * Operand tiles of 64 k (one "unit") move from global memory straight into LDS by DMA
+ * (global_load_lds_dwordx4), four stages deep (three in flight while one is computed), without
+ * staging registers.
+ * LDS row layout: 64 bytes per row (4 chunks of 16), chunk c of row r stored at position
+ * c ^ ((r >> 2) & 3); the 16 lanes of a quarter-wave read rows r..r+15 of one chunk from 16 distinct
+ * 4-bank groups. One DMA instruction fills 16 rows (1 KB): lane l writes position l % 4 of row l / 4,
+ * so it loads chunk (l % 4) ^ ((row >> 2) & 3) of that row.
  */
-__global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb,
-  const int* __restrict__ occa, const int* __restrict__ occb, oz_items_t items, int s, int Mp, int Np, int Kp,
-  int* __restrict__ c32) {
-  __shared__ __attribute__((aligned(16))) int8_t lds[2][2][LT * LLD]; /* [buffer][A|B][256 rows x LLD] */
+constexpr int QK = 64;                /* k per stage */
+constexpr int QTILE = LT * QK;        /* bytes of one operand tile per stage (16 KB) */
+
+/* operand tile of zeros: the source of the padding stages (their MFMAs add nothing) */
+__device__ __attribute__((aligned(1024))) int8_t oz_zero_tile[QTILE];
+
+__device__ __forceinline__ void oz_dma16(const int8_t* src, int8_t* lds_dst) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+    (__attribute__((address_space(3))) void*)lds_dst, 16, 0, 0);
+}
+
+__global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb, const int* __restrict__ occa,
+  const int* __restrict__ occb, oz_items_t items, int s, int Mp, int Np, int Kp, int* __restrict__ c32) {
+  /* one array per stage ([A|B][256 rows x 64 bytes]): distinct objects, so the compiler can tell the DMA
+     targets of the stages in flight from the stage being read */
+  __shared__ __attribute__((aligned(1024))) int8_t ls0[2 * QTILE], ls1[2 * QTILE], ls2[2 * QTILE], ls3[2 * QTILE];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wj = wave >> 1, wi = wave & 1;
   const int i0 = blockIdx.x * LT, j0 = blockIdx.y * LT;
   const oz_item_t it = items.it[blockIdx.z];
-  const int L = it.level, nkb = Kp / LK;
+  const int L = it.level, nkb = Kp / QK;
   const int plo = (L - s + 1 > 0 ? L - s + 1 : 0);
-  unsigned int oa = 0, ob = 0; /* occupied planes */
+  unsigned int oa = 0, ob = 0;
   for (int p = 0; p < s; ++p) {
     oa |= (0 != occa[p] ? 1u : 0u) << p;
     ob |= (0 != occb[p] ? 1u : 0u) << p;
@@ -205,77 +223,83 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
 #pragma unroll
     for (int y = 0; y < 4; ++y) acc[x][y] = i16_t{0};
   }
-  /* next unit to fetch: skip units of pairs whose planes hold no nonzero digit */
-  int fu = it.u0;
-  auto active = [&](int u) {
-    const int p = plo + u / nkb;
-    return 0 != ((oa >> p) & (ob >> (L - p)) & 1u);
-  };
-  auto advance = [&]() {
-    while (fu < it.u1 && !active(fu)) fu = (fu / nkb + 1) * nkb; /* skip the rest of an inactive pair */
-  };
-  advance();
-  if (fu < it.u1) {
-    /* staging: 16-byte chunk c = tid + 256 x of each operand tile (256 rows x 128 bytes): row c/8, chunk c%8 */
-    i4_t ra[8], rb[8];
-    auto fetch = [&]() {
+  /* active units (pairs with both planes occupied) of this item, counted up front */
+  int nact = 0;
+  for (int pp = it.u0 / nkb; pp * nkb < it.u1; ++pp) {
+    const int p = plo + pp, lo = (it.u0 > pp * nkb ? it.u0 : pp * nkb), hi = (it.u1 < (pp + 1) * nkb ? it.u1 : (pp + 1) * nkb);
+    if ((oa >> p) & (ob >> (L - p)) & 1u) nact += hi - lo;
+  }
+  if (0 < nact) {
+    int fu = it.u0; /* next unit to load */
+    auto skip = [&]() {
+      while (fu < it.u1 && !((oa >> (plo + fu / nkb)) & (ob >> (L - plo - fu / nkb)) & 1u)) fu = (fu / nkb + 1) * nkb;
+    };
+    skip();
+    const int drow = lane >> 2, dpos = lane & 3; /* this lane's row of a DMA instruction's 16, and position */
+    /* stage g into stage array buf: A tile rows 16 q .. 16 q + 15 for q = wave + 4 r, B alike; the next
+       active unit while g < nact, the zero tile after that (the stage count is padded to a multiple of 4) */
+    int g = 0;
+    auto dma = [&](int8_t* buf) {
       const int p = plo + fu / nkb, kb = fu - (fu / nkb) * nkb;
-      const int8_t* const ga = pa + ((long)p * Mp + i0) * Kp + kb * LK;
-      const int8_t* const gb = pb + ((long)(L - p) * Np + j0) * Kp + kb * LK;
+      const bool real = (g < nact);
+      const int8_t* const ga = (real ? pa + ((long)p * Mp + i0) * Kp + kb * QK : oz_zero_tile);
+      const int8_t* const gb = (real ? pb + ((long)(L - p) * Np + j0) * Kp + kb * QK : oz_zero_tile);
+      const long stride = (real ? Kp : QK);
 #pragma unroll
-      for (int x = 0; x < 8; ++x) {
-        const int c = tid + 256 * x, row = c >> 3, kc = (c & 7) * 16;
-        ra[x] = *(const i4_t*)(ga + (long)row * Kp + kc);
-        rb[x] = *(const i4_t*)(gb + (long)row * Kp + kc);
+      for (int r = 0; r < 4; ++r) {
+        const int q = wave + 4 * r, row = 16 * q + drow, chunk = dpos ^ ((row >> 2) & 3);
+        oz_dma16(ga + (long)row * stride + 16 * chunk, buf + q * 1024);
+        oz_dma16(gb + (long)row * stride + 16 * chunk, buf + QTILE + q * 1024);
       }
-      ++fu;
-      advance();
+      if (real) {
+        ++fu;
+        skip();
+      }
+      ++g;
     };
-    auto store = [&](int buf) {
-#pragma unroll
-      for (int x = 0; x < 8; ++x) {
-        const int c = tid + 256 * x, row = c >> 3, kc = (c & 7) * 16;
-        *(i4_t*)(&lds[buf][0][row * LLD + kc]) = ra[x];
-        *(i4_t*)(&lds[buf][1][row * LLD + kc]) = rb[x];
-      }
-    };
-    fetch();
-    store(0);
-    __syncthreads();
-    for (int buf = 0;; buf ^= 1) {
-      const bool last = !(fu < it.u1);
-      if (!last) fetch();
-      { /* two fragment sets: the LDS reads of the next 32 k overlap the 16 MFMAs of the current ones */
-        const int8_t* const la = &lds[buf][0][(128 * wi + (lane & 31)) * LLD + 16 * (lane >> 5)];
-        const int8_t* const lb = &lds[buf][1][(128 * wj + (lane & 31)) * LLD + 16 * (lane >> 5)];
-        i4_t fa0[4], fb0[4], fa1[4], fb1[4];
-        auto frag = [&](i4_t* fa, i4_t* fb, int kk) {
-#pragma unroll
-          for (int x = 0; x < 4; ++x) {
-            fb[x] = *(const i4_t*)(lb + 32 * x * LLD + kk);
-            fa[x] = *(const i4_t*)(la + 32 * x * LLD + kk);
-          }
-        };
-        auto mma = [&](const i4_t* fa, const i4_t* fb) {
-#pragma unroll
-          for (int x = 0; x < 4; ++x) {
-#pragma unroll
-            for (int y = 0; y < 4; ++y) acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb[x], fa[y], acc[x][y], 0, 0, 0);
-          }
-        };
-        frag(fa0, fb0, 0);
-#pragma unroll 1
-        for (int kk = 0; kk < LK; kk += 64) {
-          frag(fa1, fb1, kk + 32);
-          mma(fa0, fb0);
-          if (kk + 64 < LK) frag(fa0, fb0, kk + 64);
-          mma(fa1, fb1);
-        }
-      }
-      if (last) break;
-      store(buf ^ 1);
-      __syncthreads();
+    const int frow = lane & 31, fswz = (frow >> 2) & 3; /* (row >> 2) & 3 is the same for rows 32 x + frow */
+    const int fpos0 = (lane >> 5) ^ fswz, fpos1 = (2 + (lane >> 5)) ^ fswz;
+    /* one stage: wait for this wave's loads of the stage (two later stages may fly), barrier (every wave's
+       loads of the stage have landed, every wave has consumed the previous stage), refill the array
+       computed last time, then all 16 fragments and 32 MFMAs */
+#define OZ_STAGE(CUR, NXT) \
+  do { \
+    __builtin_amdgcn_s_waitcnt(0x4F70); /* vmcnt(16): this wave's DMA of the stage has landed */ \
+    __asm__ __volatile__("" ::: "memory"); \
+    __builtin_amdgcn_s_barrier(); /* no fence: it would wait for the stages still in flight too */ \
+    __asm__ __volatile__("" ::: "memory"); \
+    dma(NXT); \
+    const int8_t* const la_ = CUR + (128 * wi + frow) * QK; \
+    const int8_t* const lb_ = CUR + QTILE + (128 * wj + frow) * QK; \
+    i4_t fa0_[4], fb0_[4], fa1_[4], fb1_[4]; \
+    _Pragma("unroll") for (int x_ = 0; x_ < 4; ++x_) { \
+      fb0_[x_] = *(const i4_t*)(lb_ + 32 * x_ * QK + 16 * fpos0); \
+      fa0_[x_] = *(const i4_t*)(la_ + 32 * x_ * QK + 16 * fpos0); \
+      fb1_[x_] = *(const i4_t*)(lb_ + 32 * x_ * QK + 16 * fpos1); \
+      fa1_[x_] = *(const i4_t*)(la_ + 32 * x_ * QK + 16 * fpos1); \
+    } \
+    _Pragma("unroll") for (int x_ = 0; x_ < 4; ++x_) { \
+      _Pragma("unroll") for (int y_ = 0; y_ < 4; ++y_) { \
+        acc[x_][y_] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb0_[x_], fa0_[y_], acc[x_][y_], 0, 0, 0); \
+      } \
+    } \
+    _Pragma("unroll") for (int x_ = 0; x_ < 4; ++x_) { \
+      _Pragma("unroll") for (int y_ = 0; y_ < 4; ++y_) { \
+        acc[x_][y_] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb1_[x_], fa1_[y_], acc[x_][y_], 0, 0, 0); \
+      } \
+    } \
+  } while (0)
+    dma(ls0);
+    dma(ls1);
+    dma(ls2);
+    for (int t = 0; t < nact; t += 4) { /* one exit: the accumulators stay in their registers */
+      OZ_STAGE(ls0, ls3);
+      OZ_STAGE(ls1, ls0);
+      OZ_STAGE(ls2, ls1);
+      OZ_STAGE(ls3, ls2);
     }
+#undef OZ_STAGE
+    __builtin_amdgcn_s_waitcnt(0); /* the trailing zero-tile loads land before the work-group ends */
   }
   int* const out = c32 + (long)blockIdx.z * Mp * Np;
 #pragma unroll
@@ -469,6 +493,13 @@ extern "C" int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int
 extern "C" int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, const int* occa, const int* occb,
   const int* items, int nitems, int s, int Mp, int Np, int Kp, int* c32, void* hip_stream) {
   if (0 != Mp % LT || 0 != Np % LT || 0 != Kp % LK || 0 >= Kp || 1 > s || 8 < s || 0 >= nitems) return LXS_ELIMIT;
+  for (int z = 0; z < nitems; ++z) { /* units of 64 k within the level's pairs, at most 2080 per item */
+    const int L = items[4 * z], plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
+    if (0 > L || 2 * (s - 1) < L || 0 > items[4 * z + 1] || items[4 * z + 1] >= items[4 * z + 2] ||
+        2080 < items[4 * z + 2] - items[4 * z + 1] || (phi - plo + 1) * (Kp / QK) < items[4 * z + 2]) {
+      return LXS_ELIMIT;
+    }
+  }
   for (int z0 = 0; z0 < nitems; z0 += OZ_ITEMS) {
     const int nz = (nitems - z0 < OZ_ITEMS ? nitems - z0 : OZ_ITEMS);
     oz_items_t arg;
