@@ -226,7 +226,9 @@ def _event_ms(lx, stream, fn, reps):
     return ms
 
 
-OZAKI_CONFIGS = ((7, 6), (8, 7), (8, 0))  # (slices, trim): triangle at 7 and 8 slices, the reference default
+# (kind, slices or primes, trim): scheme II with 16 moduli (the reference's default kind 3 picks it at these
+# sizes), scheme I over the triangle at 7 and 8 slices, and scheme I at the reference's default 8 slices
+OZAKI_CONFIGS = ((2, 16, 0), (1, 7, 6), (1, 8, 7), (1, 8, 0))
 
 
 def ozaki_section(lx, stream, sizes, reps=3):
@@ -257,8 +259,8 @@ def ozaki_section(lx, stream, sizes, reps=3):
                     entry["rocblas_tflops"] = round(flops / ms * 1e-9, 2)
                     entry["rocblas_error"] = _sampled_error(A, B, out, rng)
                 entry["configs"] = []
-            for cfg_i, (sl, trim) in enumerate(OZAKI_CONFIGS):
-                oz = lx.Ozaki(nslices=sl, trim=trim)
+            for cfg_i, (kind, sl, trim) in enumerate(OZAKI_CONFIGS):
+                oz = lx.Ozaki(nslices=sl, trim=trim) if kind == 1 else lx.Ozaki(kind=kind, nprimes=sl, trim=trim)
                 pairs = sum(1 for p in range(sl) for q in range(sl) if p + q <= 2 * (sl - 1) - trim)
                 if e == 10:
                     lx.profile_enable(True)
@@ -267,11 +269,14 @@ def ozaki_section(lx, stream, sizes, reps=3):
                                                                stream), reps)
                     i8 = lx.profile_get("ozaki_i8")
                     pre = lx.profile_get("ozaki_pre")
+                    rec = lx.profile_get("ozaki_fp64")
                     lx.profile_enable(False)
                     i8_ms = i8["ms"] / max(i8["count"], 1)
                     best_native = min(entry["native_ms"], entry.get("rocblas_ms", float("inf")))
+                    products = pairs if kind == 1 else int(round(i8["flops"] / max(i8["count"], 1) / (2.0 * n ** 3)))
                     entry["configs"].append({
-                        "slices": sl, "trim": trim, "pairs": pairs, "ms": round(ms, 4),
+                        "scheme": kind, ("slices" if kind == 1 else "max_primes"): sl, "trim": trim,
+                        ("pairs" if kind == 1 else "primes"): products, "ms": round(ms, 4),
                         "fp64_equiv_tflops": round(flops / ms * 1e-9, 2),
                         "speedup_vs_native": round(entry["native_ms"] / ms, 3),
                         "speedup_vs_best_native": round(best_native / ms, 3),
@@ -279,6 +284,7 @@ def ozaki_section(lx, stream, sizes, reps=3):
                         "i8_tops": round(i8["flops"] / max(i8["count"], 1) / i8_ms * 1e-9, 1),
                         "i8_frac_of_5pops": round(i8["flops"] / max(i8["count"], 1) / i8_ms * 1e-9 / 5000.0, 4),
                         "preprocess_ms": round(pre["ms"] / max(pre["count"], 1), 4),
+                        "recombine_ms": round(rec["ms"] / max(rec["count"], 1), 4),
                         "errors": {}})
                 else:
                     oz.gemm("N", "N", n, n, n, 1.0, da, n, db, n, 0.0, dc, n, stream)

@@ -398,12 +398,16 @@ class Batch:
 
 
 class Ozaki:
-    """FP64/FP32 GEMM emulated with int8 matrix cores (include/ozaki.h, Ozaki scheme I)."""
+    """FP64/FP32 GEMM emulated with int8 matrix cores (include/ozaki.h). kind 1: Ozaki scheme I (nslices
+    digit slices, trim drops levels); kind 2: scheme II (CRT, at most nprimes moduli, trim drops 2 trim
+    mantissa bits); kind 3: per call, whichever the cost model picks (nprimes for scheme II, default
+    slices for scheme I), as ozaki_init resolves ndecomp in the reference."""
 
-    def __init__(self, use_double=True, nslices=0, trim=0, flags=3, maxk=0, verbosity=0):
+    def __init__(self, use_double=True, nslices=0, trim=0, flags=3, maxk=0, verbosity=0, kind=1, nprimes=0):
         self.ctx = OzakiContext()
-        check(lib().ozaki_init(ctypes.byref(self.ctx), 0, 0, 1 if use_double else 0, 1, verbosity, nslices, flags, trim,
-                               0, maxk), "ozaki_init")
+        ndecomp = nslices if kind == 1 else nprimes
+        check(lib().ozaki_init(ctypes.byref(self.ctx), 0, 0, 1 if use_double else 0, kind, verbosity, ndecomp, flags,
+                               trim, 0, maxk), "ozaki_init")
 
     def gemm(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None, dev=True):
         """C = alpha op(A) op(B) + beta C (column-major). dev: a, b, c are DeviceBuffers (else numpy arrays)."""

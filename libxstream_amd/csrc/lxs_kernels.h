@@ -76,14 +76,28 @@ int lxs_zero_launch(void* ptr, size_t nbytes, void* hip_stream);
    planes: s x Rp x Kp int8 (Rp multiple of 256, Kp multiple of 128); e: Rp ints; occ: 8 ints */
 int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int s, int Rp, int Kp, int* e,
   int* occ, signed char* planes, void* hip_stream);
-/* int8 level GEMM over work items {level, u0, u1, pad} (host array): int32 partial z at c32 + z Mp Np,
-   element (i, j) at j Mp + i; an item holds at most 2080 units (pair, k-block of 64) */
-int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, const int* occa, const int* occb,
-  const int* items, int nitems, int s, int Mp, int Np, int Kp, int* c32, void* hip_stream);
+/* int8 GEMM over work items {level, u0, u1, pad} (host array), element (i, j) of item z at
+   out + z Mp Np + j Mp + i. crt = 0 (scheme 1): s digit planes, item level L over the pairs (p, L - p),
+   int32 partials, at most 2080 units (pair, k-block of 64) per item. crt = 1 (scheme 2): s residue
+   planes, item level = prime, sums mod m_prime as bytes, at most 2047 units (k-blocks) per item. */
+int lxs_oz_level_gemm(int crt, const signed char* pa, const signed char* pb, const int* occa, const int* occb,
+  const int* items, int nitems, int s, int Mp, int Np, int Kp, void* out, void* hip_stream);
 /* FP64 recombination of the partials: zb[L] = first item of level L (cutoff + 2 entries) */
 int lxs_oz_recombine(int dtype, const int* c32, const int* zb, int cutoff, int Mp, int Np, const int* ea,
   const int* eb, int M, int N, double alpha, double beta, void* c, int ldc, void* hip_stream);
 int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* hip_stream);
+/* Ozaki scheme 2 (CRT): moduli in use order, pairwise coprime and at most 256, so that symmetric residues
+   fit int8; the first np of them serve a call. Residue planes: np x Rp x Kp int8 of the integers
+   trunc(|v| 2^(mb - trim - E)) (E: the row's exponent, mb = 53 or 24) with the sign of v. */
+#define LXS_OZ_NPRIMES_MAX 16
+#define LXS_OZ_MODULI 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191
+int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int trim, int Rp, int Kp,
+  int* e, signed char* planes, void* hip_stream);
+/* C = alpha 2^(ea + eb + sh0) x + beta C, x the integer of the residues (sum of the residue bytes of the
+   prime's items zb[p] .. zb[p + 1] - 1, mod m_p), hm = M / 2 and m = M as {high, low} 64-bit words */
+int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, const unsigned long long* hm,
+  const unsigned long long* m, int Mp, int Np, const int* ea, const int* eb, int M, int N, int sh0, double alpha,
+  double beta, void* c, int ldc, void* hip_stream);
 int lxs_dgemm(const double* a, long ars, long aks, const double* b, long bks, long bcs, int M, int N, int K,
   double alpha, double beta, double* c, int ldc, void* hip_stream);
 /* complex 3M helpers: split interleaved (rows x cols, ld) into re, im, re + im (ld rows), conj negates im;

@@ -1,21 +1,29 @@
 /*
- * Ozaki scheme I host layer: context, device scratch, K passes and the complex 3M method.
+ * Ozaki host layer: context, scheme selection, device scratch, K passes and the complex 3M method.
  *
  * Entry points and argument meaning follow samples/ozaki/ozaki_opencl.h:328-440; settings are
- * resolved as in samples/ozaki/ozaki_opencl.c:348-475 (defaults fp64 8 slices, fp32 4; cutoff
- * 2(nslices-1) - oztrim). The device work is ozaki.hip: per-operand preprocessing into int8 digit
- * planes, then one fused level GEMM per K pass.
+ * resolved as in samples/ozaki/ozaki_opencl.c:348-475 (defaults fp64 8 slices / 16 primes, fp32
+ * 4 / 9; Scheme 1 cutoff 2(nslices-1) - oztrim; Scheme 2 drops 2 oztrim mantissa bits) and the
+ * adaptive choice as in samples/ozaki/ozaki_gemm.c:520-587 (Scheme 1 while its pair count is below
+ * the prime count plus the reconstruction's share). The device work is ozaki.hip: per-operand
+ * preprocessing into int8 planes (digits or residues), one int8 GEMM launch per K pass, then the FP64
+ * recombination (Scheme 1) or the CRT reconstruction (Scheme 2).
  */
 #include "lxs_internal.h"
 #include "../../include/ozaki.h"
 #include <string.h>
 
 #define OZ_ALIGN 256
-#define OZ_TILE_MN 256    /* output tile of the int8 level GEMM; row padding of the digit planes */
-#define OZ_TILE_K 128     /* k padding of the digit planes */
-#define OZ_UNIT_K 64      /* k per unit of the level GEMM */
-#define OZ_FLUSH_MAX 2080 /* units per work item: 2080 * 64 * 127^2 < 2^31 (exact int32 partials) */
+#define OZ_TILE_MN 256    /* output tile of the int8 GEMM; row padding of the planes */
+#define OZ_TILE_K 128     /* k padding of the planes */
+#define OZ_UNIT_K 64      /* k per unit of the int8 GEMM */
+#define OZ_FLUSH_MAX 2080 /* units per Scheme 1 work item: 2080 * 64 * 127^2 < 2^31 (exact int32 partials) */
+#define OZ_FLUSH_CRT 2047 /* units per Scheme 2 work item: 2047 * 64 * 128^2 < 2^31 */
 #define OZ_WG_TARGET 1024 /* work-groups the item plan aims for */
+#define OZ_KPASS_MAX ((1 << 22) - OZ_TILE_K) /* K per pass (32-bit operand offsets of the int8 GEMM) */
+#define OZ_XOVER 16.0     /* adaptive choice: reconstruction cost per prime^2, in int8 GEMMs of K = 1 */
+
+typedef unsigned __int128 oz_u128_t;
 
 typedef struct oz_impl_t {
   int* items; /* work items {level, u0, u1, pad} of the current call */
@@ -28,6 +36,8 @@ typedef struct oz_impl_t {
   size_t carena_size;
 } oz_impl_t;
 
+static const int oz_moduli[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
+
 static size_t oz_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 static int oz_env_int(const char* name, int fallback) {
@@ -35,37 +45,79 @@ static int oz_env_int(const char* name, int fallback) {
   return (NULL != env && '\0' != *env) ? atoi(env) : fallback;
 }
 
+static int oz_mant_bits(const ozaki_context_t* ctx) { return 0 != ctx->use_double ? 53 : 24; }
+
+/* mantissa bits Scheme 2 drops: 2 per trim level, kind 2 only (samples/ozaki/ozaki_opencl.c:428-431) */
+static int oz_crt_trim(const ozaki_context_t* ctx) {
+  const int levels = (oz_mant_bits(ctx) - 1) / 2;
+  return (2 == ctx->kind ? 2 * (ctx->oztrim < levels ? ctx->oztrim : levels) : 0);
+}
+
+/* product of the first np moduli */
+static oz_u128_t oz_crt_product(int np) {
+  oz_u128_t m = 1;
+  int p;
+  for (p = 0; p < np; ++p) m *= (oz_u128_t)oz_moduli[p];
+  return m;
+}
+
+/* the integer product of a K pass is exact under np primes when 2 kk (2^bits - 1)^2 < M (|x| < M / 2) */
+static int oz_crt_exact(int np, int bits, long kk) {
+  const oz_u128_t amax = ((oz_u128_t)1 << bits) - 1;
+  return (oz_crt_product(np) - 1) / (2 * amax * amax) >= (oz_u128_t)kk;
+}
+
+/* fewest primes (at most npmax) keeping a K pass of kk exact, 0 if npmax do not */
+static int oz_crt_primes(int npmax, int bits, long kk) {
+  int np;
+  for (np = 1; np <= npmax; ++np) {
+    if (oz_crt_exact(np, bits, kk)) return np;
+  }
+  return 0;
+}
+
+/* longest K pass npmax primes keep exact (at most 2^30) */
+static long oz_crt_kmax(int npmax, int bits) {
+  const oz_u128_t amax = ((oz_u128_t)1 << bits) - 1;
+  const oz_u128_t k = (oz_crt_product(npmax) - 1) / (2 * amax * amax);
+  return (k < ((oz_u128_t)1 << 30) ? (long)k : (1L << 30));
+}
+
 LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_double, int kind, int verbosity, int ndecomp,
   int ozflags, int oztrim, int ozgroups, int maxk) {
   oz_impl_t* impl;
-  int nslices;
+  int nslices, nprimes;
   (void)ozgroups;
   if (NULL == ctx) return EXIT_FAILURE;
   memset(ctx, 0, sizeof(*ctx));
   if (0 >= kind || 3 < kind) kind = 3; /* no request: adaptive */
-  if (2 == kind) {
-    lxs_error(EXIT_FAILURE, "ozaki_init: Scheme 2 (CRT) is not available, use kind 1 or 0");
-    return EXIT_FAILURE;
-  }
   if ((0 != tm && OZ_TILE_MN != tm) || (0 != tn && OZ_TILE_MN != tn)) {
     lxs_error(EXIT_FAILURE, "ozaki_init: the output tile is 256 x 256");
     return EXIT_FAILURE;
   }
-  nslices = (0 < ndecomp ? ndecomp : (0 != use_double ? 8 : 4));
+  nslices = (0 != use_double ? 8 : 4);
+  nprimes = (0 != use_double ? 16 : 9);
+  if (0 < ndecomp) { /* the slice count for kind 1, else the prime count */
+    if (1 == kind) nslices = ndecomp;
+    else nprimes = ndecomp;
+  }
   if (8 < nslices) nslices = 8; /* 8 digits of 7 bits hold a full FP64 mantissa */
+  if (LXS_OZ_NPRIMES_MAX < nprimes) nprimes = LXS_OZ_NPRIMES_MAX; /* 16 moduli: 125 bits */
+  if (2 > nprimes) nprimes = 2;
   if (0 > ozflags) ozflags = OZAKI_TRIANGULAR | OZAKI_SYMMETRIZE;
-  if (1 == ozflags || 2 == ozflags) {
+  if (2 != kind && (1 == ozflags || 2 == ozflags)) { /* Scheme 1 pair traversal */
     lxs_error(EXIT_FAILURE, "ozaki_init: ozflags 1 and 2 (symmetric operands only) are not supported, use 0 or 3");
     return EXIT_FAILURE;
   }
   if (0 > oztrim) oztrim = 0;
-  if (oztrim > 2 * (nslices - 1)) oztrim = 2 * (nslices - 1);
+  if (2 != kind && oztrim > 2 * (nslices - 1)) oztrim = 2 * (nslices - 1);
   impl = (oz_impl_t*)calloc(1, sizeof(oz_impl_t));
   if (NULL == impl) return EXIT_FAILURE;
   ctx->use_double = (0 != use_double ? 1 : 0);
-  ctx->kind = 1;
-  ctx->nslices = ctx->ndecomp = nslices;
-  ctx->nprimes = 0;
+  ctx->kind = kind;
+  ctx->nslices = nslices;
+  ctx->nprimes = nprimes;
+  ctx->ndecomp = (1 == kind ? nslices : nprimes);
   ctx->ozflags = ozflags;
   ctx->oztrim = oztrim;
   ctx->verbosity = verbosity;
@@ -75,9 +127,10 @@ LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_doub
   if (1 > ctx->flush || OZ_FLUSH_MAX < ctx->flush) ctx->flush = OZ_FLUSH_MAX;
   ctx->impl = impl;
   if (0 != verbosity) {
-    fprintf(stderr, "INFO OZAKI: scheme 1, %s, nslices=%d trim=%d cutoff=%d pairs=%d tile=%dx%d\n",
-      ctx->use_double ? "fp64" : "fp32", nslices, oztrim, 2 * (nslices - 1) - oztrim,
-      ozaki_count_pairs(nslices, 2 * (nslices - 1) - oztrim, 0), OZ_TILE_MN, OZ_TILE_MN);
+    const int cutoff = 2 * (nslices - 1) - (2 == kind ? 0 : oztrim);
+    fprintf(stderr, "INFO OZAKI: kind=%d %s, scheme 1: nslices=%d cutoff=%d pairs=%d; scheme 2: primes<=%d trim=%d bits; tile=%dx%d\n",
+      kind, ctx->use_double ? "fp64" : "fp32", nslices, cutoff, ozaki_count_pairs(nslices, cutoff, 0), nprimes,
+      oz_crt_trim(ctx), OZ_TILE_MN, OZ_TILE_MN);
   }
   return EXIT_SUCCESS;
 }
@@ -106,26 +159,61 @@ static int oz_trans(char t) {
   }
 }
 
+/* Scheme 1 level cutoff (kind 2 keeps oztrim for the mantissa) */
+static int oz_cutoff(const ozaki_context_t* ctx) {
+  return 2 * (ctx->nslices - 1) - (2 == ctx->kind ? 0 : ctx->oztrim);
+}
+
 /*
- * Work items of the level GEMM: every level L = 0 .. cutoff has npairs(L) * nkb units (pair, k-block of 64),
- * cut into items of at most cu units, cu chosen so that items x tiles is about OZ_WG_TARGET
- * work-groups (at least 16 units per item, at most ctx->flush). zb[L] = first item of level L.
- * Returns the number of items; fills items (4 ints each) when not NULL.
+ * The scheme of a call (1 or 2, 0 when kind 2 has too few primes for even K = 1) and, for Scheme 2, its
+ * prime count and K pass: kind 1 and 2 force their scheme; kind 3 takes Scheme 1 while
+ * pairs < np + OZ_XOVER np^2 / K (int8 GEMMs of Scheme 1 against those of Scheme 2 plus its
+ * reconstruction, per unit of K).
  */
-static int oz_plan(const ozaki_context_t* ctx, int nkb, int tiles, int* items, int* zb) {
-  const int s = ctx->nslices, cutoff = 2 * (ctx->nslices - 1) - ctx->oztrim;
+static int oz_scheme(const ozaki_context_t* ctx, int K, int* np, int* kpass) {
+  const int bits = oz_mant_bits(ctx) - oz_crt_trim(ctx);
+  long kp = oz_crt_kmax(ctx->nprimes, bits);
+  int scheme;
+  if (0 < ctx->maxk && ctx->maxk < kp) kp = ctx->maxk;
+  if (OZ_KPASS_MAX < kp) kp = OZ_KPASS_MAX;
+  if (K < kp) kp = (0 < K ? K : 1);
+  *np = oz_crt_primes(ctx->nprimes, bits, kp);
+  *kpass = (int)kp;
+  if (1 == ctx->kind || (0 == *np && 2 != ctx->kind)) scheme = 1;
+  else if (2 == ctx->kind) scheme = (0 != *np ? 2 : 0);
+  else {
+    const int pairs = ozaki_count_pairs(ctx->nslices, oz_cutoff(ctx), 0);
+    scheme = ((double)pairs < *np + OZ_XOVER * *np * *np / (0 < K ? K : 1)) ? 1 : 2;
+  }
+  if (1 == scheme) {
+    *kpass = (0 < ctx->maxk && ctx->maxk < K) ? ctx->maxk : (0 < K ? K : 1);
+    if (OZ_KPASS_MAX < *kpass) *kpass = OZ_KPASS_MAX;
+  }
+  return scheme;
+}
+
+/*
+ * Work items of the int8 GEMM. Scheme 1: every level L = 0 .. cutoff has npairs(L) * nkb units (pair,
+ * k-block of 64); Scheme 2: every prime has nkb units. Levels are cut into items of at most cu units, cu
+ * chosen so that items x tiles is about OZ_WG_TARGET work-groups (at least 16 units per item, at most
+ * ctx->flush, and 2047 for Scheme 2). zb[L] = first item of level L. Returns the number of items; fills
+ * items (4 ints each) when not NULL.
+ */
+static int oz_plan(const ozaki_context_t* ctx, int crt, int np, int nkb, int tiles, int* items, int* zb) {
+  const int s = ctx->nslices, cutoff = (0 != crt ? np - 1 : oz_cutoff(ctx));
+  const int cmax = (0 != crt && OZ_FLUSH_CRT < ctx->flush ? OZ_FLUSH_CRT : ctx->flush);
   long total = 0;
   int L, n = 0, cu;
   for (L = 0; L <= cutoff; ++L) {
     const int plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
-    total += (long)(phi - plo + 1) * nkb;
+    total += (long)(0 != crt ? 1 : phi - plo + 1) * nkb;
   }
   cu = (int)((total * tiles + OZ_WG_TARGET - 1) / OZ_WG_TARGET);
   if (cu < 16) cu = 16;
-  if (cu > ctx->flush) cu = ctx->flush;
+  if (cu > cmax) cu = cmax;
   for (L = 0; L <= cutoff; ++L) {
     const int plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
-    const int units = (phi - plo + 1) * nkb;
+    const int units = (0 != crt ? 1 : phi - plo + 1) * nkb;
     int u;
     if (NULL != zb) zb[L] = n;
     for (u = 0; u < units; u += cu, ++n) {
@@ -142,32 +230,34 @@ static int oz_plan(const ozaki_context_t* ctx, int nkb, int tiles, int* items, i
 }
 
 typedef struct oz_layout_t {
-  int Mp, Np, Kp, kpass, nitems;
-  size_t planes_a, planes_b, ea, eb, occ, c32, a, b, c, total; /* byte offsets and total */
+  int Mp, Np, Kp, kpass, nitems, crt, np, scheme;
+  size_t planes_a, planes_b, ea, eb, occ, out, a, b, c, total; /* byte offsets and total */
 } oz_layout_t;
 
-static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
-  int dev, oz_layout_t* l) {
+static void oz_layout(const ozaki_context_t* ctx, int crt, int np, int kpass, int ta, int tb, int M, int N, int K,
+  int lda, int ldb, int ldc, int dev, oz_layout_t* l) {
   const size_t ts = (0 != ctx->use_double ? 8 : 4);
-  const int kpass = (0 < ctx->maxk && ctx->maxk < K) ? ctx->maxk : K;
+  const int nplanes = (0 != crt ? np : ctx->nslices);
   size_t off = 0;
+  l->crt = crt;
+  l->np = np;
   l->Mp = (int)oz_up((size_t)(0 < M ? M : 1), OZ_TILE_MN);
   l->Np = (int)oz_up((size_t)(0 < N ? N : 1), OZ_TILE_MN);
   l->Kp = (int)oz_up((size_t)(0 < kpass ? kpass : 1), OZ_TILE_K);
   l->kpass = kpass;
-  l->nitems = oz_plan(ctx, l->Kp / OZ_UNIT_K, (l->Mp / OZ_TILE_MN) * (l->Np / OZ_TILE_MN), NULL, NULL);
+  l->nitems = oz_plan(ctx, crt, np, l->Kp / OZ_UNIT_K, (l->Mp / OZ_TILE_MN) * (l->Np / OZ_TILE_MN), NULL, NULL);
   l->planes_a = off;
-  off += oz_up((size_t)ctx->nslices * l->Mp * l->Kp, OZ_ALIGN);
+  off += oz_up((size_t)nplanes * l->Mp * l->Kp, OZ_ALIGN);
   l->planes_b = off;
-  off += oz_up((size_t)ctx->nslices * l->Np * l->Kp, OZ_ALIGN);
+  off += oz_up((size_t)nplanes * l->Np * l->Kp, OZ_ALIGN);
   l->ea = off;
   off += oz_up(sizeof(int) * (size_t)l->Mp, OZ_ALIGN);
   l->eb = off;
   off += oz_up(sizeof(int) * (size_t)l->Np, OZ_ALIGN);
   l->occ = off;
   off += OZ_ALIGN; /* 2 x 8 ints */
-  l->c32 = off;
-  off += oz_up(sizeof(int) * (size_t)l->nitems * l->Mp * l->Np, OZ_ALIGN);
+  l->out = off; /* int32 partials (Scheme 1) or residue bytes (Scheme 2) per item */
+  off += oz_up((0 != crt ? 1 : sizeof(int)) * (size_t)l->nitems * l->Mp * l->Np, OZ_ALIGN);
   l->a = l->b = l->c = 0;
   if (0 == dev) {
     l->a = off;
@@ -180,12 +270,21 @@ static void oz_layout(const ozaki_context_t* ctx, int ta, int tb, int M, int N, 
   l->total = off;
 }
 
+/* layout of a call: the scheme oz_scheme picks for it */
+static void oz_layout_call(const ozaki_context_t* ctx, int ta, int tb, int M, int N, int K, int lda, int ldb, int ldc,
+  int dev, oz_layout_t* l) {
+  int np, kpass;
+  const int scheme = oz_scheme(ctx, K, &np, &kpass);
+  oz_layout(ctx, 2 == scheme, np, kpass, ta, tb, M, N, K, lda, ldb, ldc, dev, l);
+  l->scheme = scheme;
+}
+
 LIBXSTREAM_API size_t ozaki_scratch_size(const ozaki_context_t* ctx, char transa, char transb, int M, int N, int K,
   int lda, int ldb, int ldc) {
   oz_layout_t l;
   const int ta = oz_trans(transa), tb = oz_trans(transb);
   if (NULL == ctx || 0 > ta || 0 > tb || 0 > M || 0 > N || 0 > K) return 0;
-  oz_layout(ctx, ta, tb, M, N, K, lda, ldb, ldc, 0 /*host operands: upper bound*/, &l);
+  oz_layout_call(ctx, ta, tb, M, N, K, lda, ldb, ldc, 0 /*host operands: upper bound*/, &l);
   return l.total;
 }
 
@@ -264,7 +363,7 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
   const size_t ts = (0 != ctx->use_double ? 8 : 4);
   const int arows = (0 == ta ? M : K), acols = (0 == ta ? K : M);
   const int brows = (0 == tb ? K : N), bcols = (0 == tb ? N : K);
-  const int s = ctx->nslices, cutoff = 2 * (ctx->nslices - 1) - ctx->oztrim;
+  const int s = ctx->nslices, cutoff = oz_cutoff(ctx);
   oz_layout_t l;
   char* base;
   const char *da = (const char*)a, *db = (const char*)b;
@@ -275,7 +374,8 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
   }
   if (0 == M || 0 == N) return EXIT_SUCCESS;
   if ((0 != K && (NULL == a || NULL == b)) || NULL == c) return lxs_error(EXIT_FAILURE, "ozaki_gemm: NULL operand");
-  oz_layout(ctx, ta, tb, M, N, K, lda, ldb, ldc, dev, &l);
+  oz_layout_call(ctx, ta, tb, M, N, K, lda, ldb, ldc, dev, &l);
+  if (0 == l.scheme) return lxs_error(EXIT_FAILURE, "ozaki_gemm: too few CRT primes for an exact product");
   base = oz_scratch(impl, l.total, hs);
   if (NULL == base) return EXIT_FAILURE;
   if (0 == dev) { /* host operands: copy in */
@@ -300,39 +400,70 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
     /* op(A)(i, k) at da[i * ars + k * aks]; op(B)(k, j) at db[k * bks + j * bcs] */
     const long ars = (0 == ta ? 1 : lda), aks = (0 == ta ? lda : 1);
     const long bks = (0 == tb ? 1 : ldb), bcs = (0 == tb ? ldb : 1);
+    const int tiles = (l.Mp / OZ_TILE_MN) * (l.Np / OZ_TILE_MN);
+    const int trim = oz_crt_trim(ctx), mb = oz_mant_bits(ctx);
     int* const occ = (int*)(base + l.occ);
-    const double npairs = ozaki_count_pairs(s, cutoff, 0);
-    int zb[16];
+    int zb[LXS_OZ_NPRIMES_MAX + 1];
     if (impl->items_cap < l.nitems) {
       int* const items = (int*)realloc(impl->items, sizeof(int) * 4 * (size_t)l.nitems);
       if (NULL == items) return lxs_error(EXIT_FAILURE, "ozaki_gemm: out of memory");
       impl->items = items;
       impl->items_cap = l.nitems;
     }
+    if (0 != ctx->verbosity && 1 < ctx->verbosity) {
+      fprintf(stderr, "INFO OZAKI: M=%d N=%d K=%d scheme %d, %d %s, K pass %d\n", M, N, K, 0 != l.crt ? 2 : 1,
+        0 != l.crt ? l.np : ozaki_count_pairs(s, cutoff, 0), 0 != l.crt ? "primes" : "pairs", l.kpass);
+    }
     for (k0 = 0; k0 < K && 0 == result; k0 += l.kpass) {
       const int kk = (K - k0 < l.kpass ? K - k0 : l.kpass);
       const int Kp = (int)oz_up((size_t)kk, OZ_TILE_K);
-      const int nitems = oz_plan(ctx, Kp / OZ_UNIT_K, (l.Mp / OZ_TILE_MN) * (l.Np / OZ_TILE_MN), impl->items, zb);
+      /* the last pass may need fewer primes */
+      const int np = (0 != l.crt ? oz_crt_primes(l.np, mb - trim, kk) : 0);
+      const int nitems = oz_plan(ctx, l.crt, np, Kp / OZ_UNIT_K, tiles, impl->items, zb);
+      const double nprod = (0 != l.crt ? np : ozaki_count_pairs(s, cutoff, 0));
+      const int nplanes = (0 != l.crt ? np : s);
       void* token = lxs_profile_begin(hs);
-      result = lxs_oz_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, s, l.Mp, Kp, (int*)(base + l.ea), occ,
-        (signed char*)(base + l.planes_a), hs);
-      if (0 == result) {
-        result = lxs_oz_preprocess(dtype, db + ts * (size_t)k0 * bks, bcs, bks, N, kk, s, l.Np, Kp, (int*)(base + l.eb),
-          occ + 8, (signed char*)(base + l.planes_b), hs);
+      if (0 != l.crt) {
+        result = lxs_oz_crt_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, np, trim, l.Mp, Kp,
+          (int*)(base + l.ea), (signed char*)(base + l.planes_a), hs);
+        if (0 == result) {
+          result = lxs_oz_crt_preprocess(dtype, db + ts * (size_t)k0 * bks, bcs, bks, N, kk, np, trim, l.Np, Kp,
+            (int*)(base + l.eb), (signed char*)(base + l.planes_b), hs);
+        }
       }
-      lxs_profile_end(token, hs, "ozaki_pre", 0.0, (double)ts * ((double)M + N) * kk + (double)s * (l.Mp + l.Np) * Kp);
-      if (0 == result) { /* int8 work of every pair up to the cutoff (pairs of all-zero planes are skipped) */
+      else {
+        result = lxs_oz_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, s, l.Mp, Kp, (int*)(base + l.ea), occ,
+          (signed char*)(base + l.planes_a), hs);
+        if (0 == result) {
+          result = lxs_oz_preprocess(dtype, db + ts * (size_t)k0 * bks, bcs, bks, N, kk, s, l.Np, Kp, (int*)(base + l.eb),
+            occ + 8, (signed char*)(base + l.planes_b), hs);
+        }
+      }
+      lxs_profile_end(token, hs, "ozaki_pre", 0.0, (double)ts * ((double)M + N) * kk + (double)nplanes * (l.Mp + l.Np) * Kp);
+      if (0 == result) { /* int8 work of every pair up to the cutoff (pairs of all-zero planes are skipped), or every prime */
         token = lxs_profile_begin(hs);
-        result = lxs_oz_level_gemm((const signed char*)(base + l.planes_a), (const signed char*)(base + l.planes_b), occ,
-          occ + 8, impl->items, nitems, s, l.Mp, l.Np, Kp, (int*)(base + l.c32), hs);
-        lxs_profile_end(token, hs, "ozaki_i8", 2.0 * npairs * l.Mp * l.Np * Kp,
-          (double)s * (l.Mp + l.Np) * Kp + 4.0 * nitems * l.Mp * l.Np);
+        result = lxs_oz_level_gemm(l.crt, (const signed char*)(base + l.planes_a), (const signed char*)(base + l.planes_b),
+          occ, occ + 8, impl->items, nitems, nplanes, l.Mp, l.Np, Kp, base + l.out, hs);
+        lxs_profile_end(token, hs, "ozaki_i8", 2.0 * nprod * l.Mp * l.Np * Kp,
+          (double)nplanes * (l.Mp + l.Np) * Kp + (0 != l.crt ? 1.0 : 4.0) * nitems * l.Mp * l.Np);
       }
       if (0 == result) {
+        const double bc = (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2);
         token = lxs_profile_begin(hs);
-        result = lxs_oz_recombine(dtype, (const int*)(base + l.c32), zb, cutoff, l.Mp, l.Np, (const int*)(base + l.ea),
-          (const int*)(base + l.eb), M, N, alpha, (0 == k0 ? beta : 1.0), dc, ldc, hs);
-        lxs_profile_end(token, hs, "ozaki_fp64", 0.0, 4.0 * nitems * M * N + (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2));
+        if (0 != l.crt) {
+          const oz_u128_t m = oz_crt_product(np), hm = m >> 1;
+          const unsigned long long mw[2] = {(unsigned long long)(m >> 64), (unsigned long long)m};
+          const unsigned long long hw[2] = {(unsigned long long)(hm >> 64), (unsigned long long)hm};
+          result = lxs_oz_crt_recombine(dtype, (const unsigned char*)(base + l.out), zb, np, hw, mw, l.Mp, l.Np,
+            (const int*)(base + l.ea), (const int*)(base + l.eb), M, N, 2 * trim - 2 * mb, alpha, (0 == k0 ? beta : 1.0), dc,
+            ldc, hs);
+          lxs_profile_end(token, hs, "ozaki_fp64", 0.0, (double)nitems * M * N + bc);
+        }
+        else {
+          result = lxs_oz_recombine(dtype, (const int*)(base + l.out), zb, cutoff, l.Mp, l.Np, (const int*)(base + l.ea),
+            (const int*)(base + l.eb), M, N, alpha, (0 == k0 ? beta : 1.0), dc, ldc, hs);
+          lxs_profile_end(token, hs, "ozaki_fp64", 0.0, 4.0 * nitems * M * N + bc);
+        }
       }
     }
   }

@@ -87,7 +87,7 @@ __device__ __forceinline__ void oz_load_tile(double (*t)[OZ_TILE + 1], const T* 
 __global__ void __launch_bounds__(256) oz_init(int* __restrict__ e, int n, int* __restrict__ occ) {
   const int i = blockIdx.x * 256 + threadIdx.x;
   if (i < n) e[i] = OZ_EXP_NONE;
-  if (0 == blockIdx.x && (int)threadIdx.x < 8) occ[threadIdx.x] = 0;
+  if (NULL != occ && 0 == blockIdx.x && (int)threadIdx.x < 8) occ[threadIdx.x] = 0;
 }
 
 template <typename T>
@@ -161,9 +161,12 @@ constexpr int LT = 256;        /* int8 level GEMM: output tile (i and j) per wor
 constexpr int LK = 128;        /* k padding of the digit planes */
 
 /*
- * One work item: level L (pairs p = plo .. phi, q = L - p), units u0 .. u1 of the level (unit u is
- * pair plo + u / nkb, k-block u % nkb of 64 k). Every item writes its own int32 partial sums, exact
- * because an item holds at most 2080 units: 2080 * 64 * 127^2 < 2^31.
+ * One work item of the int8 GEMM, in units of 64 k:
+ * - Scheme 1: level L, pairs p = plo .. phi with q = L - p; unit u is pair plo + u / nkb, k-block u % nkb.
+ *   The item writes its own int32 partial sums, exact because it holds at most 2080 units:
+ *   2080 * 64 * 127^2 < 2^31.
+ * - Scheme 2 (CRT): prime P, A and B residue planes P; unit u is k-block u. The item reduces its sums
+ *   mod m_P and writes the residue as one byte (|residue| <= 128, at most 2047 units: 2047 * 64 * 2^14 < 2^31).
  */
 struct oz_item_t {
   int level, u0, u1, pad;
@@ -173,22 +176,31 @@ struct oz_items_t {
   oz_item_t it[OZ_ITEMS];
 };
 
+/* CRT moduli in use order (pairwise coprime, at most 256: symmetric residues fit int8) */
+__constant__ int oz_moduli[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
+
 /*
- * C32[z](j, i) = sum over the item's units of sum_k A_p(i, k) B_q(j, k), stored at z * Mp * Np + j * Mp + i.
+ * C32[z](j, i) = sum over the item's units of sum_k A_p(i, k) B_q(j, k), stored at z * Mp * Np + j * Mp + i
+ * (Scheme 2: that sum mod m_P, one byte).
  * v_mfma_i32_32x32x32_i8 with the B planes as the first operand (rows j) and the A planes as the
  * second (columns i): lane l of D holds column i = l % 32 and rows 8(r/4) + 4(l/32) + r%4, so for a
- * fixed register the 32 lanes of a half-wave store 128 contiguous bytes along i.
+ * fixed register the 32 lanes of a half-wave store contiguous elements along i.
  * 4 wavefronts in 2 x 2, each 128 x 128 (4 x 4 MFMA tiles, 256 accumulator registers).
- * Operand tiles of 64 k (one "unit") move from global memory straight into LDS by DMA
- * (global_load_lds_dwordx4), four stages deep (three in flight while one is computed), without
- * staging registers.
+ * Operand tiles of 64 k (one unit) move from global memory straight into LDS by DMA
+ * (global_load_lds_dwordx4), NST stages of 32 KB.
  * LDS row layout: 64 bytes per row (4 chunks of 16), chunk c of row r stored at position
  * c ^ ((r >> 2) & 3); the 16 lanes of a quarter-wave read rows r..r+15 of one chunk from 16 distinct
  * 4-bank groups. One DMA instruction fills 16 rows (1 KB): lane l writes position l % 4 of row l / 4,
  * so it loads chunk (l % 4) ^ ((row >> 2) & 3) of that row.
+ * Pipeline: at stage g a wave waits until its own DMA of stage g + 1 has landed and its fragment
+ * reads of stage g are complete, then meets the others at a barrier (now every wave's DMA of g + 1 has
+ * landed and every wave holds stage g in registers), refills the buffer of stage g with stage g + NST,
+ * reads the fragments of stage g + 1 and issues the 32 MFMAs of stage g, so the LDS reads of the next
+ * stage overlap the MFMAs of this one.
  */
 constexpr int QK = 64;                /* k per stage */
 constexpr int QTILE = LT * QK;        /* bytes of one operand tile per stage (16 KB) */
+constexpr int OZ_NST = 4;             /* LDS stages */
 
 /* operand tile of zeros: the source of the padding stages (their MFMAs add nothing) */
 __device__ __attribute__((aligned(1024))) int8_t oz_zero_tile[QTILE];
@@ -198,119 +210,200 @@ __device__ __forceinline__ void oz_dma16(const int8_t* src, int8_t* lds_dst) {
     (__attribute__((address_space(3))) void*)lds_dst, 16, 0, 0);
 }
 
-__global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb, const int* __restrict__ occa,
-  const int* __restrict__ occb, oz_items_t items, int s, int Mp, int Np, int Kp, int* __restrict__ c32) {
+/* s_waitcnt immediate (gfx9 encoding): vmcnt(vm), lgkmcnt(0), expcnt not waited for */
+constexpr int oz_wait_vm_lgkm0(int vm) { return (vm & 15) | ((vm >> 4) << 14) | 0x70; }
+
+template <int CRT>
+__global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb,
+  const int* __restrict__ occa, const int* __restrict__ occb, oz_items_t items, int s, int Mp, int Np, int Kp,
+  const int8_t* __restrict__ zero, void* __restrict__ out) {
   /* one array per stage ([A|B][256 rows x 64 bytes]): distinct objects, so the compiler can tell the DMA
      targets of the stages in flight from the stage being read */
   __shared__ __attribute__((aligned(1024))) int8_t ls0[2 * QTILE], ls1[2 * QTILE], ls2[2 * QTILE], ls3[2 * QTILE];
+  static_assert(4 == OZ_NST, "the stage loop below is written for four buffers");
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wj = wave >> 1, wi = wave & 1;
   const int i0 = blockIdx.x * LT, j0 = blockIdx.y * LT;
   const oz_item_t it = items.it[blockIdx.z];
   const int L = it.level, nkb = Kp / QK;
-  const int plo = (L - s + 1 > 0 ? L - s + 1 : 0);
-  unsigned int oa = 0, ob = 0;
-  for (int p = 0; p < s; ++p) {
-    oa |= (0 != occa[p] ? 1u : 0u) << p;
-    ob |= (0 != occb[p] ? 1u : 0u) << p;
+  const int plo = (0 != CRT ? L : (L - s + 1 > 0 ? L - s + 1 : 0));
+  const int pend = (0 != CRT ? L + 1 : (L < s - 1 ? L : s - 1) + 1);
+  unsigned int oa = ~0u, ob = ~0u;
+  if (0 == CRT) {
+    oa = ob = 0;
+    for (int p = 0; p < s; ++p) {
+      oa |= (0 != occa[p] ? 1u : 0u) << p;
+      ob |= (0 != occb[p] ? 1u : 0u) << p;
+    }
+    oa = __builtin_amdgcn_readfirstlane(oa);
+    ob = __builtin_amdgcn_readfirstlane(ob);
   }
-  oa = __builtin_amdgcn_readfirstlane(oa);
-  ob = __builtin_amdgcn_readfirstlane(ob);
+  /* pair p of the item has work: both planes hold a nonzero digit (always, for residues) */
+  auto active = [&](int p) -> bool { return 0 != CRT || 0 != ((oa >> p) & (ob >> (L - p)) & 1u); };
   i16_t acc[4][4];
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
 #pragma unroll
     for (int y = 0; y < 4; ++y) acc[x][y] = i16_t{0};
   }
-  /* active units (pairs with both planes occupied) of this item, counted up front */
-  int nact = 0;
+  int nact = 0; /* active units of this item */
   for (int pp = it.u0 / nkb; pp * nkb < it.u1; ++pp) {
-    const int p = plo + pp, lo = (it.u0 > pp * nkb ? it.u0 : pp * nkb), hi = (it.u1 < (pp + 1) * nkb ? it.u1 : (pp + 1) * nkb);
-    if ((oa >> p) & (ob >> (L - p)) & 1u) nact += hi - lo;
+    const int lo = (it.u0 > pp * nkb ? it.u0 : pp * nkb), hi = (it.u1 < (pp + 1) * nkb ? it.u1 : (pp + 1) * nkb);
+    if (active(plo + pp)) nact += hi - lo;
   }
   if (0 < nact) {
-    int fu = it.u0; /* next unit to load */
-    auto skip = [&]() {
-      while (fu < it.u1 && !((oa >> (plo + fu / nkb)) & (ob >> (L - plo - fu / nkb)) & 1u)) fu = (fu / nkb + 1) * nkb;
+    /* next unit to load: plane fp, k-block fkb (wave-uniform), walked in order over the active pairs;
+       ta and tb point at its operand tiles */
+    int fp = plo + it.u0 / nkb, fkb = it.u0 - (it.u0 / nkb) * nkb;
+    const int8_t *ta = pa, *tb = pb;
+    auto locate = [&]() {
+      while (fp < pend && !active(fp)) {
+        ++fp;
+        fkb = 0;
+      }
+      ta = pa + ((long)(fp < pend ? fp : plo) * Mp + i0) * Kp + fkb * QK;
+      tb = pb + ((long)(0 != CRT ? (fp < pend ? fp : plo) : L - (fp < pend ? fp : plo)) * Np + j0) * Kp + fkb * QK;
     };
-    skip();
-    const int drow = lane >> 2, dpos = lane & 3; /* this lane's row of a DMA instruction's 16, and position */
-    /* stage g into stage array buf: A tile rows 16 q .. 16 q + 15 for q = wave + 4 r, B alike; the next
-       active unit while g < nact, the zero tile after that (the stage count is padded to a multiple of 4) */
+    locate();
+    /* this lane's DMA rows 16 (wave + 4 r) + lane / 4, chunk (lane % 4) ^ ((row >> 2) & 3) (the same for all r) */
+    const int drow = 16 * wave + (lane >> 2), dchunk = (lane & 3) ^ ((lane >> 4) & 3);
+    unsigned int offr[4], offz[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      offr[r] = (unsigned int)(drow + 64 * r) * (unsigned int)Kp + 16u * dchunk;
+      offz[r] = (unsigned int)(drow + 64 * r) * QK + 16u * dchunk;
+    }
     int g = 0;
+    /* stage g into buffer buf: the next active unit while g < nact, the zero tile after that (no branch) */
     auto dma = [&](int8_t* buf) {
-      const int p = plo + fu / nkb, kb = fu - (fu / nkb) * nkb;
       const bool real = (g < nact);
-      const int8_t* const ga = (real ? pa + ((long)p * Mp + i0) * Kp + kb * QK : oz_zero_tile);
-      const int8_t* const gb = (real ? pb + ((long)(L - p) * Np + j0) * Kp + kb * QK : oz_zero_tile);
-      const long stride = (real ? Kp : QK);
+      const int8_t* const ga = (real ? ta : zero);
+      const int8_t* const gb = (real ? tb : zero);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int q = wave + 4 * r, row = 16 * q + drow, chunk = dpos ^ ((row >> 2) & 3);
-        oz_dma16(ga + (long)row * stride + 16 * chunk, buf + q * 1024);
-        oz_dma16(gb + (long)row * stride + 16 * chunk, buf + QTILE + q * 1024);
+        const unsigned int off = (real ? offr[r] : offz[r]);
+        oz_dma16(ga + off, buf + (wave + 4 * r) * 1024);
+        oz_dma16(gb + off, buf + QTILE + (wave + 4 * r) * 1024);
       }
-      if (real) {
-        ++fu;
-        skip();
+    };
+    /* the walk moves past the unit just loaded */
+    auto advance = [&]() {
+      if (g < nact) {
+        if (++fkb < nkb) {
+          ta += QK;
+          tb += QK;
+        }
+        else {
+          fkb = 0;
+          ++fp;
+          locate();
+        }
       }
       ++g;
     };
     const int frow = lane & 31, fswz = (frow >> 2) & 3; /* (row >> 2) & 3 is the same for rows 32 x + frow */
     const int fpos0 = (lane >> 5) ^ fswz, fpos1 = (2 + (lane >> 5)) ^ fswz;
-    /* one stage: wait for this wave's loads of the stage (two later stages may fly), barrier (every wave's
-       loads of the stage have landed, every wave has consumed the previous stage), refill the array
-       computed last time, then all 16 fragments and 32 MFMAs */
-#define OZ_STAGE(CUR, NXT) \
+    const int la_off = (128 * wi + frow) * QK, lb_off = QTILE + (128 * wj + frow) * QK;
+    i4_t fa[2][8], fb[2][8]; /* fragments of two stages: [k half 0 | k half 1] x 4 tiles */
+    auto frags = [&](const int8_t* buf, i4_t* va, i4_t* vb) {
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        vb[x] = *(const i4_t*)(buf + lb_off + 32 * x * QK + 16 * fpos0);
+        va[x] = *(const i4_t*)(buf + la_off + 32 * x * QK + 16 * fpos0);
+        vb[4 + x] = *(const i4_t*)(buf + lb_off + 32 * x * QK + 16 * fpos1);
+        va[4 + x] = *(const i4_t*)(buf + la_off + 32 * x * QK + 16 * fpos1);
+      }
+    };
+    auto mfmas = [&](const i4_t* va, const i4_t* vb) {
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+#pragma unroll
+          for (int y = 0; y < 4; ++y) {
+            acc[x][y] = __builtin_amdgcn_mfma_i32_32x32x32_i8(vb[4 * h + x], va[4 * h + y], acc[x][y], 0, 0, 0);
+          }
+        }
+      }
+    };
+    /* wait for own DMA of the next stage and own reads of this one, barrier */
+#define OZ_SYNC() \
   do { \
-    __builtin_amdgcn_s_waitcnt(0x4F70); /* vmcnt(16): this wave's DMA of the stage has landed */ \
+    __builtin_amdgcn_s_waitcnt(oz_wait_vm_lgkm0(8 * (OZ_NST - 2))); \
     __asm__ __volatile__("" ::: "memory"); \
     __builtin_amdgcn_s_barrier(); /* no fence: it would wait for the stages still in flight too */ \
     __asm__ __volatile__("" ::: "memory"); \
-    dma(NXT); \
-    const int8_t* const la_ = CUR + (128 * wi + frow) * QK; \
-    const int8_t* const lb_ = CUR + QTILE + (128 * wj + frow) * QK; \
-    i4_t fa0_[4], fb0_[4], fa1_[4], fb1_[4]; \
-    _Pragma("unroll") for (int x_ = 0; x_ < 4; ++x_) { \
-      fb0_[x_] = *(const i4_t*)(lb_ + 32 * x_ * QK + 16 * fpos0); \
-      fa0_[x_] = *(const i4_t*)(la_ + 32 * x_ * QK + 16 * fpos0); \
-      fb1_[x_] = *(const i4_t*)(lb_ + 32 * x_ * QK + 16 * fpos1); \
-      fa1_[x_] = *(const i4_t*)(la_ + 32 * x_ * QK + 16 * fpos1); \
+  } while (0)
+    /* stage g: CUR holds it (its fragments are in FC), NXT holds stage g + 1. Issue order: the MFMAs of
+       stage g (their operands are in registers) interleaved with the DMA of stage g + 4 and then with the
+       fragment reads of stage g + 1 */
+#define OZ_STAGE(CUR, NXT, FC, FN) \
+  do { \
+    OZ_SYNC(); \
+    dma(CUR); \
+    frags(NXT, fa[FN], fb[FN]); \
+    mfmas(fa[FC], fb[FC]); \
+    _Pragma("unroll") for (int i_ = 0; i_ < 8; ++i_) { \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* MFMA */ \
+      __builtin_amdgcn_sched_group_barrier(0x020, 1, 0); /* VMEM read (DMA) */ \
     } \
-    _Pragma("unroll") for (int x_ = 0; x_ < 4; ++x_) { \
-      _Pragma("unroll") for (int y_ = 0; y_ < 4; ++y_) { \
-        acc[x_][y_] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb0_[x_], fa0_[y_], acc[x_][y_], 0, 0, 0); \
-      } \
+    _Pragma("unroll") for (int i_ = 0; i_ < 16; ++i_) { \
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0); /* MFMA */ \
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0); /* DS read */ \
     } \
-    _Pragma("unroll") for (int x_ = 0; x_ < 4; ++x_) { \
-      _Pragma("unroll") for (int y_ = 0; y_ < 4; ++y_) { \
-        acc[x_][y_] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fb1_[x_], fa1_[y_], acc[x_][y_], 0, 0, 0); \
-      } \
-    } \
+    __builtin_amdgcn_sched_group_barrier(0x008, 8, 0); \
+    advance(); \
   } while (0)
     dma(ls0);
+    advance();
     dma(ls1);
+    advance();
     dma(ls2);
+    advance();
+    OZ_SYNC();
+    dma(ls3);
+    advance();
+    frags(ls0, fa[0], fb[0]);
     for (int t = 0; t < nact; t += 4) { /* one exit: the accumulators stay in their registers */
-      OZ_STAGE(ls0, ls3);
-      OZ_STAGE(ls1, ls0);
-      OZ_STAGE(ls2, ls1);
-      OZ_STAGE(ls3, ls2);
+      OZ_STAGE(ls0, ls1, 0, 1);
+      OZ_STAGE(ls1, ls2, 1, 0);
+      OZ_STAGE(ls2, ls3, 0, 1);
+      OZ_STAGE(ls3, ls0, 1, 0);
     }
 #undef OZ_STAGE
+#undef OZ_SYNC
     __builtin_amdgcn_s_waitcnt(0); /* the trailing zero-tile loads land before the work-group ends */
   }
-  int* const out = c32 + (long)blockIdx.z * Mp * Np;
+  if (0 == CRT) {
+    int* const o32 = (int*)out + (long)blockIdx.z * Mp * Np;
 #pragma unroll
-  for (int x = 0; x < 4; ++x) {
+    for (int x = 0; x < 4; ++x) {
 #pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int i = i0 + 128 * wi + 32 * y + (lane & 31);
+      for (int y = 0; y < 4; ++y) {
+        const int i = i0 + 128 * wi + 32 * y + (lane & 31);
 #pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        const int j = j0 + 128 * wj + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        out[(long)j * Mp + i] = acc[x][y][r];
+        for (int r = 0; r < 16; ++r) {
+          const int j = j0 + 128 * wj + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          o32[(long)j * Mp + i] = acc[x][y][r];
+        }
+      }
+    }
+  }
+  else {
+    const int m = oz_moduli[L];
+    uint8_t* const o8 = (uint8_t*)out + (long)blockIdx.z * Mp * Np;
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int i = i0 + 128 * wi + 32 * y + (lane & 31);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int j = j0 + 128 * wj + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+          const int v = acc[x][y][r] % m;
+          o8[(long)j * Mp + i] = (uint8_t)(v < 0 ? v + m : v);
+        }
       }
     }
   }
@@ -343,6 +436,160 @@ __global__ void __launch_bounds__(256) oz_recombine(const int* __restrict__ c32,
       const double v = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(cd, ex + ey) : 0.0;
       T* const pc = c + i + (long)j * ldc;
       *pc = (T)(0 != beta ? fma(alpha, v, beta * (double)*pc) : alpha * v);
+    }
+  }
+}
+
+/* ---- Scheme 2 (CRT) -------------------------------------------------------------------------- */
+
+/*
+ * Per modulus m (at most 256): the Barrett factor ceil(2^32 / m), exact for x < 2^18
+ * (floor(x / m) = mulhi(x, rcp): the excess x (rcp - 2^32 / m) / 2^32 < 2^-14 stays below 1 / m),
+ * 2^18 mod m and 2^36 mod m for the residues of 53-bit integers, and Garner's inverses
+ * inv[j][i] = m_j^-1 mod m_i (j < i).
+ */
+struct oz_crt_tab_t {
+  unsigned int m[LXS_OZ_NPRIMES_MAX], rcp[LXS_OZ_NPRIMES_MAX], k18[LXS_OZ_NPRIMES_MAX], k36[LXS_OZ_NPRIMES_MAX];
+  unsigned int inv[LXS_OZ_NPRIMES_MAX][LXS_OZ_NPRIMES_MAX];
+};
+constexpr int oz_inv_mod(int a, int m) { /* a^-1 mod m (gcd(a, m) = 1), extended Euclid */
+  int t = 0, nt = 1, r = m, nr = a % m;
+  while (0 != nr) {
+    const int q = r / nr, t2 = t - q * nt, r2 = r - q * nr;
+    t = nt, nt = t2, r = nr, nr = r2;
+  }
+  return (0 > t ? t + m : t);
+}
+constexpr oz_crt_tab_t oz_crt_make() {
+  constexpr int mod[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
+  oz_crt_tab_t t{};
+  for (int i = 0; i < LXS_OZ_NPRIMES_MAX; ++i) {
+    const unsigned long long m = (unsigned long long)mod[i];
+    t.m[i] = (unsigned int)m;
+    t.rcp[i] = (unsigned int)(((1ull << 32) + m - 1) / m);
+    t.k18[i] = (unsigned int)((1ull << 18) % m);
+    t.k36[i] = (unsigned int)((1ull << 36) % m);
+    for (int j = 0; j < i; ++j) t.inv[j][i] = (unsigned int)oz_inv_mod(mod[j] % mod[i], mod[i]);
+  }
+  return t;
+}
+__constant__ oz_crt_tab_t oz_crt = oz_crt_make();
+
+__device__ __forceinline__ unsigned int oz_mod18(unsigned int x, int p) { /* x mod m_p for x < 2^18 */
+  return x - __umulhi(x, oz_crt.rcp[p]) * oz_crt.m[p];
+}
+
+/*
+ * Residue planes of one 64 x 64 tile: every element v of row r with the row's exponent E (|v| < 2^E)
+ * becomes the integer a = trunc(|v| 2^(mb - t - E)) (mb = 53 for FP64, 24 for FP32; t trimmed bits);
+ * plane p holds the symmetric residue of sign(v) a mod m_p, in [-m/2, m/2), as int8.
+ * A thread owns 16 consecutive k of one row and writes one 16-byte vector per plane.
+ */
+template <typename T>
+__global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long rs, long ks, int R, int K,
+  const int* __restrict__ e, int np, int mb, int trim, int Rp, int Kp, int8_t* __restrict__ planes) {
+  __shared__ double t[OZ_TILE][OZ_TILE + 1];
+  const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
+  const int tid = threadIdx.x, r = tid >> 2, kq = (tid & 3) * 16;
+  oz_load_tile<T>(t, p, rs, ks, r0, k0, R, K);
+  __syncthreads();
+  const int row = r0 + r;
+  const int ex = (row < R ? e[row] : OZ_EXP_NONE);
+  unsigned int c0[16], c1[16], c2[16]; /* a = c2 2^36 + c1 2^18 + c0 */
+  unsigned int neg = 0;
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    const double v = t[r][kq + i];
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+    const int ev = oz_expo(v); /* |v| < 2^ev */
+    const unsigned long long mant = (bits & 0xFFFFFFFFFFFFFull) | (1ull << 52);
+    const int shift = (53 - mb) + trim + (ex - ev); /* a = mant >> shift */
+    const unsigned long long a = (OZ_EXP_NONE != ev && OZ_EXP_NONE != ex && shift < 53) ? (mant >> shift) : 0ull;
+    c0[i] = (unsigned int)(a & 0x3FFFF);
+    c1[i] = (unsigned int)((a >> 18) & 0x3FFFF);
+    c2[i] = (unsigned int)(a >> 36);
+    neg |= (unsigned int)(bits >> 63) << i;
+  }
+#pragma unroll
+  for (int pl = 0; pl < LXS_OZ_NPRIMES_MAX; ++pl) {
+    if (pl < np) {
+      const int m = (int)oz_crt.m[pl];
+      union {
+        int8_t b[16];
+        i4_t v;
+      } d;
+#pragma unroll
+      for (int i = 0; i < 16; ++i) {
+        const unsigned int x = oz_mod18(c2[i], pl) * oz_crt.k36[pl] + oz_mod18(c1[i], pl) * oz_crt.k18[pl] + oz_mod18(c0[i], pl);
+        int s = (int)oz_mod18(x, pl); /* x < 3 * 2^16 */
+        if (0 != ((neg >> i) & 1)) s = (0 != s ? m - s : 0);
+        d.b[i] = (int8_t)(s >= (m + 1) / 2 ? s - m : s);
+      }
+      *(i4_t*)(planes + ((long)pl * Rp + row) * Kp + k0 + kq) = d.v;
+    }
+  }
+}
+
+/*
+ * C(i, j) = fma(alpha, 2^(ea_i + eb_j + sh0) x, beta C(i, j)) where x is the integer with residues
+ * x = sum over the prime's items of their residue bytes (mod m_p), |x| < M / 2: Garner's mixed-radix
+ * digits v_p, x = sum_p v_p prod_{q<p} m_q as a 128-bit integer (minus M when at least M / 2),
+ * rounded to FP64 once (round to nearest even).
+ */
+template <typename T>
+__global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restrict__ r8, oz_levels_t lv, int np,
+  unsigned long long hmhi, unsigned long long hmlo, unsigned long long mhi, unsigned long long mlo, int Mp, int Np,
+  const int* __restrict__ ea, const int* __restrict__ eb, int M, int N, int sh0, double alpha, double beta,
+  T* __restrict__ c, int ldc) {
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < (long)M * N; x += (long)gridDim.x * 256) {
+    const int i = (int)(x % M), j = (int)(x / M);
+    const long o = (long)j * Mp + i;
+    unsigned int v[LXS_OZ_NPRIMES_MAX];
+#pragma unroll
+    for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
+      if (p < np) {
+        unsigned int sum = 0; /* at most 1024 items of bytes below 256 */
+        for (int z = lv.zb[p]; z < lv.zb[p + 1]; ++z) sum += r8[(long)z * Mp * Np + o];
+        unsigned int u = sum % oz_crt.m[p];
+#pragma unroll
+        for (int q = 0; q < p; ++q) { /* u = (u - v_q) m_q^-1 mod m_p; 2 m_p > 256 > v_q */
+          u = oz_mod18((u + 2 * oz_crt.m[p] - v[q]) * oz_crt.inv[q][p], p);
+        }
+        v[p] = u;
+      }
+    }
+    unsigned long long hi = 0, lo = 0; /* x = sum_p v_p prod_{q<p} m_q, Horner from the top digit */
+#pragma unroll
+    for (int p = LXS_OZ_NPRIMES_MAX - 1; 0 <= p; --p) {
+      if (p < np) {
+        const unsigned long long m = oz_crt.m[p], l = lo * m;
+        hi = hi * m + __umul64hi(lo, m);
+        lo = l + v[p];
+        hi += (lo < l ? 1 : 0);
+      }
+    }
+    const bool neg = (hi > hmhi || (hi == hmhi && lo >= hmlo));
+    if (neg) { /* |x| = M - x */
+      const unsigned long long l = mlo - lo;
+      hi = mhi - hi - (mlo < lo ? 1 : 0);
+      lo = l;
+    }
+    double d; /* |x| rounded to FP64: the top 64 bits with a sticky bit below, converted with one rounding */
+    {
+      unsigned long long top = lo;
+      int scale = 0;
+      if (0 != hi) {
+        const int lz = __clzll((long long)hi);
+        top = (0 != lz ? (hi << lz) | (lo >> (64 - lz)) : hi) | ((0 != (lo << lz)) ? 1ull : 0ull);
+        scale = 64 - lz;
+      }
+      d = ldexp(fma((double)(unsigned int)(top >> 32), 4294967296.0, (double)(unsigned int)top), scale);
+    }
+    {
+      const int ex = ea[i], ey = eb[j];
+      const double val = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(neg ? -d : d, ex + ey + sh0) : 0.0;
+      T* const pc = c + i + (long)j * ldc;
+      *pc = (T)(0 != beta ? fma(alpha, val, beta * (double)*pc) : alpha * val);
     }
   }
 }
@@ -488,20 +735,42 @@ extern "C" int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int
   return (int)hipGetLastError();
 }
 
-/* items: host array of nitems x {level, u0, u1, pad}; partial z of item z at c32 + z * Mp * Np;
-   planes row-padded to a multiple of 256 and k-padded to a multiple of 128 */
-extern "C" int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, const int* occa, const int* occb,
-  const int* items, int nitems, int s, int Mp, int Np, int Kp, int* c32, void* hip_stream) {
-  if (0 != Mp % LT || 0 != Np % LT || 0 != Kp % LK || 0 >= Kp || 1 > s || 8 < s || 0 >= nitems) return LXS_ELIMIT;
-  for (int z = 0; z < nitems; ++z) { /* units of 64 k within the level's pairs, at most 2080 per item */
-    const int L = items[4 * z], plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
-    if (0 > L || 2 * (s - 1) < L || 0 > items[4 * z + 1] || items[4 * z + 1] >= items[4 * z + 2] ||
-        2080 < items[4 * z + 2] - items[4 * z + 1] || (phi - plo + 1) * (Kp / QK) < items[4 * z + 2]) {
+/* device address of the zero tile on the current device (one lookup per device) */
+static const int8_t* oz_zero_ptr() {
+  static const int8_t* ptr[64];
+  int dev = 0;
+  if (hipSuccess != hipGetDevice(&dev) || 0 > dev || 64 <= dev) return NULL;
+  if (NULL == ptr[dev]) {
+    void* p = NULL;
+    if (hipSuccess != hipGetSymbolAddress(&p, HIP_SYMBOL(oz_zero_tile))) return NULL;
+    ptr[dev] = (const int8_t*)p;
+  }
+  return ptr[dev];
+}
+
+/* items: host array of nitems x {level, u0, u1, pad}; item z writes at out + z * Mp * Np (int32 partials,
+   or residue bytes with crt); planes row-padded to a multiple of 256 and k-padded to a multiple of 128 */
+extern "C" int lxs_oz_level_gemm(int crt, const signed char* pa, const signed char* pb, const int* occa,
+  const int* occb, const int* items, int nitems, int s, int Mp, int Np, int Kp, void* out, void* hip_stream) {
+  const int8_t* const zero = oz_zero_ptr();
+  const int nkb = Kp / QK;
+  if (0 != Mp % LT || 0 != Np % LT || 0 != Kp % LK || 0 >= Kp || (1 << 22) < Kp || 0 >= nitems || NULL == zero) {
+    return LXS_ELIMIT;
+  }
+  if (0 == crt ? (1 > s || 8 < s || NULL == occa || NULL == occb) : (1 > s || LXS_OZ_NPRIMES_MAX < s)) return LXS_ELIMIT;
+  for (int z = 0; z < nitems; ++z) {
+    const int L = items[4 * z], u0 = items[4 * z + 1], u1 = items[4 * z + 2];
+    if (0 == crt) { /* units of 64 k within the level's pairs, at most 2080 per item */
+      const int plo = (L - s + 1 > 0 ? L - s + 1 : 0), phi = (L < s - 1 ? L : s - 1);
+      if (0 > L || 2 * (s - 1) < L || 0 > u0 || u0 >= u1 || 2080 < u1 - u0 || (phi - plo + 1) * nkb < u1) return LXS_ELIMIT;
+    }
+    else if (0 > L || s <= L || 0 > u0 || u0 >= u1 || 2047 < u1 - u0 || nkb < u1) { /* k-blocks of prime L */
       return LXS_ELIMIT;
     }
   }
   for (int z0 = 0; z0 < nitems; z0 += OZ_ITEMS) {
     const int nz = (nitems - z0 < OZ_ITEMS ? nitems - z0 : OZ_ITEMS);
+    const dim3 grid((unsigned int)(Mp / LT), (unsigned int)(Np / LT), (unsigned int)nz);
     oz_items_t arg;
     for (int z = 0; z < OZ_ITEMS; ++z) {
       const int* const src = items + 4 * (z0 + (z < nz ? z : 0));
@@ -510,8 +779,60 @@ extern "C" int lxs_oz_level_gemm(const signed char* pa, const signed char* pb, c
       arg.it[z].u1 = src[2];
       arg.it[z].pad = 0;
     }
-    oz_level_gemm<<<dim3((unsigned int)(Mp / LT), (unsigned int)(Np / LT), (unsigned int)nz), 256, 0,
-      (hipStream_t)hip_stream>>>(pa, pb, occa, occb, arg, s, Mp, Np, Kp, c32 + (long)z0 * Mp * Np);
+    if (0 == crt) {
+      oz_level_gemm<0><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const int8_t*)pa, (const int8_t*)pb, occa, occb, arg, s,
+        Mp, Np, Kp, zero, (int*)out + (long)z0 * Mp * Np);
+    }
+    else {
+      oz_level_gemm<1><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const int8_t*)pa, (const int8_t*)pb, occa, occb, arg, s,
+        Mp, Np, Kp, zero, (uint8_t*)out + (long)z0 * Mp * Np);
+    }
+  }
+  return (int)hipGetLastError();
+}
+
+extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int trim, int Rp,
+  int Kp, int* e, signed char* planes, void* hip_stream) {
+  const hipStream_t st = (hipStream_t)hip_stream;
+  const dim3 grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
+  const int mb = (3 == dtype ? 53 : 24);
+  if (0 != Rp % LT || 0 != Kp % LK || Rp < R || Kp < K || 0 >= Kp || 1 > np || LXS_OZ_NPRIMES_MAX < np) return LXS_ELIMIT;
+  if (!((1 == rs) || (1 == ks)) || 0 > trim || mb <= trim) return LXS_ELIMIT;
+  oz_init<<<(Rp + 255) / 256, 256, 0, st>>>(e, Rp, NULL);
+  if (3 == dtype) {
+    oz_exp<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
+    oz_residues<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e, np, mb, trim, Rp, Kp, (int8_t*)planes);
+  }
+  else if (1 == dtype) {
+    oz_exp<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
+    oz_residues<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e, np, mb, trim, Rp, Kp, (int8_t*)planes);
+  }
+  else {
+    return LXS_ENOVARIANT;
+  }
+  return (int)hipGetLastError();
+}
+
+/* zb: first item per prime, np + 1 entries; hm and m: M / 2 and M = prod of the first np moduli (128 bits) */
+extern "C" int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, const unsigned long long* hm,
+  const unsigned long long* m, int Mp, int Np, const int* ea, const int* eb, int M, int N, int sh0, double alpha,
+  double beta, void* c, int ldc, void* hip_stream) {
+  oz_levels_t lv;
+  const long blocks = ((long)M * N + 255) / 256;
+  const unsigned int grid = (unsigned int)(blocks < 4096 ? blocks : 4096);
+  if (1 > np || LXS_OZ_NPRIMES_MAX < np || Mp < M || Np < N) return LXS_ELIMIT;
+  if (0 >= M || 0 >= N) return 0;
+  for (int p = 0; p < 17; ++p) lv.zb[p] = (p <= np ? zb[p] : zb[np]);
+  if (3 == dtype) {
+    oz_crt_recombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, hm[0], hm[1], m[0], m[1],
+      Mp, Np, ea, eb, M, N, sh0, alpha, beta, (double*)c, ldc);
+  }
+  else if (1 == dtype) {
+    oz_crt_recombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, hm[0], hm[1], m[0], m[1],
+      Mp, Np, ea, eb, M, N, sh0, alpha, beta, (float*)c, ldc);
+  }
+  else {
+    return LXS_ENOVARIANT;
   }
   return (int)hipGetLastError();
 }

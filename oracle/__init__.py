@@ -52,6 +52,8 @@ def lib():
         L.oracle_oz_digits_ref.restype = i
         L.oracle_ozaki_gemm.argtypes = [i, ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, i, i, i, i]
         L.oracle_ozaki_gemm.restype = i
+        L.oracle_ozaki2_gemm.argtypes = [i, ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, i, i, i]
+        L.oracle_ozaki2_gemm.restype = i
         L.oracle_mlp_loss_grad.argtypes = [i, vp, vp, vp, vp, vp, vp, i, i, vp, vp, i]
         L.oracle_mlp_loss_grad.restype = i
         _lib = L
@@ -186,6 +188,22 @@ def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, 
     use_double = 1 if a.dtype == np.float64 else 0
     rc = lib().oracle_ozaki_gemm(use_double, transa.encode(), transb.encode(), m, n, k, alpha, a.ctypes.data, lda,
                                  b.ctypes.data, ldb, beta, out.ctypes.data, ldc, s, trim, maxk, plant)
+    assert rc == 0
+    return out
+
+
+OZ2_PLANT_NONE, OZ2_PLANT_ROUND, OZ2_PLANT_SIGNB, OZ2_PLANT_SCALE, OZ2_PLANT_FPSUM = 0, 1, 2, 3, 4
+
+
+def ozaki2_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, trim=0, kpass=0, plant=OZ2_PLANT_NONE):
+    """Ozaki scheme II (CRT): the exact integer product of the aligned mantissas, rounded once, per K pass
+    of kpass (0: all of K); returns the updated copy of c (column-major flat, float64 or float32 like a)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    out = np.array(c, dtype=a.dtype, copy=True, order="C").reshape(-1)
+    use_double = 1 if a.dtype == np.float64 else 0
+    rc = lib().oracle_ozaki2_gemm(use_double, transa.encode(), transb.encode(), m, n, k, alpha, a.ctypes.data, lda,
+                                  b.ctypes.data, ldb, beta, out.ctypes.data, ldc, trim, kpass, plant)
     assert rc == 0
     return out
 

@@ -20,8 +20,10 @@
  *                         C = fma(alpha, ldexp(cd, ea + eb), beta C) (alpha * v if beta = 0).
  *                         Its results are therefore bit-identical to the HIP path's.
  * Zero, subnormal and non-finite inputs count as zero (ozaki_common.cl:784-805).
+ *   oracle_ozaki2_gemm    Ozaki scheme II (CRT): the exact integer product of the aligned mantissas
+ *                         (no residues), rounded to FP64 once and scaled (below).
  *
- * Pinning (tests/test_oracle_ozaki.py): exact products of small-integer matrices, the rigorous
+ * Pinning (tests/test_oracle_ozaki.py, tests/test_oracle_ozaki2.py): exact products of small-integer matrices, the rigorous
  * truncation bound against the double-double GEMM, the reference digit formulation, and planted
  * mistakes (the `plant` argument) that the pins must reject.
  */
@@ -236,6 +238,140 @@ int oracle_ozaki_gemm(int use_double, char transa, char transb, int M, int N, in
       }
     }
     free(da), free(db), free(ea), free(eb);
+  }
+  return 0;
+}
+
+/*
+ * Ozaki scheme II (CRT), the integer product the reference recovers from its residues
+ * (samples/ozaki/kernels/ozaki2_int8.cl:1808-1860 preprocessing: per-row (per-column for B) maximum
+ * biased exponent over normal finite values, the mantissa with its implicit bit shifted right by the
+ * exponent difference plus the trimmed bits, sign kept; :1329-1440 Garner and Horner: the exact sum
+ * of products, scaled by 2^(ea + eb - 2 (bias + mantissa bits - trim))). The oracle forms the same
+ * integers and their exact sum (128-bit) directly, without residues, and rounds it to FP64 once; then
+ * C = fma(alpha, x 2^sh, beta C) (alpha x 2^sh if beta = 0), per K pass of kpass (the first pass
+ * applies beta, later ones accumulate). float C receives that double rounded to float.
+ * Planted mistakes (tests only): 1 rounds the aligned mantissa to nearest, 2 drops the sign of B,
+ * 3 scales by 2^(sh + 1), 4 sums the products in FP64.
+ */
+#define OZ2_PLANT_ROUND 1
+#define OZ2_PLANT_SIGNB 2
+#define OZ2_PLANT_SCALE 3
+#define OZ2_PLANT_FPSUM 4
+
+/* biased exponent and mantissa with its implicit bit (0, 0 for zero, subnormal and non-finite) */
+static void oz2_decompose(int use_double, const void* src, long idx, int* sign, int* be, uint64_t* mant) {
+  if (use_double) {
+    uint64_t bits;
+    memcpy(&bits, (const double*)src + idx, 8);
+    *sign = (int)(bits >> 63);
+    *be = (int)((bits >> 52) & 0x7FF);
+    *mant = (bits & 0x000FFFFFFFFFFFFFull) | 0x0010000000000000ull;
+    if (0 == *be || 0x7FF == *be) *be = 0, *mant = 0;
+  }
+  else {
+    uint32_t bits;
+    memcpy(&bits, (const float*)src + idx, 4);
+    *sign = (int)(bits >> 31);
+    *be = (int)((bits >> 23) & 0xFF);
+    *mant = (uint64_t)((bits & 0x007FFFFFu) | 0x00800000u);
+    if (0 == *be || 0xFF == *be) *be = 0, *mant = 0;
+  }
+}
+
+int oracle_ozaki2_gemm(int use_double, char transa, char transb, int M, int N, int K, double alpha, const void* a,
+  int lda, const void* b, int ldb, double beta, void* c, int ldc, int trim, int kpass, int plant) {
+  const int ta = ('N' == transa || 'n' == transa) ? 0 : 1;
+  const int tb = ('N' == transb || 'n' == transb) ? 0 : 1;
+  const int mant_bits = (use_double ? 52 : 23), bias_plus_mant = (use_double ? 1075 : 150);
+  int k0;
+  if (0 > M || 0 > N || 0 > K || 0 > trim || mant_bits < trim) return 1;
+  if (0 >= kpass || K < kpass) kpass = (0 < K ? K : 1);
+  if (0 == M || 0 == N) return 0;
+  if (0 == K || 0 == alpha) {
+    long x;
+    for (x = 0; x < (long)M * N; ++x) {
+      const long o = (x % M) + (x / M) * (long)ldc;
+      if (use_double) ((double*)c)[o] = (0 != beta ? beta * ((double*)c)[o] : 0.0);
+      else ((float*)c)[o] = (float)(0 != beta ? beta * (double)((float*)c)[o] : 0.0);
+    }
+    return 0;
+  }
+  for (k0 = 0; k0 < K; k0 += kpass) {
+    const int kk = (K - k0 < kpass ? K - k0 : kpass);
+    const double beta_eff = (0 == k0 ? beta : 1.0);
+    int64_t* const ia = (int64_t*)malloc(sizeof(int64_t) * (size_t)M * kk);
+    int64_t* const ib = (int64_t*)malloc(sizeof(int64_t) * (size_t)N * kk);
+    int* const ea = (int*)malloc(sizeof(int) * M);
+    int* const eb = (int*)malloc(sizeof(int) * N);
+    int i, j;
+    if (NULL == ia || NULL == ib || NULL == ea || NULL == eb) {
+      free(ia), free(ib), free(ea), free(eb);
+      return 2;
+    }
+    for (i = 0; i < M + N; ++i) { /* op(A) rows, then op(B) columns */
+      const int isa = (i < M), r = (isa ? i : i - M), tr = (isa ? ta : tb), ld = (isa ? lda : ldb);
+      const void* const src = (isa ? a : b);
+      int64_t* const dst = (isa ? ia : ib) + (long)r * kk;
+      int k, maxe = 0;
+      for (k = 0; k < kk; ++k) {
+        const long idx = isa ? (0 == tr ? r + (long)(k0 + k) * ld : (k0 + k) + (long)r * ld)
+                             : (0 == tr ? (k0 + k) + (long)r * ld : r + (long)(k0 + k) * ld);
+        int sign, be;
+        uint64_t mant;
+        oz2_decompose(use_double, src, idx, &sign, &be, &mant);
+        if (be > maxe) maxe = be;
+      }
+      (isa ? ea : eb)[r] = maxe;
+      for (k = 0; k < kk; ++k) {
+        const long idx = isa ? (0 == tr ? r + (long)(k0 + k) * ld : (k0 + k) + (long)r * ld)
+                             : (0 == tr ? (k0 + k) + (long)r * ld : r + (long)(k0 + k) * ld);
+        int sign, be;
+        uint64_t mant, aligned = 0;
+        oz2_decompose(use_double, src, idx, &sign, &be, &mant);
+        if (0 != mant) {
+          const int shift = (maxe - be) + trim;
+          if (shift <= mant_bits) {
+            aligned = mant >> shift;
+            if (OZ2_PLANT_ROUND == plant && 0 < shift && 0 != ((mant >> (shift - 1)) & 1)) ++aligned;
+          }
+        }
+        if (OZ2_PLANT_SIGNB == plant && !isa) sign = 0;
+        dst[k] = (sign ? -(int64_t)aligned : (int64_t)aligned);
+      }
+    }
+#if defined(_OPENMP)
+#  pragma omp parallel for schedule(dynamic, 1) private(i)
+#endif
+    for (j = 0; j < N; ++j) {
+      for (i = 0; i < M; ++i) {
+        const int64_t* const ra = ia + (long)i * kk;
+        const int64_t* const rb = ib + (long)j * kk;
+        const int sh = ea[i] + eb[j] - 2 * (bias_plus_mant - trim) + (OZ2_PLANT_SCALE == plant ? 1 : 0);
+        const long o = i + (long)j * ldc;
+        double v, val;
+        int k;
+        if (OZ2_PLANT_FPSUM == plant) {
+          v = 0;
+          for (k = 0; k < kk; ++k) v += (double)ra[k] * (double)rb[k];
+        }
+        else {
+          __int128 x = 0;
+          for (k = 0; k < kk; ++k) x += (__int128)ra[k] * (__int128)rb[k];
+          v = (double)x; /* round to nearest even, once */
+        }
+        val = ldexp(v, sh);
+        if (use_double) {
+          double* const pc = (double*)c + o;
+          *pc = (0 != beta_eff ? fma(alpha, val, beta_eff * *pc) : alpha * val);
+        }
+        else {
+          float* const pc = (float*)c + o;
+          *pc = (float)(0 != beta_eff ? fma(alpha, val, beta_eff * (double)*pc) : alpha * val);
+        }
+      }
+    }
+    free(ia), free(ib), free(ea), free(eb);
   }
   return 0;
 }
