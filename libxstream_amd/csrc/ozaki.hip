@@ -29,6 +29,8 @@
  */
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string.h>
+#include <stdlib.h>
 #include "lxs_kernels.h"
 
 namespace {
@@ -38,6 +40,15 @@ typedef int i16_t __attribute__((ext_vector_type(16)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_TILE = 64; /* rows x k of a preprocessing tile */
+
+/*
+ * Plane layout (digits and residues alike): tiles of 256 rows x 64 k, each 16 KB contiguous (row r of a
+ * tile at r * 64), tile (plane p, row block rb, k block kb) at ((p Rp / 256 + rb) Kp / 64 + kb) 16 KB.
+ * One k-block of an operand tile of the int8 GEMM is then one contiguous 16 KB read.
+ */
+__device__ __forceinline__ long oz_plane_off(int p, int row, int k, int Rp, int Kp) {
+  return (((long)p * (Rp >> 8) + (row >> 8)) * (Kp >> 6) + (k >> 6)) * 16384 + (row & 255) * 64 + (k & 63);
+}
 constexpr int OZ_EXP_NONE = -(1 << 30);
 
 /* exponent ex with |v| < 2^ex for a normal finite v, OZ_EXP_NONE otherwise */
@@ -136,7 +147,7 @@ __global__ void __launch_bounds__(256) oz_slice(const T* __restrict__ p, long rs
       nz |= (int)(0.0 != q);
       x[i] = y - q; /* exact */
     }
-    *(i4_t*)(planes + ((long)pl * Rp + row) * Kp + k0 + kq) = d.v;
+    *(i4_t*)(planes + oz_plane_off(pl, row, k0 + kq, Rp, Kp)) = d.v;
     if (0 != __ballot(nz) && 0 == (tid & 63) && 0 == __hip_atomic_load(occ + pl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
       atomicOr(occ + pl, 1);
     }
@@ -200,7 +211,6 @@ __constant__ int oz_moduli[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
  */
 constexpr int QK = 64;                /* k per stage */
 constexpr int QTILE = LT * QK;        /* bytes of one operand tile per stage (16 KB) */
-constexpr int OZ_NST = 4;             /* LDS stages */
 
 /* operand tile of zeros: the source of the padding stages (their MFMAs add nothing) */
 __device__ __attribute__((aligned(1024))) int8_t oz_zero_tile[QTILE];
@@ -213,14 +223,15 @@ __device__ __forceinline__ void oz_dma16(const int8_t* src, int8_t* lds_dst) {
 /* s_waitcnt immediate (gfx9 encoding): vmcnt(vm), lgkmcnt(0), expcnt not waited for */
 constexpr int oz_wait_vm_lgkm0(int vm) { return (vm & 15) | ((vm >> 4) << 14) | 0x70; }
 
-template <int CRT>
+template <int CRT, int NST>
 __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ pa, const int8_t* __restrict__ pb,
   const int* __restrict__ occa, const int* __restrict__ occb, oz_items_t items, int s, int Mp, int Np, int Kp,
   const int8_t* __restrict__ zero, void* __restrict__ out) {
   /* one array per stage ([A|B][256 rows x 64 bytes]): distinct objects, so the compiler can tell the DMA
      targets of the stages in flight from the stage being read */
   __shared__ __attribute__((aligned(1024))) int8_t ls0[2 * QTILE], ls1[2 * QTILE], ls2[2 * QTILE], ls3[2 * QTILE];
-  static_assert(4 == OZ_NST, "the stage loop below is written for four buffers");
+  __shared__ __attribute__((aligned(1024))) int8_t ls4[4 == NST ? 1 : 2 * QTILE]; /* the fifth stage (NST = 5) */
+  static_assert(4 == NST || 5 == NST, "the stage loop below is written for four or five buffers");
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wj = wave >> 1, wi = wave & 1;
@@ -262,18 +273,14 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
         ++fp;
         fkb = 0;
       }
-      ta = pa + ((long)(fp < pend ? fp : plo) * Mp + i0) * Kp + fkb * QK;
-      tb = pb + ((long)(0 != CRT ? (fp < pend ? fp : plo) : L - (fp < pend ? fp : plo)) * Np + j0) * Kp + fkb * QK;
+      const int pla = (fp < pend ? fp : plo), plb = (0 != CRT ? pla : L - pla); /* tiles of 256 x 64, 16 KB each */
+      ta = pa + (((long)pla * (Mp / LT) + i0 / LT) * nkb + fkb) * (long)QTILE;
+      tb = pb + (((long)plb * (Np / LT) + j0 / LT) * nkb + fkb) * (long)QTILE;
     };
     locate();
     /* this lane's DMA rows 16 (wave + 4 r) + lane / 4, chunk (lane % 4) ^ ((row >> 2) & 3) (the same for all r) */
     const int drow = 16 * wave + (lane >> 2), dchunk = (lane & 3) ^ ((lane >> 4) & 3);
-    unsigned int offr[4], offz[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      offr[r] = (unsigned int)(drow + 64 * r) * (unsigned int)Kp + 16u * dchunk;
-      offz[r] = (unsigned int)(drow + 64 * r) * QK + 16u * dchunk;
-    }
+    const unsigned int off0 = (unsigned int)drow * QK + 16u * dchunk; /* rows 64 bytes apart in a tile */
     int g = 0;
     /* stage g into buffer buf: the next active unit while g < nact, the zero tile after that (no branch) */
     auto dma = [&](int8_t* buf) {
@@ -282,17 +289,16 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
       const int8_t* const gb = (real ? tb : zero);
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const unsigned int off = (real ? offr[r] : offz[r]);
-        oz_dma16(ga + off, buf + (wave + 4 * r) * 1024);
-        oz_dma16(gb + off, buf + QTILE + (wave + 4 * r) * 1024);
+        oz_dma16(ga + (off0 + r * 64 * QK), buf + (wave + 4 * r) * 1024);
+        oz_dma16(gb + (off0 + r * 64 * QK), buf + QTILE + (wave + 4 * r) * 1024);
       }
     };
     /* the walk moves past the unit just loaded */
     auto advance = [&]() {
       if (g < nact) {
         if (++fkb < nkb) {
-          ta += QK;
-          tb += QK;
+          ta += QTILE;
+          tb += QTILE;
         }
         else {
           fkb = 0;
@@ -330,7 +336,7 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
     /* wait for own DMA of the next stage and own reads of this one, barrier */
 #define OZ_SYNC() \
   do { \
-    __builtin_amdgcn_s_waitcnt(oz_wait_vm_lgkm0(8 * (OZ_NST - 2))); \
+    __builtin_amdgcn_s_waitcnt(oz_wait_vm_lgkm0(8 * (NST - 2))); \
     __asm__ __volatile__("" ::: "memory"); \
     __builtin_amdgcn_s_barrier(); /* no fence: it would wait for the stages still in flight too */ \
     __asm__ __volatile__("" ::: "memory"); \
@@ -361,15 +367,37 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
     advance();
     dma(ls2);
     advance();
-    OZ_SYNC();
-    dma(ls3);
-    advance();
-    frags(ls0, fa[0], fb[0]);
-    for (int t = 0; t < nact; t += 4) { /* one exit: the accumulators stay in their registers */
-      OZ_STAGE(ls0, ls1, 0, 1);
-      OZ_STAGE(ls1, ls2, 1, 0);
-      OZ_STAGE(ls2, ls3, 0, 1);
-      OZ_STAGE(ls3, ls0, 1, 0);
+    if constexpr (4 == NST) {
+      OZ_SYNC();
+      dma(ls3);
+      advance();
+      frags(ls0, fa[0], fb[0]);
+      for (int t = 0; t < nact; t += 4) { /* one exit: the accumulators stay in their registers */
+        OZ_STAGE(ls0, ls1, 0, 1);
+        OZ_STAGE(ls1, ls2, 1, 0);
+        OZ_STAGE(ls2, ls3, 0, 1);
+        OZ_STAGE(ls3, ls0, 1, 0);
+      }
+    }
+    else { /* five buffers and two fragment sets: ten stages per trip */
+      dma(ls3);
+      advance();
+      OZ_SYNC();
+      dma(ls4);
+      advance();
+      frags(ls0, fa[0], fb[0]);
+      for (int t = 0; t < nact; t += 10) {
+        OZ_STAGE(ls0, ls1, 0, 1);
+        OZ_STAGE(ls1, ls2, 1, 0);
+        OZ_STAGE(ls2, ls3, 0, 1);
+        OZ_STAGE(ls3, ls4, 1, 0);
+        OZ_STAGE(ls4, ls0, 0, 1);
+        OZ_STAGE(ls0, ls1, 1, 0);
+        OZ_STAGE(ls1, ls2, 0, 1);
+        OZ_STAGE(ls2, ls3, 1, 0);
+        OZ_STAGE(ls3, ls4, 0, 1);
+        OZ_STAGE(ls4, ls0, 1, 0);
+      }
     }
 #undef OZ_STAGE
 #undef OZ_SYNC
@@ -391,7 +419,9 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
     }
   }
   else {
+    /* v mod m for |v| < 2^31: floor(v / m) from FP64 (exact but at multiples of m, which leave r = m) */
     const int m = oz_moduli[L];
+    const double rm = 1.0 / m;
     uint8_t* const o8 = (uint8_t*)out + (long)blockIdx.z * Mp * Np;
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
@@ -401,8 +431,8 @@ __global__ void __launch_bounds__(256) oz_level_gemm(const int8_t* __restrict__ 
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int j = j0 + 128 * wj + 32 * x + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-          const int v = acc[x][y][r] % m;
-          o8[(long)j * Mp + i] = (uint8_t)(v < 0 ? v + m : v);
+          const int v = acc[x][y][r], q = (int)floor((double)v * rm), rr = v - q * m;
+          o8[(long)j * Mp + i] = (uint8_t)(rr >= m ? rr - m : rr);
         }
       }
     }
@@ -525,71 +555,126 @@ __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long
         if (0 != ((neg >> i) & 1)) s = (0 != s ? m - s : 0);
         d.b[i] = (int8_t)(s >= (m + 1) / 2 ? s - m : s);
       }
-      *(i4_t*)(planes + ((long)pl * Rp + row) * Kp + k0 + kq) = d.v;
+      *(i4_t*)(planes + oz_plane_off(pl, row, k0 + kq, Rp, Kp)) = d.v;
     }
   }
 }
 
+/* constants of the reconstruction for the first np moduli (M their product) */
+struct oz_crt_consts_t {
+  unsigned int w[LXS_OZ_NPRIMES_MAX];     /* ((M / m_p) mod m_p)^-1 mod m_p */
+  unsigned int lm[LXS_OZ_NPRIMES_MAX][4]; /* M / m_p in 32-bit limbs, least significant first */
+  double rm[LXS_OZ_NPRIMES_MAX];          /* 1 / m_p */
+  unsigned int mm[4], hm[4];              /* M and M / 2 in limbs */
+};
+
+/* 128-bit two's complement value (hi signed, lo unsigned) */
+struct oz_i128_t {
+  long long hi;
+  unsigned long long lo;
+};
+__device__ __forceinline__ oz_i128_t oz_add(oz_i128_t a, long long bhi, unsigned long long blo) {
+  oz_i128_t r;
+  r.lo = a.lo + blo;
+  r.hi = a.hi + bhi + (r.lo < blo ? 1 : 0);
+  return r;
+}
+__device__ __forceinline__ oz_i128_t oz_sub(oz_i128_t a, long long bhi, unsigned long long blo) {
+  oz_i128_t r;
+  r.lo = a.lo - blo;
+  r.hi = a.hi - bhi - (a.lo < blo ? 1 : 0);
+  return r;
+}
+
 /*
- * C(i, j) = fma(alpha, 2^(ea_i + eb_j + sh0) x, beta C(i, j)) where x is the integer with residues
- * x = sum over the prime's items of their residue bytes (mod m_p), |x| < M / 2: Garner's mixed-radix
- * digits v_p, x = sum_p v_p prod_{q<p} m_q as a 128-bit integer (minus M when at least M / 2),
- * rounded to FP64 once (round to nearest even).
+ * C(i, j) = fma(alpha, 2^(ea_i + eb_j + sh0) x, beta C(i, j)) where x is the integer of the residues
+ * r_p = sum of the prime's residue bytes mod m_p, |x| < M / 2. Direct CRT: with c_p = r_p w_p mod m_p,
+ * x_raw = sum_p c_p (M / m_p) = x (mod M) and x_raw / M = sum_p c_p / m_p, so x = x_raw - q M for
+ * q = round(sum_p c_p / m_p) (FP64), then at most one exact step of M into [-M/2, M/2).
+ * The limb sums c_p (M / m_p)_k stay below 2^44 (no carries until the end). |x| is rounded to FP64
+ * once (round to nearest even). Four consecutive elements of a column per thread.
  */
 template <typename T>
 __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restrict__ r8, oz_levels_t lv, int np,
-  unsigned long long hmhi, unsigned long long hmlo, unsigned long long mhi, unsigned long long mlo, int Mp, int Np,
-  const int* __restrict__ ea, const int* __restrict__ eb, int M, int N, int sh0, double alpha, double beta,
-  T* __restrict__ c, int ldc) {
-  for (long x = blockIdx.x * 256L + threadIdx.x; x < (long)M * N; x += (long)gridDim.x * 256) {
-    const int i = (int)(x % M), j = (int)(x / M);
-    const long o = (long)j * Mp + i;
-    unsigned int v[LXS_OZ_NPRIMES_MAX];
+  oz_crt_consts_t cc, int Mp, int Np, const int* __restrict__ ea, const int* __restrict__ eb, int M, int N, int sh0,
+  double alpha, double beta, T* __restrict__ c, int ldc) {
+  const int mq = Mp >> 2;
+  const long zs = (long)Mp * Np;
+  const long long hm_hi = ((long long)cc.hm[3] << 32) | cc.hm[2];
+  const unsigned long long hm_lo = ((unsigned long long)cc.hm[1] << 32) | cc.hm[0];
+  const long long m_hi = ((long long)cc.mm[3] << 32) | cc.mm[2];
+  const unsigned long long m_lo = ((unsigned long long)cc.mm[1] << 32) | cc.mm[0];
+  for (long x = blockIdx.x * 256L + threadIdx.x; x < (long)mq * N; x += (long)gridDim.x * 256) {
+    const int i0 = 4 * (int)(x % mq), j = (int)(x / mq);
+    if (i0 >= M) continue;
+    const long o = (long)j * Mp + i0;
+    unsigned long long acc[4][4];
+    double f[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      f[e] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 4; ++k) acc[e][k] = 0;
+    }
 #pragma unroll
     for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
       if (p < np) {
-        unsigned int sum = 0; /* at most 1024 items of bytes below 256 */
-        for (int z = lv.zb[p]; z < lv.zb[p + 1]; ++z) sum += r8[(long)z * Mp * Np + o];
-        unsigned int u = sum % oz_crt.m[p];
+        unsigned int s[4] = {0, 0, 0, 0};
+        for (int z = lv.zb[p]; z < lv.zb[p + 1]; ++z) { /* r_p = sum of the items' bytes mod m_p */
+          const unsigned int w = *(const unsigned int*)(r8 + z * zs + o);
 #pragma unroll
-        for (int q = 0; q < p; ++q) { /* u = (u - v_q) m_q^-1 mod m_p; 2 m_p > 256 > v_q */
-          u = oz_mod18((u + 2 * oz_crt.m[p] - v[q]) * oz_crt.inv[q][p], p);
+          for (int e = 0; e < 4; ++e) s[e] = oz_mod18(s[e] + ((w >> (8 * e)) & 0xFF), p);
         }
-        v[p] = u;
-      }
-    }
-    unsigned long long hi = 0, lo = 0; /* x = sum_p v_p prod_{q<p} m_q, Horner from the top digit */
 #pragma unroll
-    for (int p = LXS_OZ_NPRIMES_MAX - 1; 0 <= p; --p) {
-      if (p < np) {
-        const unsigned long long m = oz_crt.m[p], l = lo * m;
-        hi = hi * m + __umul64hi(lo, m);
-        lo = l + v[p];
-        hi += (lo < l ? 1 : 0);
+        for (int e = 0; e < 4; ++e) {
+          const unsigned int cp = oz_mod18(s[e] * cc.w[p], p); /* < 256 * 256 before the reduction */
+#pragma unroll
+          for (int k = 0; k < 4; ++k) acc[e][k] += (unsigned long long)cp * cc.lm[p][k];
+          f[e] = fma((double)cp, cc.rm[p], f[e]);
+        }
       }
     }
-    const bool neg = (hi > hmhi || (hi == hmhi && lo >= hmlo));
-    if (neg) { /* |x| = M - x */
-      const unsigned long long l = mlo - lo;
-      hi = mhi - hi - (mlo < lo ? 1 : 0);
-      lo = l;
-    }
-    double d; /* |x| rounded to FP64: the top 64 bits with a sticky bit below, converted with one rounding */
-    {
-      unsigned long long top = lo;
-      int scale = 0;
-      if (0 != hi) {
-        const int lz = __clzll((long long)hi);
-        top = (0 != lz ? (hi << lz) | (lo >> (64 - lz)) : hi) | ((0 != (lo << lz)) ? 1ull : 0ull);
-        scale = 64 - lz;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int i = i0 + e;
+      if (i < M) {
+        const long long q = (long long)floor(f[e] + 0.5); /* 0 .. np */
+        /* x_raw - q M in limbs, carries as arithmetic shifts */
+        const long long t0 = (long long)acc[e][0] - q * cc.mm[0];
+        const long long t1 = (long long)acc[e][1] - q * cc.mm[1] + (t0 >> 32);
+        const long long t2 = (long long)acc[e][2] - q * cc.mm[2] + (t1 >> 32);
+        const long long t3 = (long long)acc[e][3] - q * cc.mm[3] + (t2 >> 32);
+        oz_i128_t v;
+        v.hi = (t3 << 32) | (long long)(t2 & 0xFFFFFFFF);
+        v.lo = ((unsigned long long)(t1 & 0xFFFFFFFF) << 32) | (unsigned long long)(t0 & 0xFFFFFFFF);
+        { /* into [-M/2, M/2) */
+          const oz_i128_t up = oz_sub(v, hm_hi, hm_lo); /* v - M/2 >= 0: v -= M */
+          const oz_i128_t dn = oz_add(v, hm_hi, hm_lo); /* v + M/2 < 0: v += M */
+          if (0 <= up.hi) v = oz_sub(v, m_hi, m_lo);
+          else if (0 > dn.hi) v = oz_add(v, m_hi, m_lo);
+        }
+        const bool neg = (0 > v.hi);
+        unsigned long long hi = (unsigned long long)v.hi, lo = v.lo;
+        if (neg) { /* |x| */
+          lo = ~lo + 1;
+          hi = ~hi + (0 == lo ? 1 : 0);
+        }
+        double d; /* |x| rounded to FP64: the top 64 bits with a sticky bit below, converted with one rounding */
+        {
+          unsigned long long top = lo;
+          int scale = 0;
+          if (0 != hi) {
+            const int lz = __clzll((long long)hi);
+            top = (0 != lz ? (hi << lz) | (lo >> (64 - lz)) : hi) | ((0 != (lo << lz)) ? 1ull : 0ull);
+            scale = 64 - lz;
+          }
+          d = ldexp(fma((double)(unsigned int)(top >> 32), 4294967296.0, (double)(unsigned int)top), scale);
+        }
+        const int ex = ea[i], ey = eb[j];
+        const double val = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(neg ? -d : d, ex + ey + sh0) : 0.0;
+        T* const pc = c + i + (long)j * ldc;
+        *pc = (T)(0 != beta ? fma(alpha, val, beta * (double)*pc) : alpha * val);
       }
-      d = ldexp(fma((double)(unsigned int)(top >> 32), 4294967296.0, (double)(unsigned int)top), scale);
-    }
-    {
-      const int ex = ea[i], ey = eb[j];
-      const double val = (OZ_EXP_NONE != ex && OZ_EXP_NONE != ey) ? ldexp(neg ? -d : d, ex + ey + sh0) : 0.0;
-      T* const pc = c + i + (long)j * ldc;
-      *pc = (T)(0 != beta ? fma(alpha, val, beta * (double)*pc) : alpha * val);
     }
   }
 }
@@ -768,6 +853,17 @@ extern "C" int lxs_oz_level_gemm(int crt, const signed char* pa, const signed ch
       return LXS_ELIMIT;
     }
   }
+  /* LDS stages: five (ten-stage trips) when every item is long enough for the padding to vanish, else four;
+     LIBXSTREAM_OZAKI_STAGES=4|5 overrides */
+  int nst = 5;
+  {
+    const char* const s_env = getenv("LIBXSTREAM_OZAKI_STAGES");
+    const int env = (NULL != s_env ? atoi(s_env) : 0);
+    for (int z = 0; z < nitems; ++z) {
+      if (40 > items[4 * z + 2] - items[4 * z + 1]) nst = 4;
+    }
+    if (4 == env || 5 == env) nst = env;
+  }
   for (int z0 = 0; z0 < nitems; z0 += OZ_ITEMS) {
     const int nz = (nitems - z0 < OZ_ITEMS ? nitems - z0 : OZ_ITEMS);
     const dim3 grid((unsigned int)(Mp / LT), (unsigned int)(Np / LT), (unsigned int)nz);
@@ -779,14 +875,13 @@ extern "C" int lxs_oz_level_gemm(int crt, const signed char* pa, const signed ch
       arg.it[z].u1 = src[2];
       arg.it[z].pad = 0;
     }
-    if (0 == crt) {
-      oz_level_gemm<0><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const int8_t*)pa, (const int8_t*)pb, occa, occb, arg, s,
-        Mp, Np, Kp, zero, (int*)out + (long)z0 * Mp * Np);
-    }
-    else {
-      oz_level_gemm<1><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const int8_t*)pa, (const int8_t*)pb, occa, occb, arg, s,
-        Mp, Np, Kp, zero, (uint8_t*)out + (long)z0 * Mp * Np);
-    }
+    const hipStream_t st = (hipStream_t)hip_stream;
+    void* const o = (0 == crt ? (void*)((int*)out + (long)z0 * Mp * Np) : (void*)((uint8_t*)out + (long)z0 * Mp * Np));
+    const int8_t *a8 = (const int8_t*)pa, *b8 = (const int8_t*)pb;
+    if (0 == crt && 4 == nst) oz_level_gemm<0, 4><<<grid, 256, 0, st>>>(a8, b8, occa, occb, arg, s, Mp, Np, Kp, zero, o);
+    else if (0 == crt) oz_level_gemm<0, 5><<<grid, 256, 0, st>>>(a8, b8, occa, occb, arg, s, Mp, Np, Kp, zero, o);
+    else if (4 == nst) oz_level_gemm<1, 4><<<grid, 256, 0, st>>>(a8, b8, occa, occb, arg, s, Mp, Np, Kp, zero, o);
+    else oz_level_gemm<1, 5><<<grid, 256, 0, st>>>(a8, b8, occa, occb, arg, s, Mp, Np, Kp, zero, o);
   }
   return (int)hipGetLastError();
 }
@@ -813,23 +908,46 @@ extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks,
   return (int)hipGetLastError();
 }
 
-/* zb: first item per prime, np + 1 entries; hm and m: M / 2 and M = prod of the first np moduli (128 bits) */
-extern "C" int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, const unsigned long long* hm,
-  const unsigned long long* m, int Mp, int Np, const int* ea, const int* eb, int M, int N, int sh0, double alpha,
-  double beta, void* c, int ldc, void* hip_stream) {
+/* zb: first item per prime, np + 1 entries; C = alpha 2^(ea + eb + sh0) x + beta C */
+extern "C" int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, int Mp, int Np,
+  const int* ea, const int* eb, int M, int N, int sh0, double alpha, double beta, void* c, int ldc, void* hip_stream) {
+  constexpr int mod[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
   oz_levels_t lv;
-  const long blocks = ((long)M * N + 255) / 256;
+  oz_crt_consts_t cc;
+  unsigned __int128 m = 1, hm;
+  const long blocks = ((long)(Mp / 4) * N + 255) / 256;
   const unsigned int grid = (unsigned int)(blocks < 4096 ? blocks : 4096);
-  if (1 > np || LXS_OZ_NPRIMES_MAX < np || Mp < M || Np < N) return LXS_ELIMIT;
+  if (1 > np || LXS_OZ_NPRIMES_MAX < np || Mp < M || Np < N || 0 != Mp % 4) return LXS_ELIMIT;
   if (0 >= M || 0 >= N) return 0;
   for (int p = 0; p < 17; ++p) lv.zb[p] = (p <= np ? zb[p] : zb[np]);
+  memset(&cc, 0, sizeof(cc));
+  for (int p = 0; p < np; ++p) m *= (unsigned __int128)mod[p];
+  hm = m >> 1;
+  for (int p = 0; p < np; ++p) {
+    const unsigned __int128 mp = m / (unsigned __int128)mod[p];
+    const int r = (int)(mp % (unsigned __int128)mod[p]);
+    int inv = 0;
+    for (int t = 1; t < mod[p]; ++t) {
+      if (1 == (r * t) % mod[p]) {
+        inv = t;
+        break;
+      }
+    }
+    cc.w[p] = (unsigned int)inv;
+    for (int k = 0; k < 4; ++k) cc.lm[p][k] = (unsigned int)(mp >> (32 * k));
+    cc.rm[p] = 1.0 / mod[p];
+  }
+  for (int k = 0; k < 4; ++k) {
+    cc.mm[k] = (unsigned int)(m >> (32 * k));
+    cc.hm[k] = (unsigned int)(hm >> (32 * k));
+  }
   if (3 == dtype) {
-    oz_crt_recombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, hm[0], hm[1], m[0], m[1],
-      Mp, Np, ea, eb, M, N, sh0, alpha, beta, (double*)c, ldc);
+    oz_crt_recombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, cc, Mp, Np, ea, eb,
+      M, N, sh0, alpha, beta, (double*)c, ldc);
   }
   else if (1 == dtype) {
-    oz_crt_recombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, hm[0], hm[1], m[0], m[1],
-      Mp, Np, ea, eb, M, N, sh0, alpha, beta, (float*)c, ldc);
+    oz_crt_recombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, cc, Mp, Np, ea, eb,
+      M, N, sh0, alpha, beta, (float*)c, ldc);
   }
   else {
     return LXS_ENOVARIANT;

@@ -219,3 +219,18 @@ def test_complex_3m(stream, ta, tb):
     got = out.view(np.complex128).reshape(n, m).T
     ref = alpha * (A @ B) + beta * C0  # numpy complex128 product: rounding ~1e-16, far below the bound
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-13
+
+
+@pytest.mark.parametrize("stages", [4, 5])
+def test_bitexact_lds_stages(stream, monkeypatch, stages):
+    """both pipeline depths of the int8 GEMM (4 or 5 LDS stages; 5 pads items to ten-stage trips)"""
+    monkeypatch.setenv("LIBXSTREAM_OZAKI_STAGES", str(stages))
+    rng = np.random.default_rng(40 + stages)
+    m, n, k = 300, 260, 1500
+    A, B, C0 = _mixed(rng, m, k), _mixed(rng, k, n), rng.uniform(-1, 1, (m, n))
+    oz = lx.Ozaki(nslices=6, trim=2)
+    try:
+        out, args = _run(oz, stream, "N", "T", m, n, k, 1.0, A, B, 0.5, C0)
+    finally:
+        oz.close()
+    assert np.array_equal(out, _expect("N", "T", m, n, k, 1.0, args, 0.5, 6, trim=2))

@@ -246,3 +246,18 @@ def test_crt_complex_3m(stream):
     got = out.view(np.complex128).reshape(n, m).T
     ref = alpha * (A @ B) + beta * C
     assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-14
+
+
+@pytest.mark.parametrize("stages", [4, 5])
+def test_crt_lds_stages(stream, monkeypatch, stages):
+    """both pipeline depths of the int8 GEMM (4 or 5 LDS stages; 5 pads items to ten-stage trips)"""
+    monkeypatch.setenv("LIBXSTREAM_OZAKI_STAGES", str(stages))
+    rng = np.random.default_rng(50 + stages)
+    m, n, k = 270, 300, 2100
+    A, B, C0 = _mixed(rng, m, k), _mixed(rng, k, n), rng.uniform(-1, 1, (m, n))
+    oz = lx.Ozaki(kind=2)
+    try:
+        out, args = _run(oz, stream, "T", "N", m, n, k, 1.0, A, B, -1.0, C0)
+    finally:
+        oz.close()
+    assert np.array_equal(out, _expect("T", "N", m, n, k, 1.0, args, -1.0))
