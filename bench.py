@@ -227,8 +227,9 @@ def _event_ms(lx, stream, fn, reps):
 
 
 # (kind, slices or primes, trim): scheme II with 16 moduli (the reference's default kind 3 picks it at these
-# sizes), scheme I over the triangle at 7 and 8 slices, and scheme I at the reference's default 8 slices
-OZAKI_CONFIGS = ((2, 16, 0), (1, 7, 6), (1, 8, 7), (1, 8, 0))
+# sizes), scheme I over the slice triangle (cutoff s - 1) for s = 3 .. 8, and scheme I at the reference's
+# default 8 slices over all pairs
+OZAKI_CONFIGS = ((2, 16, 0),) + tuple((1, s, s - 1) for s in range(3, 9)) + ((1, 8, 0),)
 
 
 def ozaki_section(lx, stream, sizes, reps=3):
@@ -304,6 +305,91 @@ def ozaki_section(lx, stream, sizes, reps=3):
         "native": "own v_mfma_f64_16x16x4 kernel (ozaki_dgemm_native) and rocBLAS dgemm",
         "results": results,
     }
+
+
+def _stack_sampled_reference(stack, a, bl, m, n, k, rng, count=2048):
+    """flat C positions of `count` sampled elements of the C blocks the (sorted) stack addresses, and their
+    values C = sum over the block's entries of A B as 80-bit extended-precision dot products (C zero)"""
+    cofs = stack[:, 2] - 1
+    blocks = np.unique(cofs)
+    pick = blocks[rng.integers(0, len(blocks), count)]
+    i, j = rng.integers(0, m, count), rng.integers(0, n, count)
+    ref = np.zeros(count, dtype=np.longdouble)
+    af = a.reshape(-1).astype(np.longdouble)
+    bf = bl.reshape(-1).astype(np.longdouble)
+    kk = np.arange(k)
+    for c0 in blocks:
+        ents = stack[cofs == c0]
+        s = np.nonzero(pick == c0)[0]
+        if len(s) == 0:
+            continue
+        arow = af[(ents[:, 0] - 1)[:, None, None] + i[s][None, :, None] + kk[None, None, :] * m]  # A(i, kk)
+        bcol = bf[(ents[:, 1] - 1)[:, None, None] + kk[None, None, :] * n + j[s][None, :, None]]  # B(kk, j)
+        ref[s] = (arow * bcol).sum(axis=(0, 2))
+    return pick + i + j * m, ref
+
+
+OZAKI_STACK_CONFIGS = tuple((s, s - 1) for s in range(3, 9)) + ((8, 0),)  # (slices, trim): triangles, all pairs
+
+
+def ozaki_stack_section(lx, stream, stack_size=100000, reps=3):
+    """Batched 64^3 stack (100k entries; pools of 2000 A, 2000 B, 1000 C blocks): native FP64
+    (libsmm_acc_process) against ozaki_stack at slice counts 3 .. 8. Blocks hold mantissas uniform
+    +-[1,2) times 2^x per A-block row and per B-block column, x uniform in [-e, e]."""
+    from libxstream_amd.workload import make_stack_workload
+
+    m = n = k = 64
+    w = make_stack_workload(m, n, k, stack_size, nc=1000, na=2000, nb=2000, dtype=np.float64)
+    rng = np.random.default_rng(20260701)
+    cblk = (w.stack[:, 2] - 1) // (m * n)
+    head = int(np.searchsorted(cblk, 40))  # the entries of the first 40 C blocks (sorted stack)
+    flops = 2.0 * m * n * k * stack_size
+    st = lx.DeviceBuffer.from_array(w.stack)
+    zeros = np.zeros(w.c.size)
+    entry = {"stack_size": stack_size, "m": m, "n": n, "k": k, "blocks_a_b_c": [2000, 2000, 1000], "configs": []}
+    for e in (10, 0, 30):  # e = 10 first: it carries the timing
+        sgn = lambda shape: np.where(rng.integers(0, 2, shape, dtype=np.int8) != 0, 1.0, -1.0)
+        A = rng.uniform(1, 2, (2000, k, m)) * sgn((2000, k, m)) * np.exp2(rng.integers(-e, e + 1, (2000, 1, m)))
+        Bh = rng.uniform(1, 2, (2000, n, k)) * sgn((2000, n, k)) * np.exp2(rng.integers(-e, e + 1, (2000, n, 1)))
+        a = np.ascontiguousarray(A.reshape(2000, -1))
+        bl = np.ascontiguousarray(Bh.transpose(0, 2, 1).reshape(2000, -1))  # kernel layout: k x n row-major
+        da, db = lx.DeviceBuffer.from_array(a), lx.DeviceBuffer.from_array(bl)
+        dc = lx.DeviceBuffer.from_array(zeros)
+        out = np.empty(w.c.size)
+        sel, ref = _stack_sampled_reference(w.stack[:head], a, bl, m, n, k, rng)
+        err = lambda: float(np.sqrt(((out[sel].astype(np.longdouble) - ref) ** 2).sum() / (ref ** 2).sum()))
+        if e == 10:
+            ms = _event_ms(lx, stream, lambda: lx.process(st, stack_size, lx.REAL_8, da, db, dc, m, n, k, stream), reps)
+            entry["native_ms"] = round(ms, 4)
+            entry["native_tflops"] = round(flops / ms * 1e-9, 2)
+            entry["native_errors"] = {}
+        dc.upload(zeros)
+        lx.process(st, stack_size, lx.REAL_8, da, db, dc, m, n, k, stream)
+        stream.sync()
+        dc.download(out)
+        entry["native_errors"]["e=%d" % e] = err()
+        for cfg_i, (sl, trim) in enumerate(OZAKI_STACK_CONFIGS):
+            oz = lx.Ozaki(nslices=sl, trim=trim)
+            if e == 10:
+                ms = _event_ms(lx, stream, lambda: oz.stack(st, stack_size, m, n, k, da, 2000, db, 2000, dc, stream), reps)
+                pairs = sum(1 for p in range(sl) for q in range(sl) if p + q <= 2 * (sl - 1) - trim)
+                entry["configs"].append({"slices": sl, "trim": trim, "pairs": pairs, "ms": round(ms, 4),
+                                         "fp64_equiv_tflops": round(flops / ms * 1e-9, 2),
+                                         "speedup_vs_native": round(entry["native_ms"] / ms, 3), "errors": {}})
+            dc.upload(zeros)
+            oz.stack(st, stack_size, m, n, k, da, 2000, db, 2000, dc, stream)
+            stream.sync()
+            dc.download(out)
+            oz.close()
+            entry["configs"][cfg_i]["errors"]["e=%d" % e] = err()
+        for x in (da, db, dc):
+            x.free()
+    st.free()
+    entry["workload"] = ("64x64x64 stack of %d entries sorted by C (pools 2000/2000/1000 blocks); A blocks m 2^x per row, "
+                         "B blocks m 2^y per column, m uniform +-[1,2), x, y uniform in [-e, e]; timing at e=10" % stack_size)
+    entry["error"] = ("normwise over 2048 sampled elements of the first 40 C blocks against 80-bit extended-precision "
+                      "dot products")
+    return entry
 
 
 def learned_section(lx):
@@ -441,6 +527,7 @@ def main():
         lx.init()
         stream = lx.Stream("bench")
         out = ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
+        out["stack_64"] = ozaki_stack_section(lx, stream)
         stream.close()
         lx.finalize()
         print(json.dumps({"ozaki": out}))
@@ -470,6 +557,8 @@ def main():
                  "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
     batch = None if args.no_batch else batched(lx, stream, args.steps, args.warmup)
     ozaki = None if args.no_ozaki else ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
+    if ozaki is not None:
+        ozaki["stack_64"] = ozaki_stack_section(lx, stream)
     learned_rep = None if args.no_learned else learned_section(lx)
     ncus, arch = lx.device_info()
     stream.close()

@@ -8,8 +8,9 @@
  * Scheme: every row of op(A) and column of op(B) is scaled by a power of two and cut into nslices
  * signed 7-bit digits (int8 planes); slice pairs (p, q) with p + q <= cutoff = 2(nslices-1) - oztrim
  * are multiplied with int8 MFMA, summed exactly per level p + q and recombined in FP64.
- * Only Scheme 1 (mantissa slicing) is built: kind 2 (CRT) is rejected, kind 0/3 (adaptive) resolves
- * to Scheme 1.
+ * Scheme 2 (kind 2) turns the operands into int8 residues modulo 16 pairwise coprime moduli <= 256 and
+ * reconstructs the exact integer product by CRT before one rounding to FP64; kind 0/3 picks a scheme per
+ * call (samples/ozaki/ozaki_gemm.c:520-587).
  */
 #ifndef OZAKI_H
 #define OZAKI_H
@@ -94,5 +95,20 @@ LIBXSTREAM_API void ozaki_invalidate_cache(ozaki_context_t* ctx, const void* a, 
  */
 LIBXSTREAM_API int ozaki_dgemm_native(libxstream_stream_t* stream, char transa, char transb, int M, int N, int K,
   double alpha, const double* a, int lda, const double* b, int ldb, double beta, double* c, int ldc);
+
+/**
+ * No counterpart in the reference (its Ozaki sample multiplies whole matrices): Ozaki scheme I over a
+ * DBCSR parameter stack, C[c] += A[a] B[b] for every entry, with the semantics of libsmm_acc_process
+ * (include/acc_libsmm.h): stack of stack_size triplets of 1-based element offsets (a, b, c), A blocks
+ * m x k column-major, B blocks in the kernel layout (k x n row-major, as libsmm_acc_transpose leaves
+ * them), C blocks m x n column-major, 1 <= m, n, k <= 64, FP64 or FP32 as ctx->use_double. na and nb are
+ * the blocks of the A and B pools (every block is sliced once per call; an entry whose A or B offset is
+ * not a block start inside its pool contributes nothing). Each product is ozaki_gemm's scheme I with the
+ * context's slice count and trim (8 slices over all pairs for a kind 2 context), added to C entry by
+ * entry in stack order: C = C + v (FP32: rounded through FP64). A stack whose C offsets decrease
+ * somewhere runs on one work-group, in order.
+ */
+LIBXSTREAM_API int ozaki_stack(ozaki_context_t* ctx, libxstream_stream_t* stream, const int* dev_stack, int stack_size,
+  int m, int n, int k, const void* dev_a, int na, const void* dev_b, int nb, void* dev_c);
 
 #endif /*OZAKI_H*/

@@ -153,6 +153,7 @@ def _declare(L):
                                    ctypes.POINTER(ctypes.c_double), vp, i, vp, i, ctypes.POINTER(ctypes.c_double), vp,
                                    i]),
         "ozaki_invalidate_cache": (None, [ctypes.POINTER(OzakiContext), vp, vp]),
+        "ozaki_stack": (i, [ctypes.POINTER(OzakiContext), vp, vp, i, i, i, i, vp, i, vp, i, vp]),
         "ozaki_dgemm_native": (i, [vp, ctypes.c_char, ctypes.c_char, i, i, i, ctypes.c_double, vp, i, vp, i,
                                    ctypes.c_double, vp, i]),
     }
@@ -423,6 +424,12 @@ class Ozaki:
         check(lib().ozaki_gemm_complex(ctypes.byref(self.ctx), stream.handle if stream else None, transa.encode(),
                                        transb.encode(), m, n, k, al, a.ptr, lda, b.ptr, ldb, be, c.ptr, ldc),
               "ozaki_gemm_complex")
+
+    def stack(self, stack, stack_size, m, n, k, a, na, b, nb, c, stream=None):
+        """C[c] += A[a] B[b] over a parameter stack (DeviceBuffers; B blocks in the kernel layout), each
+        product by scheme I with this context's slices (include/ozaki.h: ozaki_stack)."""
+        check(lib().ozaki_stack(ctypes.byref(self.ctx), stream.handle if stream else None, stack.ptr, stack_size, m, n,
+                                k, a.ptr, na, b.ptr, nb, c.ptr), "ozaki_stack")
 
     def scratch_size(self, transa, transb, m, n, k, lda, ldb, ldc):
         return lib().ozaki_scratch_size(ctypes.byref(self.ctx), transa.encode(), transb.encode(), m, n, k, lda, ldb,

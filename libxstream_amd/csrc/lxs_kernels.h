@@ -93,11 +93,10 @@ int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* h
 #define LXS_OZ_MODULI 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191
 int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int trim, int Rp, int Kp,
   int* e, signed char* planes, void* hip_stream);
-/* C = alpha 2^(ea + eb + sh0) x + beta C, x the integer of the residues (sum of the residue bytes of the
-   prime's items zb[p] .. zb[p + 1] - 1, mod m_p), hm = M / 2 and m = M as {high, low} 64-bit words */
-int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, const unsigned long long* hm,
-  const unsigned long long* m, int Mp, int Np, const int* ea, const int* eb, int M, int N, int sh0, double alpha,
-  double beta, void* c, int ldc, void* hip_stream);
+/* C = alpha 2^(ea + eb + sh0) x + beta C, x the integer (|x| < M / 2, M the product of the first np moduli) of
+   the residues r_p = sum of the residue bytes of the prime's items zb[p] .. zb[p + 1] - 1, mod m_p */
+int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, int Mp, int Np, const int* ea,
+  const int* eb, int M, int N, int sh0, double alpha, double beta, void* c, int ldc, void* hip_stream);
 int lxs_dgemm(const double* a, long ars, long aks, const double* b, long bks, long bcs, int M, int N, int K,
   double alpha, double beta, double* c, int ldc, void* hip_stream);
 /* complex 3M helpers: split interleaved (rows x cols, ld) into re, im, re + im (ld rows), conj negates im;
@@ -106,6 +105,12 @@ int lxs_oz_csplit(int dtype, const void* z, int ld, int rows, int cols, int conj
   void* hip_stream);
 int lxs_oz_ccombine(int dtype, const void* t1, const void* t2, const void* t3, int M, int N, const double* alpha,
   const double* beta, void* c, int ldc, void* hip_stream);
+
+/* Ozaki scheme I over a parameter stack (ozaki_stack.hip): planes_a na * s * 4 KB, planes_b nb * s * 4 KB,
+   ea na * 64 ints, eb nb * 64 ints, info 1 int of device scratch */
+int lxs_ozs_process(int dtype, const int* stack, int S, int m, int n, int k, const void* a, int na, const void* b,
+  int nb, void* c, int s, int cutoff, signed char* planes_a, int* ea, signed char* planes_b, int* eb, int* info,
+  void* hip_stream);
 
 /* learned predictor MLP (mlp.hip): activations [batch][features], weights [out][in] then bias [out] */
 int lxs_mlp_forward(const float* x, const float* w, int batch, int nin, int nout, int relu, float* y, void* hip_stream);

@@ -34,6 +34,8 @@ typedef struct oz_impl_t {
   size_t user_size;
   void* carena; /* complex 3M operands and products */
   size_t carena_size;
+  void* sarena; /* stack path: digit planes and exponents of the pools */
+  size_t sarena_size;
 } oz_impl_t;
 
 static const int oz_moduli[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
@@ -138,10 +140,11 @@ LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_doub
 LIBXSTREAM_API void ozaki_destroy(ozaki_context_t* ctx) {
   if (NULL != ctx && NULL != ctx->impl) {
     oz_impl_t* const impl = (oz_impl_t*)ctx->impl;
-    if (NULL != impl->arena || NULL != impl->carena) {
+    if (NULL != impl->arena || NULL != impl->carena || NULL != impl->sarena) {
       (void)hipDeviceSynchronize();
       if (NULL != impl->arena) (void)hipFree(impl->arena);
       if (NULL != impl->carena) (void)hipFree(impl->carena);
+      if (NULL != impl->sarena) (void)hipFree(impl->sarena);
     }
     free(impl->items);
     free(impl);
@@ -451,10 +454,7 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
         const double bc = (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2);
         token = lxs_profile_begin(hs);
         if (0 != l.crt) {
-          const oz_u128_t m = oz_crt_product(np), hm = m >> 1;
-          const unsigned long long mw[2] = {(unsigned long long)(m >> 64), (unsigned long long)m};
-          const unsigned long long hw[2] = {(unsigned long long)(hm >> 64), (unsigned long long)hm};
-          result = lxs_oz_crt_recombine(dtype, (const unsigned char*)(base + l.out), zb, np, hw, mw, l.Mp, l.Np,
+          result = lxs_oz_crt_recombine(dtype, (const unsigned char*)(base + l.out), zb, np, l.Mp, l.Np,
             (const int*)(base + l.ea), (const int*)(base + l.eb), M, N, 2 * trim - 2 * mb, alpha, (0 == k0 ? beta : 1.0), dc,
             ldc, hs);
           lxs_profile_end(token, hs, "ozaki_fp64", 0.0, (double)nitems * M * N + bc);
@@ -567,4 +567,48 @@ LIBXSTREAM_API int ozaki_dgemm_native(libxstream_stream_t* stream, char transa, 
     beta, c, ldc, hs);
   lxs_profile_end(token, hs, "dgemm", 2.0 * M * N * K, 8.0 * ((double)M * K + (double)K * N + 2.0 * M * N));
   return (0 == result ? EXIT_SUCCESS : lxs_error(EXIT_FAILURE, "ozaki_dgemm_native: kernel launch failed"));
+}
+
+LIBXSTREAM_API int ozaki_stack(ozaki_context_t* ctx, libxstream_stream_t* stream, const int* dev_stack, int stack_size,
+  int m, int n, int k, const void* dev_a, int na, const void* dev_b, int nb, void* dev_c) {
+  const int dtype = (NULL != ctx && 0 != ctx->use_double ? 3 : 1);
+  oz_impl_t* impl;
+  hipStream_t hs;
+  size_t pla, plb, need;
+  char* base;
+  void* token;
+  int s, result;
+  if (NULL == ctx || NULL == ctx->impl) return lxs_error(EXIT_FAILURE, "ozaki_stack: context not initialised");
+  if (0 > stack_size || 1 > m || 64 < m || 1 > n || 64 < n || 1 > k || 64 < k) {
+    return lxs_error(EXIT_FAILURE, "ozaki_stack: invalid argument (blocks of at most 64 x 64 x 64)");
+  }
+  if (0 == stack_size) return EXIT_SUCCESS;
+  if (NULL == dev_stack || NULL == dev_a || NULL == dev_b || NULL == dev_c || 0 >= na || 0 >= nb) {
+    return lxs_error(EXIT_FAILURE, "ozaki_stack: invalid argument");
+  }
+  if (0 != lxs_ensure_device()) return EXIT_FAILURE;
+  impl = (oz_impl_t*)ctx->impl;
+  hs = lxs_stream_hip(stream);
+  s = ctx->nslices;
+  pla = oz_up((size_t)na * s * 4096, OZ_ALIGN);
+  plb = oz_up((size_t)nb * s * 4096, OZ_ALIGN);
+  need = pla + plb + oz_up(sizeof(int) * 64 * (size_t)na, OZ_ALIGN) + oz_up(sizeof(int) * 64 * (size_t)nb, OZ_ALIGN) + OZ_ALIGN;
+  if (impl->sarena_size < need) {
+    if (NULL != impl->sarena) {
+      if (0 != lxs_hip(hipStreamSynchronize(hs), "ozaki_stack: stream synchronize")) return EXIT_FAILURE;
+      (void)hipFree(impl->sarena);
+      impl->sarena = NULL;
+      impl->sarena_size = 0;
+    }
+    if (0 != lxs_hip(hipMalloc(&impl->sarena, need), "ozaki_stack: scratch allocation")) return EXIT_FAILURE;
+    impl->sarena_size = need;
+  }
+  base = (char*)impl->sarena;
+  token = lxs_profile_begin(hs);
+  result = lxs_ozs_process(dtype, dev_stack, stack_size, m, n, k, dev_a, na, dev_b, nb, dev_c, s, oz_cutoff(ctx),
+    (signed char*)base, (int*)(base + pla + plb), (signed char*)(base + pla),
+    (int*)(base + pla + plb + oz_up(sizeof(int) * 64 * (size_t)na, OZ_ALIGN)), (int*)(base + need - OZ_ALIGN), hs);
+  lxs_profile_end(token, hs, "ozaki_stack", 2.0 * stack_size * m * n * k,
+    (double)stack_size * (0 != ctx->use_double ? 8 : 4) * ((double)m * k + (double)k * n + 2.0 * m * n));
+  return (0 == result ? EXIT_SUCCESS : lxs_error(EXIT_FAILURE, "ozaki_stack: kernel launch failed"));
 }

@@ -192,6 +192,22 @@ def ozaki_gemm(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, s, 
     return out
 
 
+def ozaki_stack(m, n, k, stack, a, b, c, s, trim=0, dtype=np.float64):
+    """Scheme I over a parameter stack: every entry (1-based element offsets a, b, c) in stack order is
+    the scheme I product of its A block (m x k column-major) and B block (k x n row-major, i.e. an n x k
+    column-major store read transposed) added to its C block, C = fma(1, v, 1 C) as ozaki_gemm computes
+    with alpha = beta = 1. Returns the updated copy of the flat C pool."""
+    a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+    b = np.ascontiguousarray(b, dtype=dtype).reshape(-1)
+    out = np.array(c, dtype=dtype, copy=True).reshape(-1)
+    for ao, bo, co in np.asarray(stack, dtype=np.int64).reshape(-1, 3):
+        ab = a[ao - 1:ao - 1 + m * k]
+        bb = b[bo - 1:bo - 1 + k * n]
+        out[co - 1:co - 1 + m * n] = ozaki_gemm("N", "T", m, n, k, 1.0, ab, m, bb, n, 1.0, out[co - 1:co - 1 + m * n],
+                                                m, s, trim=trim)
+    return out
+
+
 OZ2_PLANT_NONE, OZ2_PLANT_ROUND, OZ2_PLANT_SIGNB, OZ2_PLANT_SCALE, OZ2_PLANT_FPSUM = 0, 1, 2, 3, 4
 
 
