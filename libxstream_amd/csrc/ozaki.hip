@@ -480,6 +480,8 @@ __global__ void __launch_bounds__(256) oz_recombine(const int* __restrict__ c32,
  */
 struct oz_crt_tab_t {
   unsigned int m[LXS_OZ_NPRIMES_MAX], rcp[LXS_OZ_NPRIMES_MAX], k18[LXS_OZ_NPRIMES_MAX], k36[LXS_OZ_NPRIMES_MAX];
+  unsigned int rcp27[LXS_OZ_NPRIMES_MAX]; /* ceil(2^39 / m) < 2^32 (m > 128): exact floor(x / m) for x < 2^27 */
+  float rmf[LXS_OZ_NPRIMES_MAX];          /* 1 / m rounded to FP32 */
   unsigned int inv[LXS_OZ_NPRIMES_MAX][LXS_OZ_NPRIMES_MAX];
 };
 constexpr int oz_inv_mod(int a, int m) { /* a^-1 mod m (gcd(a, m) = 1), extended Euclid */
@@ -497,6 +499,8 @@ constexpr oz_crt_tab_t oz_crt_make() {
     const unsigned long long m = (unsigned long long)mod[i];
     t.m[i] = (unsigned int)m;
     t.rcp[i] = (unsigned int)(((1ull << 32) + m - 1) / m);
+    t.rcp27[i] = (unsigned int)(((1ull << 39) + m - 1) / m);
+    t.rmf[i] = 1.0f / (float)m;
     t.k18[i] = (unsigned int)((1ull << 18) % m);
     t.k36[i] = (unsigned int)((1ull << 36) % m);
     for (int j = 0; j < i; ++j) t.inv[j][i] = (unsigned int)oz_inv_mod(mod[j] % mod[i], mod[i]);
@@ -506,7 +510,11 @@ constexpr oz_crt_tab_t oz_crt_make() {
 __constant__ oz_crt_tab_t oz_crt = oz_crt_make();
 
 __device__ __forceinline__ unsigned int oz_mod18(unsigned int x, int p) { /* x mod m_p for x < 2^18 */
-  return x - __umulhi(x, oz_crt.rcp[p]) * oz_crt.m[p];
+  return x - __umul24(__umulhi(x, oz_crt.rcp[p]), oz_crt.m[p]);
+}
+/* x mod m_p for x < 2^27: the excess x (rcp27 - 2^39 / m) / 2^39 < 2^-12 stays below 1 / m */
+__device__ __forceinline__ unsigned int oz_mod27(unsigned int x, int p) {
+  return x - __umul24(__umulhi(x, oz_crt.rcp27[p]) >> 7, oz_crt.m[p]);
 }
 
 /*
@@ -550,8 +558,9 @@ __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long
       } d;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        const unsigned int x = oz_mod18(c2[i], pl) * oz_crt.k36[pl] + oz_mod18(c1[i], pl) * oz_crt.k18[pl] + oz_mod18(c0[i], pl);
-        int s = (int)oz_mod18(x, pl); /* x < 3 * 2^16 */
+        /* a = c2 2^36 + c1 2^18 + c0 = c2 k36 + c1 k18 + c0 (mod m), which is below 2^26.6 */
+        const unsigned int x = __umul24(c2[i], oz_crt.k36[pl]) + __umul24(c1[i], oz_crt.k18[pl]) + c0[i];
+        int s = (int)oz_mod27(x, pl);
         if (0 != ((neg >> i) & 1)) s = (0 != s ? m - s : 0);
         d.b[i] = (int8_t)(s >= (m + 1) / 2 ? s - m : s);
       }
@@ -560,13 +569,39 @@ __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long
   }
 }
 
-/* constants of the reconstruction for the first np moduli (M their product) */
-struct oz_crt_consts_t {
-  unsigned int w[LXS_OZ_NPRIMES_MAX];     /* ((M / m_p) mod m_p)^-1 mod m_p */
-  unsigned int lm[LXS_OZ_NPRIMES_MAX][4]; /* M / m_p in 32-bit limbs, least significant first */
-  double rm[LXS_OZ_NPRIMES_MAX];          /* 1 / m_p */
-  unsigned int mm[4], hm[4];              /* M and M / 2 in limbs */
+/*
+ * Constants of the reconstruction for every prime count np (M = product of the first np moduli),
+ * computed at compile time: w = ((M / m_p) mod m_p)^-1 mod m_p, M / m_p in 16-bit limbs (least
+ * significant first), M in 16-bit limbs, M / 2 and M as {high, low} 64-bit words.
+ */
+struct oz_crt_rec_t {
+  unsigned int w[LXS_OZ_NPRIMES_MAX + 1][LXS_OZ_NPRIMES_MAX];
+  unsigned int l16[LXS_OZ_NPRIMES_MAX + 1][LXS_OZ_NPRIMES_MAX][8];
+  unsigned int m16[LXS_OZ_NPRIMES_MAX + 1][8];
+  unsigned long long hm[LXS_OZ_NPRIMES_MAX + 1][2], mm[LXS_OZ_NPRIMES_MAX + 1][2];
+  double rm[LXS_OZ_NPRIMES_MAX]; /* 1 / m_p */
 };
+constexpr oz_crt_rec_t oz_crt_rec_make() {
+  constexpr int mod[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
+  oz_crt_rec_t t{};
+  for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) t.rm[p] = 1.0 / mod[p];
+  for (int np = 1; np <= LXS_OZ_NPRIMES_MAX; ++np) {
+    unsigned __int128 m = 1;
+    for (int p = 0; p < np; ++p) m *= (unsigned __int128)mod[p];
+    for (int k = 0; k < 8; ++k) t.m16[np][k] = (unsigned int)((m >> (16 * k)) & 0xFFFF);
+    t.mm[np][0] = (unsigned long long)(m >> 64);
+    t.mm[np][1] = (unsigned long long)m;
+    t.hm[np][0] = (unsigned long long)((m >> 1) >> 64);
+    t.hm[np][1] = (unsigned long long)(m >> 1);
+    for (int p = 0; p < np; ++p) {
+      const unsigned __int128 mp = m / (unsigned __int128)mod[p];
+      t.w[np][p] = (unsigned int)oz_inv_mod((int)(mp % (unsigned __int128)mod[p]), mod[p]);
+      for (int k = 0; k < 8; ++k) t.l16[np][p][k] = (unsigned int)((mp >> (16 * k)) & 0xFFFF);
+    }
+  }
+  return t;
+}
+__constant__ oz_crt_rec_t oz_rec = oz_crt_rec_make();
 
 /* 128-bit two's complement value (hi signed, lo unsigned) */
 struct oz_i128_t {
@@ -591,46 +626,57 @@ __device__ __forceinline__ oz_i128_t oz_sub(oz_i128_t a, long long bhi, unsigned
  * r_p = sum of the prime's residue bytes mod m_p, |x| < M / 2. Direct CRT: with c_p = r_p w_p mod m_p,
  * x_raw = sum_p c_p (M / m_p) = x (mod M) and x_raw / M = sum_p c_p / m_p, so x = x_raw - q M for
  * q = round(sum_p c_p / m_p) (FP64), then at most one exact step of M into [-M/2, M/2).
- * The limb sums c_p (M / m_p)_k stay below 2^44 (no carries until the end). |x| is rounded to FP64
- * once (round to nearest even). Four consecutive elements of a column per thread.
+ * x_raw is summed in eight 16-bit limbs of 32-bit accumulators (c_p (M / m_p)_k < 2^24, 16 terms < 2^28:
+ * no carries until the end). |x| is rounded to FP64 once (round to nearest even). Four consecutive
+ * elements of a column per thread.
  */
 template <typename T>
 __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restrict__ r8, oz_levels_t lv, int np,
-  oz_crt_consts_t cc, int Mp, int Np, const int* __restrict__ ea, const int* __restrict__ eb, int M, int N, int sh0,
-  double alpha, double beta, T* __restrict__ c, int ldc) {
-  const int mq = Mp >> 2;
+  int Mp, int Np, const int* __restrict__ ea, const int* __restrict__ eb, int M, int N, int sh0, double alpha,
+  double beta, T* __restrict__ c, int ldc) {
+  /* one thread per 4 elements, no loop: the constants are read where they are used (scalar operands) */
+  const unsigned int mq = (unsigned int)Mp >> 2, total = mq * (unsigned int)N; /* < 2^32 (launcher) */
   const long zs = (long)Mp * Np;
-  const long long hm_hi = ((long long)cc.hm[3] << 32) | cc.hm[2];
-  const unsigned long long hm_lo = ((unsigned long long)cc.hm[1] << 32) | cc.hm[0];
-  const long long m_hi = ((long long)cc.mm[3] << 32) | cc.mm[2];
-  const unsigned long long m_lo = ((unsigned long long)cc.mm[1] << 32) | cc.mm[0];
-  for (long x = blockIdx.x * 256L + threadIdx.x; x < (long)mq * N; x += (long)gridDim.x * 256) {
-    const int i0 = 4 * (int)(x % mq), j = (int)(x / mq);
-    if (i0 >= M) continue;
-    const long o = (long)j * Mp + i0;
-    unsigned long long acc[4][4];
+  const long long hm_hi = (long long)oz_rec.hm[np][0], m_hi = (long long)oz_rec.mm[np][0];
+  const unsigned long long hm_lo = oz_rec.hm[np][1], m_lo = oz_rec.mm[np][1];
+  {
+    const unsigned int x = blockIdx.x * 256u + threadIdx.x;
+    if (x >= total) return;
+    const unsigned int jq = x / mq;
+    const int i0 = (int)(4 * (x - jq * mq)), j = (int)jq;
+    if (i0 >= M) return;
+    const uint8_t* const r8o = r8 + ((long)j * Mp + i0);
+    unsigned int acc[4][8];
     double f[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       f[e] = 0.0;
 #pragma unroll
-      for (int k = 0; k < 4; ++k) acc[e][k] = 0;
+      for (int k = 0; k < 8; ++k) acc[e][k] = 0;
     }
 #pragma unroll
     for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
       if (p < np) {
+        const unsigned int mp = oz_crt.m[p];
+        const float rmp = oz_crt.rmf[p];
         unsigned int s[4] = {0, 0, 0, 0};
-        for (int z = lv.zb[p]; z < lv.zb[p + 1]; ++z) { /* r_p = sum of the items' bytes mod m_p */
-          const unsigned int w = *(const unsigned int*)(r8 + z * zs + o);
+        for (int z = lv.zb[p]; z < lv.zb[p + 1]; ++z) { /* r_p = sum of the items' bytes (each < m_p) mod m_p */
+          const unsigned int w = *(const unsigned int*)(r8o + z * zs);
 #pragma unroll
-          for (int e = 0; e < 4; ++e) s[e] = oz_mod18(s[e] + ((w >> (8 * e)) & 0xFF), p);
+          for (int e = 0; e < 4; ++e) {
+            const unsigned int t = s[e] + ((w >> (8 * e)) & 0xFF);
+            s[e] = (t >= mp ? t - mp : t);
+          }
         }
+        const unsigned int wp = oz_rec.w[np][p];
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const unsigned int cp = oz_mod18(s[e] * cc.w[p], p); /* < 256 * 256 before the reduction */
+          /* cp = s w mod m_p, s w < 2^16: floor(s w / m) from FP32 is exact but at multiples of m (r = m) */
+          const unsigned int sw_ = __umul24(s[e], wp), r = sw_ - __umul24((unsigned int)((float)sw_ * rmp), mp);
+          const unsigned int cp = (r >= mp ? r - mp : r);
 #pragma unroll
-          for (int k = 0; k < 4; ++k) acc[e][k] += (unsigned long long)cp * cc.lm[p][k];
-          f[e] = fma((double)cp, cc.rm[p], f[e]);
+          for (int k = 0; k < 8; ++k) acc[e][k] = __umul24(cp, oz_rec.l16[np][p][k]) + acc[e][k];
+          f[e] = fma((double)cp, oz_rec.rm[p], f[e]);
         }
       }
     }
@@ -638,15 +684,20 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
     for (int e = 0; e < 4; ++e) {
       const int i = i0 + e;
       if (i < M) {
-        const long long q = (long long)floor(f[e] + 0.5); /* 0 .. np */
-        /* x_raw - q M in limbs, carries as arithmetic shifts */
-        const long long t0 = (long long)acc[e][0] - q * cc.mm[0];
-        const long long t1 = (long long)acc[e][1] - q * cc.mm[1] + (t0 >> 32);
-        const long long t2 = (long long)acc[e][2] - q * cc.mm[2] + (t1 >> 32);
-        const long long t3 = (long long)acc[e][3] - q * cc.mm[3] + (t2 >> 32);
+        const int q = (int)floor(f[e] + 0.5); /* 0 .. np */
+        /* x_raw - q M in 16-bit limbs, carries as arithmetic shifts; the top limb keeps the sign */
+        unsigned long long lo = 0, hi = 0;
+        int carry = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+          const int t = (int)acc[e][k] - q * (int)oz_rec.m16[np][k] + carry;
+          carry = t >> 16;
+          if (k < 4) lo |= (unsigned long long)(t & 0xFFFF) << (16 * k);
+          else hi |= (unsigned long long)(t & 0xFFFF) << (16 * (k - 4));
+        }
         oz_i128_t v;
-        v.hi = (t3 << 32) | (long long)(t2 & 0xFFFFFFFF);
-        v.lo = ((unsigned long long)(t1 & 0xFFFFFFFF) << 32) | (unsigned long long)(t0 & 0xFFFFFFFF);
+        v.hi = (long long)hi; /* |x_raw - q M| < M < 2^126: bits 126, 127 are the sign */
+        v.lo = lo;
         { /* into [-M/2, M/2) */
           const oz_i128_t up = oz_sub(v, hm_hi, hm_lo); /* v - M/2 >= 0: v -= M */
           const oz_i128_t dn = oz_add(v, hm_hi, hm_lo); /* v + M/2 < 0: v += M */
@@ -654,18 +705,18 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
           else if (0 > dn.hi) v = oz_add(v, m_hi, m_lo);
         }
         const bool neg = (0 > v.hi);
-        unsigned long long hi = (unsigned long long)v.hi, lo = v.lo;
+        unsigned long long uh = (unsigned long long)v.hi, ul = v.lo;
         if (neg) { /* |x| */
-          lo = ~lo + 1;
-          hi = ~hi + (0 == lo ? 1 : 0);
+          ul = ~ul + 1;
+          uh = ~uh + (0 == ul ? 1 : 0);
         }
         double d; /* |x| rounded to FP64: the top 64 bits with a sticky bit below, converted with one rounding */
         {
-          unsigned long long top = lo;
+          unsigned long long top = ul;
           int scale = 0;
-          if (0 != hi) {
-            const int lz = __clzll((long long)hi);
-            top = (0 != lz ? (hi << lz) | (lo >> (64 - lz)) : hi) | ((0 != (lo << lz)) ? 1ull : 0ull);
+          if (0 != uh) {
+            const int lz = __clzll((long long)uh);
+            top = (0 != lz ? (uh << lz) | (ul >> (64 - lz)) : uh) | ((0 != (ul << lz)) ? 1ull : 0ull);
             scale = 64 - lz;
           }
           d = ldexp(fma((double)(unsigned int)(top >> 32), 4294967296.0, (double)(unsigned int)top), scale);
@@ -911,43 +962,20 @@ extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks,
 /* zb: first item per prime, np + 1 entries; C = alpha 2^(ea + eb + sh0) x + beta C */
 extern "C" int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, int Mp, int Np,
   const int* ea, const int* eb, int M, int N, int sh0, double alpha, double beta, void* c, int ldc, void* hip_stream) {
-  constexpr int mod[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
   oz_levels_t lv;
-  oz_crt_consts_t cc;
-  unsigned __int128 m = 1, hm;
-  const long blocks = ((long)(Mp / 4) * N + 255) / 256;
-  const unsigned int grid = (unsigned int)(blocks < 4096 ? blocks : 4096);
+  const long blocks = ((long)(Mp / 4) * N + 255) / 256; /* one thread per 4 elements */
+  const unsigned int grid = (unsigned int)blocks;
   if (1 > np || LXS_OZ_NPRIMES_MAX < np || Mp < M || Np < N || 0 != Mp % 4) return LXS_ELIMIT;
+  if (4294967295.0 - 256.0 < (double)(Mp / 4) * N) return LXS_ELIMIT; /* 32-bit element index */
   if (0 >= M || 0 >= N) return 0;
   for (int p = 0; p < 17; ++p) lv.zb[p] = (p <= np ? zb[p] : zb[np]);
-  memset(&cc, 0, sizeof(cc));
-  for (int p = 0; p < np; ++p) m *= (unsigned __int128)mod[p];
-  hm = m >> 1;
-  for (int p = 0; p < np; ++p) {
-    const unsigned __int128 mp = m / (unsigned __int128)mod[p];
-    const int r = (int)(mp % (unsigned __int128)mod[p]);
-    int inv = 0;
-    for (int t = 1; t < mod[p]; ++t) {
-      if (1 == (r * t) % mod[p]) {
-        inv = t;
-        break;
-      }
-    }
-    cc.w[p] = (unsigned int)inv;
-    for (int k = 0; k < 4; ++k) cc.lm[p][k] = (unsigned int)(mp >> (32 * k));
-    cc.rm[p] = 1.0 / mod[p];
-  }
-  for (int k = 0; k < 4; ++k) {
-    cc.mm[k] = (unsigned int)(m >> (32 * k));
-    cc.hm[k] = (unsigned int)(hm >> (32 * k));
-  }
   if (3 == dtype) {
-    oz_crt_recombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, cc, Mp, Np, ea, eb,
-      M, N, sh0, alpha, beta, (double*)c, ldc);
+    oz_crt_recombine<double><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, Mp, Np, ea, eb, M, N,
+      sh0, alpha, beta, (double*)c, ldc);
   }
   else if (1 == dtype) {
-    oz_crt_recombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, cc, Mp, Np, ea, eb,
-      M, N, sh0, alpha, beta, (float*)c, ldc);
+    oz_crt_recombine<float><<<grid, 256, 0, (hipStream_t)hip_stream>>>((const uint8_t*)r8, lv, np, Mp, Np, ea, eb, M, N,
+      sh0, alpha, beta, (float*)c, ldc);
   }
   else {
     return LXS_ENOVARIANT;
