@@ -53,6 +53,8 @@ def parse():
     p.add_argument("--ozaki-sizes", default="1024,4096,8192", help="square DGEMM sizes for the Ozaki section")
     p.add_argument("--ozaki-only", action="store_true", help="print only the Ozaki section (no stack measurement)")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-predictor evaluation")
+    p.add_argument("--sweep", action="store_true",
+                   help="print only the plain-stack sweep: 7 shapes x FP64/FP32 x S in {30k, 100k, 500k}")
     p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
 
@@ -104,6 +106,38 @@ def timed(lx, res, stream, steps, warmup):
     prof = lx.profile_get("smm")
     lx.profile_enable(False)
     return (t1 - t0) * 1e3 / steps, prof["ms"] / max(prof["count"], 1)
+
+
+SWEEP_SHAPES = ((5, 5, 5), (13, 13, 13), (23, 23, 23), (32, 32, 32), (64, 64, 64), (4, 9, 22), (45, 45, 13))
+
+
+def sweep_section(lx, stream, steps=10, warmup=2):
+    """Plain stacks over the sweep shapes, FP64 and FP32, S = 30k, 100k, 500k (the bench's generator: pools
+    proportional to S, nc = S/30, na = nb = 2 nc, runs of 30 +- 4): kernel time from the launch records,
+    FLOP/s and algorithmic GB/s (S ts (mk + kn + 2mn), samples/smm/smm_kernel.c:504-506) against 8 TB/s."""
+    from libxstream_amd.workload import make_stack_workload
+
+    rows = []
+    for dtype in (np.float64, np.float32):
+        for (m, n, k) in SWEEP_SHAPES:
+            for S in (30000, 100000, 500000):
+                w = make_stack_workload(m, n, k, S, dtype=dtype, seed=25071975)
+                res = Resident(lx, w, stream)
+                _, kernel_ms = timed(lx, res, stream, steps, warmup)
+                cfg, src_ = lx.config_get(lx.typeid(dtype), m, n, k)
+                res.free()
+                gbs = w.bytes_alg() / (kernel_ms * 1e-3) * 1e-9
+                rows.append({"dtype": "f64" if dtype == np.float64 else "f32", "m": m, "n": n, "k": k,
+                             "stack_size": S, "kernel_ms": round(kernel_ms, 5),
+                             "gflops": round(w.flops() / (kernel_ms * 1e-3) * 1e-9, 1),
+                             "alg_gbs": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 4),
+                             "pool_mb": round((w.a.nbytes + w.b.nbytes + w.c.nbytes) / 1e6, 1),
+                             "variant": cfg.as_dict().get("variant"), "config_source":
+                             ["default", "tuned", "env", "learned"][src_]})
+                print("sweep %s %dx%dx%d S=%d: %.4f ms, %.1f GB/s" % (rows[-1]["dtype"], m, n, k, S, kernel_ms, gbs),
+                      file=sys.stderr, flush=True)
+                del w, res
+    return rows
 
 
 def batched(lx, stream, steps, warmup):
@@ -521,6 +555,16 @@ def main():
         print("bench.py: this build measures one GPU; --gpus %d is not supported (the stack path shards by C-run "
               "ranges with no exchange between GPUs)" % args.gpus, file=sys.stderr)
         sys.exit(2)
+    if args.sweep:
+        import libxstream_amd as lx
+
+        lx.init()
+        stream = lx.Stream("bench")
+        rows = sweep_section(lx, stream)
+        stream.close()
+        lx.finalize()
+        print(json.dumps({"sweep": rows}))
+        return
     if args.ozaki_only:
         import libxstream_amd as lx
 
