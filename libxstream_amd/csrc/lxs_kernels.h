@@ -30,6 +30,8 @@ typedef struct lxs_smm_args_t {
 } lxs_smm_args_t;
 
 /** number of LDS bytes the stack kernel needs for this configuration (0 if invalid) */
+/* 1 when the configuration can be launched (LDS fits; wavefront-per-run kernels need none) */
+int lxs_smm_launchable(const lxs_smm_args_t* args);
 size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args);
 /** 1 if a kernel is compiled for (dtype, variant, tm, tn, nt, kt) and shape, 2 if specialised for
  *  (m, n, k), else 0 */

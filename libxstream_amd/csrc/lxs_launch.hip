@@ -118,12 +118,26 @@ static size_t smm_lds(const lxs_smm_args_t* a, int kt) {
   return ts * (size_t)groups * (slab > red ? slab : red);
 }
 
+/* 1 when the vector-kernel configuration can be launched (its LDS fits; none for wavefront-per-run kernels) */
+extern "C" int lxs_smm_launchable(const lxs_smm_args_t* args) {
+  const smm_entry_t* e;
+  size_t lds;
+  int kt;
+  if (1 == args->variant) return 1;
+  e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
+  if (NULL == e) return 0;
+  if (0 > e->kt) return 1;
+  kt = (0 < e->kt ? e->kt : (0 < args->kt ? (args->kt < args->k ? args->kt : args->k) : args->k));
+  lds = smm_lds(args, kt);
+  return (0 < lds && lds <= lds_limit) ? 1 : 0;
+}
+
 extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
   const smm_entry_t* e;
   int kt;
   if (1 == args->variant) return 0;
   e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
-  if (NULL == e) return 0;
+  if (NULL == e || 0 > e->kt) return 0; /* no LDS for the wavefront-per-run kernels */
   kt = (0 < e->kt ? e->kt : (0 < args->kt ? (args->kt < args->k ? args->kt : args->k) : args->k));
   return smm_lds(args, kt);
 }
@@ -270,8 +284,8 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
     if (NULL == e) return LXS_ENOVARIANT;
     fn = e->fn;
     kt = (0 < e->kt ? e->kt : (0 < a->kt ? (a->kt < a->k ? a->kt : a->k) : a->k));
-    lds = smm_lds(a, kt);
-    if (0 == lds || lds > lds_limit) return LXS_ELIMIT;
+    lds = (0 > e->kt ? 0 : smm_lds(a, kt)); /* kt < 0: wavefront-per-run kernel without LDS */
+    if (0 <= e->kt && (0 == lds || lds > lds_limit)) return LXS_ELIMIT;
   }
   if (0 >= stack_size) return 0;
   bs = (0 < a->bs ? a->bs : 32);

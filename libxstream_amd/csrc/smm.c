@@ -89,7 +89,6 @@ static lxs_smm_slot_t* lxs_smm_lookup(int type, int m, int n, int k, int insert)
 LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, int n, int k,
   const libsmm_acc_smm_config_t* config) {
   lxs_smm_args_t args;
-  size_t lds;
   if (NULL == config || 0 >= m || 0 >= n || 0 >= k || 0 == lxs_typesize(datatype)) return EXIT_FAILURE;
   if ((0 != config->variant && 1 != config->variant) || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt ||
       0 > config->bs || 0 > config->kt) {
@@ -110,8 +109,7 @@ LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, in
   args.tn = config->tn;
   args.nt = config->nt;
   args.kt = config->kt;
-  lds = lxs_smm_lds_bytes(&args);
-  return (0 < lds && lds <= lxs_lds_limit()) ? EXIT_SUCCESS : EXIT_FAILURE;
+  return (0 != lxs_smm_launchable(&args)) ? EXIT_SUCCESS : EXIT_FAILURE;
 }
 
 /* largest k-slice that keeps the staged blocks of all groups within the LDS limit */

@@ -33,7 +33,7 @@ typedef LXS_REAL real_t;
 
 struct smm_entry_t {
   int tm, tn, nt, m, n, k; /* m = n = k = 0: runtime shape */
-  int kt;                  /* compile-time k-slice (0: runtime) */
+  int kt;                  /* compile-time k-slice (0: runtime, -1: wavefront-per-run kernel without LDS) */
   const void* fn;
 };
 
@@ -233,6 +233,74 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
   }
 }
 
+/*
+ * Tiny blocks (m n <= 64), compile-time shape: one wavefront per C-run and no LDS or barriers. Lane l is
+ * entry slot l / (m n) and C element l % (m n) (column-major i, j); the P = 64 / (m n) slots take the
+ * run's entries round-robin, each lane forming its element's dot products straight from the (cached) A
+ * and B blocks, an entry's k-sum from zero and then added to the lane's run total. The slots are summed in
+ * slot order by lane shuffles at the run end and the C block is updated once (or atomically). Wavefront w
+ * of a work-group owns the runs that start in its part of the work-group's bs entries.
+ */
+template <typename T, int NT, int MC, int NC, int KC>
+__global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* __restrict__ adata,
+  const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
+  int kt_rt, int atomic) {
+  constexpr int MN = MC * NC, P = 64 / MN, W = NT / 64;
+  static_assert(0 < MN && MN <= 64 && 0 == NT % 64, "one C block per wavefront slot");
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int slot = lane / MN, el = lane - slot * MN;
+  const bool live = (slot < P);
+  const int i = el % MC, j = el / MC;
+  const int bsw = (bs + W - 1) / W;
+  const int wbeg = blockIdx.x * bs + wave * bsw;
+  const int wend = min(min(wbeg + bsw, (int)(blockIdx.x + 1) * bs), stack_size);
+  (void)m_rt, (void)n_rt, (void)k_rt, (void)kt_rt;
+  if (wbeg >= wend) return;
+  int h = wbeg; /* the first run that starts in [wbeg, wend), 64 entries per step */
+  for (;;) {
+    const int x = h + lane;
+    const bool start = (x < wend) && (0 == x || stack[3 * x + 2] != stack[3 * x - 1]);
+    const unsigned long long bal = __ballot(start);
+    if (0 != bal) {
+      h += __ffsll((long long)bal) - 1;
+      break;
+    }
+    h += 64;
+    if (h >= wend) return;
+  }
+  while (h < wend) {
+    const int c0 = stack[3 * h + 2];
+    int r_end = h + 1; /* the run's end, 64 entries per step */
+    for (;;) {
+      const int x = r_end + lane;
+      const unsigned long long bal = __ballot(x >= stack_size || stack[3 * x + 2] != c0);
+      if (0 != bal) {
+        r_end += __ffsll((long long)bal) - 1;
+        break;
+      }
+      r_end += 64;
+    }
+    T acc = T(0);
+    if (live) {
+      for (int e = h + slot; e < r_end; e += P) {
+        const T* const ablk = adata + (stack[3 * e] - 1);
+        const T* const bblk = bdata + (stack[3 * e + 1] - 1);
+        T prod = T(0);
+#pragma unroll
+        for (int kk = 0; kk < KC; ++kk) prod = fma(ablk[i + kk * MC], bblk[kk * NC + j], prod);
+        acc += prod;
+      }
+    }
+    T tot = acc;
+#pragma unroll
+    for (int s = 1; s < P; ++s) tot += __shfl(acc, (lane + s * MN) & 63);
+    if (0 == slot) flush_one(cdata + (c0 - 1) + i + j * MC, tot, atomic);
+    h = r_end;
+  }
+}
+
+/* kt = -1 marks a wavefront-per-run kernel: launched without LDS */
+#define LXS_LANES(M, N, K, TM, TN, NT) {TM, TN, NT, M, N, K, -1, (const void*)&smm_lanes<real_t, NT, M, N, K>},
 #define LXS_GENERIC(TM, TN, NT) {TM, TN, NT, 0, 0, 0, 0, (const void*)&smm_stack<real_t, TM, TN, NT, 0, 0, 0, 0>},
 #define LXS_SPECIAL(M, N, K, TM, TN, NT, KT) {TM, TN, NT, M, N, K, KT, (const void*)&smm_stack<real_t, TM, TN, NT, M, N, K, KT>},
 
@@ -244,7 +312,7 @@ const smm_entry_t smm_table[] = {
   LXS_GENERIC(4, 4, 64) LXS_GENERIC(4, 4, 128) LXS_GENERIC(4, 4, 256) LXS_GENERIC(4, 4, 512)
   LXS_GENERIC(2, 4, 128) LXS_GENERIC(4, 2, 128) LXS_GENERIC(5, 5, 256)
   /* compile-time shapes of the stack workloads */
-  LXS_SPECIAL(5, 5, 5, 1, 1, 64, 5) LXS_SPECIAL(5, 5, 5, 1, 1, 256, 5)
+  LXS_LANES(5, 5, 5, 1, 1, 64) LXS_LANES(5, 5, 5, 1, 1, 256)
   LXS_SPECIAL(13, 13, 13, 2, 2, 64, 13) LXS_SPECIAL(13, 13, 13, 2, 2, 256, 13) LXS_SPECIAL(13, 13, 13, 3, 3, 64, 13)
   LXS_SPECIAL(23, 23, 23, 3, 3, 64, 23) LXS_SPECIAL(23, 23, 23, 3, 3, 128, 23) LXS_SPECIAL(23, 23, 23, 3, 3, 256, 23)
   LXS_SPECIAL(23, 23, 23, 4, 4, 64, 23) LXS_SPECIAL(23, 23, 23, 2, 2, 256, 23)
