@@ -64,6 +64,38 @@ LIBXSTREAM_API int c_calculate_norms(const double* mat, int nblks, const int* of
 LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t def_mnk, libsmm_acc_data_t datatype,
   int stack_size, int m_max, int n_max, int k_max, int max_kernel_dim);
 
+/**
+ * DBM reuse of the stack kernels (samples/smm/smm_acc_opencl.h:29-31,114-124; smm_acc.c:579-585;
+ * smm_kernel.c:73-76). Bits per extent when libsmm_acc_process packs a shape into param_format.
+ */
+#define LIBSMM_ACC_PFORMAT 8
+
+/** samples/smm/smm_acc_opencl.h:115-116: CP2K's DBM launcher (shard_c_data += pack_a * pack_b per task) */
+typedef int (*opencl_libsmm_acc_dbm_launch_fn_t)(void* stream, double alpha, int ntasks, int param_format,
+  const int* params_host, const int* params, const double* pack_a_data, const double* pack_b_data,
+  double* shard_c_data);
+
+/**
+ * samples/smm/smm_acc_opencl.h:117-118 (smm_kernel.c:74-76). Once a function is registered (NULL
+ * unregisters), libsmm_acc_process hands every real_8 stack with def_mnk != 0 and m, n, k < 128 to it,
+ * with alpha 1, param_format = m | n << 8 | k << 16, and returns its result (smm_acc.c:579-585).
+ */
+LIBXSTREAM_API void opencl_libsmm_acc_set_dbm_launch_fn(opencl_libsmm_acc_dbm_launch_fn_t launch_fn);
+
+/**
+ * samples/smm/smm_acc_opencl.h:120-123 (smm_kernel.c:82-528): libsmm_acc_process over records of
+ * either format, so DBM tasks run on the same kernels.
+ *   - param_format == 0: records of 3 ints (a, b, c), 1-based element offsets (as libsmm_acc_process).
+ *   - param_format != 0: DBM task records of 6 ints (m, n, k, a, b, c), 0-based element offsets; m, n, k
+ *     of a record are not read: every task has the shape m_max x n_max x k_max (multiply.cl:96-99).
+ * The result convention is libsmm_acc_process's (-1: host fallback). event: NULL, or an event from
+ * c_dbcsr_acc_event_create, recorded on stream after the launch (in place of the reference's cl_event).
+ */
+LIBXSTREAM_API int opencl_libsmm_acc_process(const int* host_param_stack, const int* dev_param_stack, int stack_size,
+  libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m_max, int n_max,
+  int k_max, int max_kernel_dim, c_dbcsr_acc_bool_t def_mnk, void* stream, void* c_stream, int param_format,
+  void* event);
+
 /** DBCSR timer names (samples/smm/acc_libsmm.h:51-57) */
 static const char libsmm_acc_transpose_routine_name_str[] = "jit_kernel_transpose";
 static const char libsmm_acc_process_routine_name_str[] = "jit_kernel_multiply";

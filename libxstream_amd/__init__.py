@@ -71,6 +71,12 @@ class OzakiContext(ctypes.Structure):
         ("impl", ctypes.c_void_p)]
 
 
+# samples/smm/smm_acc_opencl.h:115-116: (stream, alpha, ntasks, param_format, params_host, params, a, b, c)
+DBM_LAUNCH_FN = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                 ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+PFORMAT = 8  # bits per extent of a shape packed into param_format
+
+
 def _declare(L):
     vp, i, sz = ctypes.c_void_p, ctypes.c_int, ctypes.c_size_t
     pvp, pi, psz = ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_size_t)
@@ -91,6 +97,8 @@ def _declare(L):
         "libxstream_event_sync": (i, [vp]),
         "libxstream_event_elapsed": (i, [vp, vp, ctypes.POINTER(ctypes.c_double)]),
         "libxstream_mem_allocate": (i, [pvp, sz]),
+        "libxstream_mem_host_allocate": (i, [pvp, sz, vp]),
+        "libxstream_mem_host_deallocate": (i, [vp, vp]),
         "libxstream_mem_deallocate": (i, [vp]),
         "libxstream_mem_copy_h2d": (i, [vp, vp, sz, vp]),
         "libxstream_mem_copy_d2h": (i, [vp, vp, sz, vp]),
@@ -106,6 +114,8 @@ def _declare(L):
         "libsmm_acc_is_thread_safe": (i, []),
         "libsmm_acc_process_suitable": (i, [i, i, i, i, i, i, i]),
         "libsmm_acc_process": (i, [vp, vp, i, i, vp, vp, vp, i, i, i, i, i, vp, vp]),
+        "opencl_libsmm_acc_process": (i, [vp, vp, i, i, vp, vp, vp, i, i, i, i, i, vp, vp, i, vp]),
+        "opencl_libsmm_acc_set_dbm_launch_fn": (None, [DBM_LAUNCH_FN]),
         "libsmm_acc_transpose": (i, [vp, i, i, vp, i, i, i, i, vp]),
         "libsmm_acc_process_config": (i, [ctypes.POINTER(SmmConfig), vp, i, i, vp, vp, vp, i, i, i, vp]),
         "libsmm_acc_config_get": (i, [i, i, i, i, ctypes.POINTER(SmmConfig), pi]),
@@ -297,6 +307,27 @@ class DeviceBuffer:
             self.ptr = None
 
 
+class HostBuffer:
+    """Pinned host memory (libxstream_mem_host_allocate) seen as a numpy array of dtype and count."""
+
+    def __init__(self, dtype, count, stream=None):
+        self.stream = stream
+        self.nbytes = int(np.dtype(dtype).itemsize * count)
+        p = ctypes.c_void_p()
+        check(lib().libxstream_mem_host_allocate(ctypes.byref(p), max(self.nbytes, 1),
+                                                 stream.handle if stream else None), "mem_host_allocate")
+        self.ptr = p.value
+        self.array = np.ctypeslib.as_array((ctypes.c_char * max(self.nbytes, 1)).from_address(self.ptr))
+        self.array = self.array.view(np.uint8)[:self.nbytes].view(dtype)
+
+    def free(self):
+        if self.ptr:
+            self.array = None
+            check(lib().libxstream_mem_host_deallocate(self.ptr, self.stream.handle if self.stream else None),
+                  "mem_host_deallocate")
+            self.ptr = None
+
+
 def process(stack_dev, stack_size, tid, a_dev, b_dev, c_dev, m, n, k, stream=None,
             max_kernel_dim=MAX_KERNEL_DIM, def_mnk=1, host_stack=None):
     """libsmm_acc_process; returns its code (0, or -1 for host fallback); raises on other errors."""
@@ -307,6 +338,30 @@ def process(stack_dev, stack_size, tid, a_dev, b_dev, c_dev, m, n, k, stream=Non
     if code not in (0, -1):
         check(code, "libsmm_acc_process")
     return code
+
+
+def pack_format(m, n, k):
+    """The param_format libsmm_acc_process hands a DBM launch function (smm_acc.c:583)."""
+    return m | (n << PFORMAT) | (k << (2 * PFORMAT))
+
+
+def process_format(stack_dev, stack_size, tid, a_dev, b_dev, c_dev, m, n, k, param_format, stream=None,
+                   event=None, max_kernel_dim=MAX_KERNEL_DIM, def_mnk=1):
+    """opencl_libsmm_acc_process: param_format 0 reads (a, b, c) 1-based triplets, any other value DBM task
+    records (m, n, k, a, b, c) with 0-based offsets; returns 0, or -1 for host fallback."""
+    code = lib().opencl_libsmm_acc_process(None, stack_dev.ptr if stack_dev else None, stack_size, tid,
+                                           a_dev.ptr if a_dev else None, b_dev.ptr if b_dev else None,
+                                           c_dev.ptr if c_dev else None, m, n, k, max_kernel_dim, def_mnk,
+                                           stream.handle if stream else None, None, param_format,
+                                           event.handle if event else None)
+    if code not in (0, -1):
+        check(code, "opencl_libsmm_acc_process")
+    return code
+
+
+def set_dbm_launch_fn(fn):
+    """Register a DBM_LAUNCH_FN (keep a reference to it while registered), or None to unregister."""
+    lib().opencl_libsmm_acc_set_dbm_launch_fn(fn if fn is not None else DBM_LAUNCH_FN())
 
 
 def process_config(config, stack_dev, stack_size, tid, a_dev, b_dev, c_dev, m, n, k, stream=None):

@@ -20,13 +20,15 @@ typedef struct lxs_smm_args_t {
   void* c;
   const void* a;
   const void* b;
-  const int* stack;  /* (a,b,c) triplets, 1-based element offsets */
+  const int* stack;  /* records of the (a,b,c) element offsets, see pfmt */
   int stack_size;
   int m, n, k;
   int dtype;
   int variant;       /* 0: vector-FMA kernel (LDS staging), 1: MFMA kernel (registers only) */
   int tm, tn, nt, bs, kt;
   int atomic;        /* 1: flush C-runs with atomic adds; 0: plain read-modify-write */
+  int pfmt;          /* 0: 3-int records, 1-based offsets; else 6-int DBM task records (m, n, k, a, b, c),
+                        0-based offsets (samples/smm/kernels/multiply.cl:96-99) */
 } lxs_smm_args_t;
 
 /** number of LDS bytes the stack kernel needs for this configuration (0 if invalid) */

@@ -574,7 +574,7 @@ LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t
 }
 
 static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,
-  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream) {
+  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream, int param_format) {
   const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
   lxs_smm_args_t args;
   void* token;
@@ -599,6 +599,7 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   /* flush: 1 = read-modify-write (caller guarantees each C block is one run), else atomic add,
      which is exact-once per element when the stack is grouped by C and correct otherwise */
   args.atomic = (1 == config->flush ? 0 : 1);
+  args.pfmt = param_format;
   token = lxs_profile_begin(hs);
   result = lxs_smm_launch(&args, (void*)hs);
   lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,
@@ -618,17 +619,23 @@ LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* conf
     return lxs_error(EXIT_FAILURE, "libsmm_acc_process_config: configuration not valid for this shape");
   }
   result = lxs_smm_run(config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data, m, n, k,
-    stream);
+    stream, 0);
   return (0 <= result ? lxs_hip((hipError_t)result, "stack kernel launch") : lxs_error(result, "stack kernel launch"));
 }
 
-LIBXSTREAM_API int libsmm_acc_process(const int* host_param_stack, const int* dev_param_stack, int stack_size,
-  libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m_max, int n_max,
-  int k_max, int max_kernel_dim, c_dbcsr_acc_bool_t def_mnk, void* stack_stream, void* c_stream) {
+/* the DBM launch function registered by opencl_libsmm_acc_set_dbm_launch_fn (NULL: none) */
+static opencl_libsmm_acc_dbm_launch_fn_t lxs_dbm_launch_fn;
+
+LIBXSTREAM_API void opencl_libsmm_acc_set_dbm_launch_fn(opencl_libsmm_acc_dbm_launch_fn_t launch_fn) {
+  __atomic_store_n(&lxs_dbm_launch_fn, launch_fn, __ATOMIC_RELEASE);
+}
+
+/* libsmm_acc_process over records of either format (param_format 0: DBCSR triplets, else DBM tasks) */
+static int lxs_smm_process(const int* dev_param_stack, int stack_size, libsmm_acc_data_t datatype,
+  const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m_max, int n_max, int k_max, int max_kernel_dim,
+  c_dbcsr_acc_bool_t def_mnk, void* stack_stream, int param_format) {
   libsmm_acc_smm_config_t config;
   int result;
-  (void)host_param_stack;
-  (void)c_stream;
   if (0 == libsmm_acc_process_suitable(def_mnk, datatype, stack_size, m_max, n_max, k_max, max_kernel_dim)) {
     return (0 < stack_size ? -1 : EXIT_SUCCESS); /* smm_kernel.c:524-526 */
   }
@@ -638,14 +645,48 @@ LIBXSTREAM_API int libsmm_acc_process(const int* host_param_stack, const int* de
   lxs_smm_resolve((int)datatype, m_max, n_max, k_max, &config, NULL);
   if (0 >= config.nt) return -1; /* no compiled kernel covers this shape: host fallback */
   result = lxs_smm_run(&config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data, m_max,
-    n_max, k_max, stack_stream);
+    n_max, k_max, stack_stream, param_format);
   if (LXS_ENOVARIANT == result || LXS_ELIMIT == result) {
     if (EXIT_SUCCESS != lxs_smm_default((int)datatype, m_max, n_max, k_max, &config)) return -1;
     result = lxs_smm_run(&config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data,
-      m_max, n_max, k_max, stack_stream);
+      m_max, n_max, k_max, stack_stream, param_format);
     if (0 > result) return -1;
   }
   return lxs_hip((hipError_t)result, "stack kernel launch");
+}
+
+LIBXSTREAM_API int libsmm_acc_process(const int* host_param_stack, const int* dev_param_stack, int stack_size,
+  libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m_max, int n_max,
+  int k_max, int max_kernel_dim, c_dbcsr_acc_bool_t def_mnk, void* stack_stream, void* c_stream) {
+  const opencl_libsmm_acc_dbm_launch_fn_t dbm = __atomic_load_n(&lxs_dbm_launch_fn, __ATOMIC_ACQUIRE);
+  (void)c_stream;
+  /* smm_acc.c:579-585: FP64 stacks of one shape go to a registered DBM launch function, the shape packed
+     into its param_format argument (8 bits per extent) */
+  if (dbcsr_type_real_8 == datatype && 0 != def_mnk && NULL != dbm && 0 <= m_max && 0 <= n_max && 0 <= k_max &&
+      m_max < (1 << (LIBSMM_ACC_PFORMAT - 1)) && n_max < (1 << (LIBSMM_ACC_PFORMAT - 1)) &&
+      k_max < (1 << (LIBSMM_ACC_PFORMAT - 1)))
+  {
+    return dbm(stack_stream, 1.0, stack_size,
+      m_max | (n_max << LIBSMM_ACC_PFORMAT) | (k_max << (2 * LIBSMM_ACC_PFORMAT)), host_param_stack, dev_param_stack,
+      (const double*)dev_a_data, (const double*)dev_b_data, (double*)dev_c_data);
+  }
+  return lxs_smm_process(dev_param_stack, stack_size, datatype, dev_a_data, dev_b_data, dev_c_data, m_max, n_max, k_max,
+    max_kernel_dim, def_mnk, stack_stream, 0);
+}
+
+LIBXSTREAM_API int opencl_libsmm_acc_process(const int* host_param_stack, const int* dev_param_stack, int stack_size,
+  libsmm_acc_data_t datatype, const void* dev_a_data, const void* dev_b_data, void* dev_c_data, int m_max, int n_max,
+  int k_max, int max_kernel_dim, c_dbcsr_acc_bool_t def_mnk, void* stream, void* c_stream, int param_format,
+  void* event) {
+  int result;
+  (void)host_param_stack;
+  (void)c_stream;
+  result = lxs_smm_process(dev_param_stack, stack_size, datatype, dev_a_data, dev_b_data, dev_c_data, m_max, n_max,
+    k_max, max_kernel_dim, def_mnk, stream, param_format);
+  if (EXIT_SUCCESS == result && NULL != event) { /* completes when the launch has */
+    result = libxstream_event_record((libxstream_event_t*)event, (libxstream_stream_t*)stream);
+  }
+  return result;
 }
 
 LIBXSTREAM_API int libsmm_acc_transpose(const int* dev_trs_stack, int offset, int stack_size, void* dev_data,

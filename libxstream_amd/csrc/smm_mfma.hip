@@ -3,7 +3,9 @@
  * (v_mfma_f64_16x16x4f64 for FP64, v_mfma_f32_16x16x4f32 for FP32).
  *
  * Same semantics as smm_stack.hip (samples/smm/kernels/multiply.cl:12-15: A m-by-k column-major,
- * B k-by-n row-major, C m-by-n column-major, 1-based element offsets); a different design:
+ * B k-by-n row-major, C m-by-n column-major; a stack record is ps ints whose first three are the a, b, c
+ * element offsets with index base pz: ps = 3, pz = 1 for DBCSR stacks, ps = 6, pz = 0 for DBM task
+ * records seen from their fourth int, multiply.cl:96-99); a different design:
  *   - a wavefront covers the C block with 16x16 output tiles (TI x TJ tiles, padded); a
  *     work-group of NWI wavefronts splits the tile rows, each wavefront walking the same entries
  *     independently (no LDS, no barriers);
@@ -47,7 +49,7 @@ using lxs::mfma;
 template <typename T, int TI, int TJ, int NWI, int KB, int MC, int NC, int KC>
 __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
-  int kt_unused, int atomic) {
+  int kt_unused, int atomic, int ps, int pz) {
   typedef typename mfma<T>::acc_t acc_t;
   constexpr int RI = TI / NWI; /* tile rows per wavefront */
   /* summation order: the matrix core accumulates up to SEG_LEN entries in prod, which is then
@@ -69,7 +71,7 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
   for (int base = begin; base < end; base += 64) {
     const int i = base + lane;
     bool head = false;
-    if (i < end) head = (0 == i) || (stack[3 * i + 2] != stack[3 * i - 1]);
+    if (i < end) head = (0 == i) || (stack[ps * i + 2] != stack[ps * (i - 1) + 2]);
     const unsigned long long mask = __ballot(head);
     if (0 != mask) {
       h = base + __ffsll((long long)mask) - 1;
@@ -112,12 +114,12 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
 
   /* stack indices: current entry e, next entry (1) and the one after (2), as scalar loads */
   int e = h;
-  int a0 = stack[3 * e + 0] - 1, b0 = stack[3 * e + 1] - 1, c0 = stack[3 * e + 2];
+  int a0 = stack[ps * e + 0] - pz, b0 = stack[ps * e + 1] - pz, c0 = stack[ps * e + 2];
   bool have1 = (e + 1 < stack_size), have2 = (e + 2 < stack_size);
-  int a1 = (have1 ? stack[3 * e + 3] - 1 : a0), b1 = (have1 ? stack[3 * e + 4] - 1 : b0);
-  int c1 = (have1 ? stack[3 * e + 5] : -1);
-  int a2 = (have2 ? stack[3 * e + 6] - 1 : a0), b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
-  int c2 = (have2 ? stack[3 * e + 8] : -1);
+  int a1 = (have1 ? stack[ps * (e + 1) + 0] - pz : a0), b1 = (have1 ? stack[ps * (e + 1) + 1] - pz : b0);
+  int c1 = (have1 ? stack[ps * (e + 1) + 2] : -1);
+  int a2 = (have2 ? stack[ps * (e + 2) + 0] - pz : a0), b2 = (have2 ? stack[ps * (e + 2) + 1] - pz : b0);
+  int c2 = (have2 ? stack[ps * (e + 2) + 2] : -1);
   int s = 0;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
@@ -172,13 +174,13 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
       have1 = have2;
       s = 0;
       have2 = (e + 2 < stack_size);
-      a2 = (have2 ? stack[3 * e + 6] - 1 : a0);
-      b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
-      c2 = (have2 ? stack[3 * e + 8] : -1);
+      a2 = (have2 ? stack[ps * (e + 2) + 0] - pz : a0);
+      b2 = (have2 ? stack[ps * (e + 2) + 1] - pz : b0);
+      c2 = (have2 ? stack[ps * (e + 2) + 2] : -1);
     }
     /* run done: fold and flush C once (read-modify-write, or atomic add for unsorted stacks) */
     {
-      T* const cblk = cdata + (c0 - 1);
+      T* const cblk = cdata + (c0 - pz);
 #pragma unroll
       for (int ri = 0; ri < RI; ++ri) {
 #pragma unroll
@@ -213,9 +215,9 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
     have1 = have2;
     s = 0;
     have2 = (e + 2 < stack_size);
-    a2 = (have2 ? stack[3 * e + 6] - 1 : a0);
-    b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
-    c2 = (have2 ? stack[3 * e + 8] : -1);
+    a2 = (have2 ? stack[ps * (e + 2) + 0] - pz : a0);
+    b2 = (have2 ? stack[ps * (e + 2) + 1] - pz : b0);
+    c2 = (have2 ? stack[ps * (e + 2) + 2] : -1);
   }
 #undef LXS_LOAD_KSTEP
 }
