@@ -108,6 +108,61 @@ def timed(lx, res, stream, steps, warmup):
     return (t1 - t0) * 1e3 / steps, prof["ms"] / max(prof["count"], 1)
 
 
+def copyin_section(lx, w, stream, reps=5):
+    """Copy-in of the A and B pools and the stack from pinned host memory (acc_bench.c:288-307): the
+    three host-to-device copies enqueued on one stream, timed over reps after one untimed round."""
+    names = ("a", "b", "stack")
+    arrays = (w.a, w.b, w.stack)
+    host = [lx.HostBuffer(x.dtype, x.size, stream) for x in arrays]
+    dev = [lx.DeviceBuffer(x.nbytes) for x in arrays]
+    try:
+        for h, x in zip(host, arrays):
+            h.array[...] = x.ravel()
+        L = lx.lib()
+
+        def copy_all():
+            for h, d, x in zip(host, dev, arrays):
+                check = L.libxstream_mem_copy_h2d(h.ptr, d.ptr, x.nbytes, stream.handle)
+                assert check == 0, "libxstream_mem_copy_h2d returned %d" % check
+
+        copy_all()
+        stream.sync()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            copy_all()
+        stream.sync()
+        sec = (time.perf_counter() - t0) / reps
+    finally:
+        for buf in host + dev:
+            buf.free()
+    size = float(sum(x.nbytes for x in arrays))
+    return {"arrays": list(names), "bytes": size, "ms": round(sec * 1e3, 4), "gbs": round(size / sec * 1e-9, 2),
+            "gibs": round(size / (sec * (1 << 30)), 2), "pinned": True}
+
+
+def transpose_section(lx, res, stream, pairs=10):
+    """libsmm_acc_transpose over the whole B pool (smm_trans.c:48-228): kernel time from the launch records
+    and HBM rate over the reference's bytes, 2 S m n ts per launch (smm_trans.c:208-212). Transposes go
+    in pairs, (k, n) then (n, k), so B is left in the kernel layout."""
+    w = res.w
+    for _ in range(2):
+        lx.transpose(res.trs, 0, w.nb, res.b, res.tid, w.k, w.n, stream)
+        lx.transpose(res.trs, 0, w.nb, res.b, res.tid, w.n, w.k, stream)
+    stream.sync()
+    lx.profile_enable(True)
+    lx.profile_reset()
+    for _ in range(pairs):
+        lx.transpose(res.trs, 0, w.nb, res.b, res.tid, w.k, w.n, stream)
+        lx.transpose(res.trs, 0, w.nb, res.b, res.tid, w.n, w.k, stream)
+    stream.sync()
+    prof = lx.profile_get("trans")
+    lx.profile_enable(False)
+    ms = prof["ms"] / max(prof["count"], 1)
+    nbytes = 2.0 * w.nb * w.k * w.n * w.a.itemsize
+    return {"blocks": w.nb, "shape": [w.k, w.n], "kernel_ms": round(ms, 5), "bytes_per_launch": nbytes,
+            "gbs": round(nbytes / (ms * 1e-3) * 1e-9, 1), "frac_of_8tbs": round(nbytes / (ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4)}
+
+
 SWEEP_SHAPES = ((5, 5, 5), (13, 13, 13), (23, 23, 23), (32, 32, 32), (64, 64, 64), (4, 9, 22), (45, 45, 13))
 
 
@@ -589,7 +644,9 @@ def main():
     res = Resident(lx, w, stream)
     cfg, src = lx.config_get(res.tid, m, n, k)
     ms_per_step, kernel_ms = timed(lx, res, stream, args.steps, args.warmup)
+    trans = transpose_section(lx, res, stream)
     res.free()
+    copyin = copyin_section(lx, w, stream)
     small = None
     if not args.no_small:  # the 30000-entry stack over 2000/2000/1000 blocks (the reference's stack size)
         ws = make_stack_workload(m, n, k, 30000, nc=1000, na=2000, nb=2000, dtype=w.dtype, seed=25071975)
@@ -655,6 +712,8 @@ def main():
         },
         "cpu_baseline": cpu,
         "stack_30000": small,
+        "transpose_b_pool": trans,
+        "copy_in_pinned": copyin,
         "batched_mixed": batch,
         "ozaki": ozaki,
         "learned": learned_rep,

@@ -21,25 +21,39 @@ struct mfma_entry_t {
 
 const size_t lds_limit = 65536;
 
-/* In-place transpose of listed blocks: column-major m x n -> row-major m x n. One work-group
- * handles `bs` consecutive blocks of the transpose stack, staging each block in LDS. */
+/* In-place transpose of listed blocks: column-major m x n -> row-major m x n. Wavefront w of a work-group
+ * transposes block blockIdx.x * W + w on its own (W = blockDim.x / 64 blocks in flight per work-group),
+ * staging it in its own m n elements of LDS; loads and stores go 4 per lane at a time. */
 template <typename T>
 __global__ void __launch_bounds__(256) trans_stack(const int* __restrict__ trs, int offset, int stack_size,
-  T* __restrict__ data, int m, int n, int bs) {
+  T* __restrict__ data, int m, int n) {
   extern __shared__ double lxs_tbuf[];
-  T* const buf = reinterpret_cast<T*>(lxs_tbuf);
   const int mn = m * n;
-  const int first = blockIdx.x * bs;
-  const int last = min(first + bs, stack_size);
-  for (int i = first; i < last; ++i) {
-    T* const mat = data + trs[offset + i];
-    for (int x = threadIdx.x; x < mn; x += blockDim.x) buf[x] = mat[x];
-    __syncthreads();
-    for (int y = threadIdx.x; y < mn; y += blockDim.x) {
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int i = blockIdx.x * (int)(blockDim.x >> 6) + wave;
+  T* const buf = reinterpret_cast<T*>(lxs_tbuf) + (size_t)wave * mn;
+  T* const mat = data + (i < stack_size ? trs[offset + i] : 0);
+  if (i < stack_size) {
+    for (int x0 = 0; x0 < mn; x0 += 256) {
+      T v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int x = x0 + 64 * u + lane;
+        v[u] = (x < mn ? mat[x] : T(0));
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int x = x0 + 64 * u + lane;
+        if (x < mn) buf[x] = v[u];
+      }
+    }
+  }
+  __syncthreads();
+  if (i < stack_size) {
+    for (int y = lane; y < mn; y += 64) {
       const int r = y / n, c = y - r * n; /* row-major position (r, c) */
       mat[y] = buf[c * m + r];
     }
-    __syncthreads();
   }
 }
 
@@ -264,14 +278,14 @@ extern "C" int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, 
 }
 
 extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
-  int kt, bs, atomic, stack_size = a->stack_size, m = a->m, n = a->n, k = a->k;
+  int kt, bs, atomic, ps, pz, stack_size = a->stack_size, m = a->m, n = a->n, k = a->k;
   size_t lds;
   unsigned int nblocks;
   const void* fn;
   void* c;
   const void *pa, *pb;
   const int* stack;
-  void* args[11];
+  void* args[13];
   if (1 == a->variant) {
     const mfma_entry_t* const e = mfma_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt, a->kt);
     if (NULL == e) return LXS_ENOVARIANT;
@@ -294,7 +308,10 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
   c = a->c;
   pa = a->a;
   pb = a->b;
-  stack = a->stack;
+  /* record stride and index base; a DBM task record (m, n, k, a, b, c) is read from its fourth int */
+  ps = (0 == a->pfmt ? 3 : 6);
+  pz = (0 == a->pfmt ? 1 : 0);
+  stack = a->stack + (0 == a->pfmt ? 0 : 3);
   args[0] = &c;
   args[1] = &pa;
   args[2] = &pb;
@@ -306,24 +323,27 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
   args[8] = &k;
   args[9] = &kt;
   args[10] = &atomic;
+  args[11] = &ps;
+  args[12] = &pz;
   return (int)hipLaunchKernel(fn, dim3(nblocks), dim3((unsigned int)a->nt), args, lds, (hipStream_t)hip_stream);
 }
 
 extern "C" int lxs_trans_launch(const int* trs_stack, int offset, int stack_size, void* data, int dtype, int m, int n,
   void* hip_stream) {
-  const int bs = 8;
   const size_t ts = (3 == dtype ? 8 : 4);
-  const size_t lds = ts * (size_t)m * (size_t)n;
-  const unsigned int nblocks = (unsigned int)((stack_size + bs - 1) / bs);
+  const size_t blk = ts * (size_t)m * (size_t)n; /* LDS bytes per block (one wavefront each) */
+  unsigned int waves, nblocks;
   if (0 >= stack_size) return 0;
-  if (lds > lds_limit) return LXS_ELIMIT;
+  if (blk > lds_limit) return LXS_ELIMIT;
+  waves = (unsigned int)(lds_limit / blk < 4 ? lds_limit / blk : 4);
+  nblocks = (unsigned int)((stack_size + (int)waves - 1) / (int)waves);
   if (3 == dtype) {
-    trans_stack<double><<<nblocks, 256, lds, (hipStream_t)hip_stream>>>(trs_stack, offset, stack_size, (double*)data,
-      m, n, bs);
+    trans_stack<double><<<nblocks, 64 * waves, waves * blk, (hipStream_t)hip_stream>>>(trs_stack, offset, stack_size,
+      (double*)data, m, n);
   }
   else if (1 == dtype) {
-    trans_stack<float><<<nblocks, 256, lds, (hipStream_t)hip_stream>>>(trs_stack, offset, stack_size, (float*)data,
-      m, n, bs);
+    trans_stack<float><<<nblocks, 64 * waves, waves * blk, (hipStream_t)hip_stream>>>(trs_stack, offset, stack_size,
+      (float*)data, m, n);
   }
   else {
     return LXS_ENOVARIANT;

@@ -3,8 +3,9 @@
  *
  * Semantics follow samples/smm/kernels/multiply.cl:12-15 with libxstream_common.h:86-87:
  * A block m-by-k column-major, B block k-by-n row-major (transposed by libsmm_acc_transpose),
- * C block m-by-n column-major; the stack holds (a,b,c) 1-based element offsets
- * (multiply.cl:96-99, param_format 0).
+ * C block m-by-n column-major; a stack record is ps ints whose first three are the a, b, c element
+ * offsets with index base pz (multiply.cl:96-99: ps = 3, pz = 1 for param_format 0; ps = 6, pz = 0 for
+ * DBM task records, seen from their fourth int).
  *
  * Work decomposition (new design, not the reference's mini-batch + atomic flush):
  *   - work-group w owns every C-run (maximal sequence of equal c) whose first entry lies in
@@ -50,7 +51,7 @@ __device__ __forceinline__ void flush_one(T* p, T v, int atomic) {
 template <typename T, int TM, int TN, int NT, int MC, int NC, int KC, int KTC>
 __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
-  int kt_rt, int atomic) {
+  int kt_rt, int atomic, int ps, int pz) {
   const int M = (0 < MC ? MC : m_rt), N = (0 < NC ? NC : n_rt), K = (0 < KC ? KC : k_rt);
   const int KT = (0 < KTC ? KTC : kt_rt);
   constexpr bool KEVEN = (0 < KC && 0 < KTC && 0 == (KC % KTC));
@@ -78,7 +79,7 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
   __syncthreads();
   /* first run head in [begin, end): entry i starts a run if its c differs from entry i-1 */
   for (int i = begin + tid; i < end; i += NT) {
-    if (0 == i || stack[3 * i + 2] != stack[3 * i - 1]) atomicMin(&s_idx, i);
+    if (0 == i || stack[ps * i + 2] != stack[ps * (i - 1) + 2]) atomicMin(&s_idx, i);
   }
   __syncthreads();
   int h = s_idx;
@@ -86,14 +87,14 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
   if (h >= end) return; /* the whole range continues a run owned by an earlier work-group */
 
   for (;;) {
-    const int c0 = stack[3 * h + 2];
+    const int c0 = stack[ps * h + 2];
     /* end of this run: first entry after h with a different c (or stack_size) */
     if (0 == tid) s_idx = stack_size;
     __syncthreads();
     int r_end;
     for (int base = h + 1;; base += NT) {
       const int i = base + tid;
-      if (i < stack_size && stack[3 * i + 2] != c0) atomicMin(&s_idx, i);
+      if (i < stack_size && stack[ps * i + 2] != c0) atomicMin(&s_idx, i);
       __syncthreads();
       r_end = s_idx;
       __syncthreads();
@@ -120,8 +121,8 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
       const T* ablk = adata;
       const T* bblk = bdata;
       if (valid) {
-        ablk = adata + (stack[3 * e + 0] - 1);
-        bblk = bdata + (stack[3 * e + 1] - 1);
+        ablk = adata + (stack[ps * e + 0] - pz);
+        bblk = bdata + (stack[ps * e + 1] - pz);
       }
       T prod[TM][TN];
 #pragma unroll
@@ -194,7 +195,7 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
       }
     }
 
-    T* const cblk = cdata + (c0 - 1);
+    T* const cblk = cdata + (c0 - pz);
     if (1 == G) {
       if (active) {
 #pragma unroll
@@ -244,7 +245,7 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
 template <typename T, int NT, int MC, int NC, int KC>
 __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
-  int kt_rt, int atomic) {
+  int kt_rt, int atomic, int ps, int pz) {
   constexpr int MN = MC * NC, P = 64 / MN, W = NT / 64;
   static_assert(0 < MN && MN <= 64 && 0 == NT % 64, "one C block per wavefront slot");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -259,7 +260,7 @@ __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* 
   int h = wbeg; /* the first run that starts in [wbeg, wend), 64 entries per step */
   for (;;) {
     const int x = h + lane;
-    const bool start = (x < wend) && (0 == x || stack[3 * x + 2] != stack[3 * x - 1]);
+    const bool start = (x < wend) && (0 == x || stack[ps * x + 2] != stack[ps * (x - 1) + 2]);
     const unsigned long long bal = __ballot(start);
     if (0 != bal) {
       h += __ffsll((long long)bal) - 1;
@@ -269,11 +270,11 @@ __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* 
     if (h >= wend) return;
   }
   while (h < wend) {
-    const int c0 = stack[3 * h + 2];
+    const int c0 = stack[ps * h + 2];
     int r_end = h + 1; /* the run's end, 64 entries per step */
     for (;;) {
       const int x = r_end + lane;
-      const unsigned long long bal = __ballot(x >= stack_size || stack[3 * x + 2] != c0);
+      const unsigned long long bal = __ballot(x >= stack_size || stack[ps * x + 2] != c0);
       if (0 != bal) {
         r_end += __ffsll((long long)bal) - 1;
         break;
@@ -282,19 +283,40 @@ __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* 
     }
     T acc = T(0);
     if (live) {
-      for (int e = h + slot; e < r_end; e += P) {
-        const T* const ablk = adata + (stack[3 * e] - 1);
-        const T* const bblk = bdata + (stack[3 * e + 1] - 1);
-        T prod = T(0);
+      /* U of the slot's entries per step, their loads issued together (entries past the run read the
+         slot's first entry and are not added); the run total still adds the entries in order */
+      constexpr int U = 4;
+      for (int e = h + slot; e < r_end; e += U * P) {
+        int ao[U], bo[U];
+        T av[U][KC], bv[U][KC];
 #pragma unroll
-        for (int kk = 0; kk < KC; ++kk) prod = fma(ablk[i + kk * MC], bblk[kk * NC + j], prod);
-        acc += prod;
+        for (int u = 0; u < U; ++u) {
+          const int x = (e + u * P < r_end ? e + u * P : e);
+          ao[u] = stack[ps * x] - pz;
+          bo[u] = stack[ps * x + 1] - pz;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+#pragma unroll
+          for (int kk = 0; kk < KC; ++kk) {
+            av[u][kk] = adata[ao[u] + i + kk * MC];
+            bv[u][kk] = bdata[bo[u] + kk * NC + j];
+          }
+        }
+        __builtin_amdgcn_sched_barrier(0); /* every load of the step is issued before the first product */
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          T prod = T(0);
+#pragma unroll
+          for (int kk = 0; kk < KC; ++kk) prod = fma(av[u][kk], bv[u][kk], prod);
+          if (e + u * P < r_end) acc += prod;
+        }
       }
     }
     T tot = acc;
 #pragma unroll
     for (int s = 1; s < P; ++s) tot += __shfl(acc, (lane + s * MN) & 63);
-    if (0 == slot) flush_one(cdata + (c0 - 1) + i + j * MC, tot, atomic);
+    if (0 == slot) flush_one(cdata + (c0 - pz) + i + j * MC, tot, atomic);
     h = r_end;
   }
 }
