@@ -315,10 +315,11 @@ def _event_ms(lx, stream, fn, reps):
     return ms
 
 
-# (kind, slices or primes, trim): scheme II with 16 moduli (the reference's default kind 3 picks it at these
-# sizes), scheme I over the slice triangle (cutoff s - 1) for s = 3 .. 8, and scheme I at the reference's
-# default 8 slices over all pairs
-OZAKI_CONFIGS = ((2, 16, 0),) + tuple((1, s, s - 1) for s in range(3, 9)) + ((1, 8, 0),)
+# (kind, slices or primes, trim): scheme II with up to 16 moduli (the reference's default kind 3 picks it at
+# these sizes) on the full 53-bit mantissas and with 2 and 6 bits trimmed (one and two moduli fewer), scheme I
+# over the slice triangle (cutoff s - 1) for s = 3 .. 8, and scheme I at the reference's default 8 slices over
+# all pairs
+OZAKI_CONFIGS = ((2, 16, 0), (2, 16, 1), (2, 16, 3)) + tuple((1, s, s - 1) for s in range(3, 9)) + ((1, 8, 0),)
 
 
 def ozaki_section(lx, stream, sizes, reps=3):

@@ -40,6 +40,8 @@ typedef int i16_t __attribute__((ext_vector_type(16)));
 typedef double d4_t __attribute__((ext_vector_type(4)));
 
 constexpr int OZ_TILE = 64; /* rows x k of a preprocessing tile */
+constexpr int LT = 256;        /* int8 level GEMM: output tile (i and j) per work-group */
+constexpr int LK = 128;        /* k padding of the digit planes */
 
 /*
  * Plane layout (digits and residues alike): tiles of 256 rows x 64 k, each 16 KB contiguous (row r of a
@@ -63,35 +65,44 @@ __device__ __forceinline__ int oz_expo(float v) {
   return (0 == be || 0xFF == be) ? OZ_EXP_NONE : be - 126;
 }
 
-/* load a 64x64 tile (rows r0.., k k0..) into LDS t[r][k] as double, coalesced along the contiguous
-   dimension; elements outside R x K, zero, subnormal or non-finite become 0 */
-template <typename T>
-__device__ __forceinline__ void oz_load_tile(double (*t)[OZ_TILE + 1], const T* __restrict__ p, long rs, long ks,
-  int r0, int k0, int R, int K) {
+/*
+ * Preprocessing work-groups cover 64 rows x 64 k; thread t owns 16 consecutive k of one row. With rows
+ * contiguous (rs == 1) a wavefront spans 64 rows (lane = row, wave = k chunk), so each of its 16 loads
+ * reads 64 consecutive elements; with k contiguous (ks == 1) four lanes share a row (lane % 4 = k chunk)
+ * and each lane reads 16 consecutive elements.
+ */
+__device__ __forceinline__ void oz_owner(long rs, int& r, int& kq) {
   const int tid = threadIdx.x;
-  if (1 == ks) { /* k contiguous: 16 threads per row, 4 consecutive k each; 16 rows per pass */
+  if (1 == rs) {
+    r = tid & 63;
+    kq = (tid >> 6) * 16;
+  }
+  else {
+    r = tid >> 2;
+    kq = (tid & 3) * 16;
+  }
+}
+
+/* elements (row, k .. k + 15) as double; outside R x K, zero, subnormal or non-finite (in T) become 0 */
+template <typename T>
+__device__ __forceinline__ double oz_keep(T x) { return (OZ_EXP_NONE != oz_expo(x)) ? (double)x : 0.0; }
+template <typename T>
+__device__ __forceinline__ void oz_load16(double* v, const T* __restrict__ p, long rs, long ks, int row, int k, int R,
+  int K) {
+  const T* const q = p + (long)row * rs + (long)k * ks;
+  if (row < R && k + 16 <= K) {
+    if (1 == ks) {
 #pragma unroll
-    for (int pass = 0; pass < 4; ++pass) {
-      const int r = pass * 16 + (tid >> 4), kq = (tid & 15) * 4;
+      for (int i = 0; i < 16; ++i) v[i] = oz_keep<T>(q[i]);
+    }
+    else {
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const int gr = r0 + r, gk = k0 + kq + x;
-        const T v = (gr < R && gk < K) ? p[(long)gr * rs + gk] : T(0);
-        t[r][kq + x] = (OZ_EXP_NONE != oz_expo(v)) ? (double)v : 0.0;
-      }
+      for (int i = 0; i < 16; ++i) v[i] = oz_keep<T>(q[(long)i * ks]);
     }
   }
-  else { /* rows contiguous: 16 threads per k, 4 consecutive rows each; 16 k per pass */
+  else {
 #pragma unroll
-    for (int pass = 0; pass < 4; ++pass) {
-      const int k = pass * 16 + (tid >> 4), rq = (tid & 15) * 4;
-#pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const int gr = r0 + rq + x, gk = k0 + k;
-        const T v = (gr < R && gk < K) ? p[gr + (long)gk * ks] : T(0);
-        t[rq + x][k] = (OZ_EXP_NONE != oz_expo(v)) ? (double)v : 0.0;
-      }
-    }
+    for (int i = 0; i < 16; ++i) v[i] = (row < R && k + i < K) ? oz_keep<T>(q[(long)i * ks]) : 0.0;
   }
 }
 
@@ -104,35 +115,51 @@ __global__ void __launch_bounds__(256) oz_init(int* __restrict__ e, int n, int* 
 template <typename T>
 __global__ void __launch_bounds__(256) oz_exp(const T* __restrict__ p, long rs, long ks, int R, int K,
   int* __restrict__ e) {
-  __shared__ double t[OZ_TILE][OZ_TILE + 1];
-  const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
-  oz_load_tile<T>(t, p, rs, ks, r0, k0, R, K);
-  __syncthreads();
-  { /* 4 threads per row, 16 k each, then a 4-lane maximum */
-    const int r = threadIdx.x >> 2, kq = (threadIdx.x & 3) * 16;
-    int ex = OZ_EXP_NONE;
+  __shared__ int red[4][64];
+  /* consecutive work-groups along the contiguous dimension */
+  const int r0 = (1 == ks ? blockIdx.y : blockIdx.x) * OZ_TILE, k0 = (1 == ks ? blockIdx.x : blockIdx.y) * OZ_TILE;
+  int r, kq;
+  oz_owner(rs, r, kq);
+  double v[16];
+  oz_load16<T>(v, p, rs, ks, r0 + r, k0 + kq, R, K);
+  int ex = OZ_EXP_NONE;
 #pragma unroll
-    for (int x = 0; x < 16; ++x) ex = max(ex, oz_expo(t[r][kq + x]));
+  for (int x = 0; x < 16; ++x) ex = max(ex, oz_expo(v[x]));
+  if (1 == rs) { /* the row's four chunks are in four wavefronts */
+    red[kq >> 4][r] = ex;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+      ex = max(max(red[0][r], red[1][r]), max(red[2][r], red[3][r]));
+      if (OZ_EXP_NONE != ex && r0 + r < R) atomicMax(e + r0 + r, ex);
+    }
+  }
+  else { /* four consecutive lanes */
     ex = max(ex, __shfl_xor(ex, 1));
     ex = max(ex, __shfl_xor(ex, 2));
     if (0 == (threadIdx.x & 3) && OZ_EXP_NONE != ex && r0 + r < R) atomicMax(e + r0 + r, ex);
   }
 }
 
+/* exponent pass: 64 x 64 tiles, the first grid dimension along the contiguous one */
+static dim3 oz_exp_grid(long ks, int Rp, int Kp) {
+  return (1 == ks) ? dim3((unsigned int)(Kp / OZ_TILE), (unsigned int)(Rp / OZ_TILE))
+                   : dim3((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
+}
+
 /* s int8 digits of 16 consecutive k of one row, written as one 16-byte vector per plane */
 template <typename T>
 __global__ void __launch_bounds__(256) oz_slice(const T* __restrict__ p, long rs, long ks, int R, int K,
   const int* __restrict__ e, int s, int Rp, int Kp, int8_t* __restrict__ planes, int* __restrict__ occ) {
-  __shared__ double t[OZ_TILE][OZ_TILE + 1];
   const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
-  const int tid = threadIdx.x, r = tid >> 2, kq = (tid & 3) * 16;
-  oz_load_tile<T>(t, p, rs, ks, r0, k0, R, K);
-  __syncthreads();
+  const int tid = threadIdx.x;
+  int r, kq;
+  oz_owner(rs, r, kq);
   const int row = r0 + r;
   const int ex = (row < R ? e[row] : OZ_EXP_NONE);
   double x[16];
+  oz_load16<T>(x, p, rs, ks, row, k0 + kq, R, K);
 #pragma unroll
-  for (int i = 0; i < 16; ++i) x[i] = (OZ_EXP_NONE != ex ? ldexp(t[r][kq + i], -ex) : 0.0); /* |x| < 1, exact */
+  for (int i = 0; i < 16; ++i) x[i] = (OZ_EXP_NONE != ex ? ldexp(x[i], -ex) : 0.0); /* |x| < 1, exact */
   for (int pl = 0; pl < s; ++pl) {
     union {
       int8_t b[16];
@@ -168,8 +195,6 @@ __global__ void __launch_bounds__(256) oz_scale_c(T* __restrict__ c, int ldc, in
 /* ---- int8 level GEMM ------------------------------------------------------------------------- */
 
 constexpr int GT = 128;        /* output tile of the FP64 comparator */
-constexpr int LT = 256;        /* int8 level GEMM: output tile (i and j) per work-group */
-constexpr int LK = 128;        /* k padding of the digit planes */
 
 /*
  * One work item of the int8 GEMM, in units of 64 k:
@@ -526,18 +551,18 @@ __device__ __forceinline__ unsigned int oz_mod27(unsigned int x, int p) {
 template <typename T>
 __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long rs, long ks, int R, int K,
   const int* __restrict__ e, int np, int mb, int trim, int Rp, int Kp, int8_t* __restrict__ planes) {
-  __shared__ double t[OZ_TILE][OZ_TILE + 1];
   const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
-  const int tid = threadIdx.x, r = tid >> 2, kq = (tid & 3) * 16;
-  oz_load_tile<T>(t, p, rs, ks, r0, k0, R, K);
-  __syncthreads();
+  int r, kq;
+  oz_owner(rs, r, kq);
   const int row = r0 + r;
   const int ex = (row < R ? e[row] : OZ_EXP_NONE);
+  double t[16];
+  oz_load16<T>(t, p, rs, ks, row, k0 + kq, R, K);
   unsigned int c0[16], c1[16], c2[16]; /* a = c2 2^36 + c1 2^18 + c0 */
   unsigned int neg = 0;
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
-    const double v = t[r][kq + i];
+    const double v = t[i];
     const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
     const int ev = oz_expo(v); /* |v| < 2^ev */
     const unsigned long long mant = (bits & 0xFFFFFFFFFFFFFull) | (1ull << 52);
@@ -577,6 +602,7 @@ __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long
 struct oz_crt_rec_t {
   unsigned int w[LXS_OZ_NPRIMES_MAX + 1][LXS_OZ_NPRIMES_MAX];
   unsigned int l16[LXS_OZ_NPRIMES_MAX + 1][LXS_OZ_NPRIMES_MAX][8];
+  unsigned int l16p[LXS_OZ_NPRIMES_MAX + 1][LXS_OZ_NPRIMES_MAX / 2][8]; /* limbs of primes 2h, 2h + 1 packed */
   unsigned int m16[LXS_OZ_NPRIMES_MAX + 1][8];
   unsigned long long hm[LXS_OZ_NPRIMES_MAX + 1][2], mm[LXS_OZ_NPRIMES_MAX + 1][2];
   double rm[LXS_OZ_NPRIMES_MAX]; /* 1 / m_p */
@@ -598,10 +624,15 @@ constexpr oz_crt_rec_t oz_crt_rec_make() {
       t.w[np][p] = (unsigned int)oz_inv_mod((int)(mp % (unsigned __int128)mod[p]), mod[p]);
       for (int k = 0; k < 8; ++k) t.l16[np][p][k] = (unsigned int)((mp >> (16 * k)) & 0xFFFF);
     }
+    for (int h = 0; h < LXS_OZ_NPRIMES_MAX / 2; ++h) {
+      for (int k = 0; k < 8; ++k) t.l16p[np][h][k] = t.l16[np][2 * h][k] | (t.l16[np][2 * h + 1][k] << 16);
+    }
   }
   return t;
 }
 __constant__ oz_crt_rec_t oz_rec = oz_crt_rec_make();
+
+typedef unsigned short oz_u16x2_t __attribute__((ext_vector_type(2)));
 
 /* 128-bit two's complement value (hi signed, lo unsigned) */
 struct oz_i128_t {
@@ -625,9 +656,9 @@ __device__ __forceinline__ oz_i128_t oz_sub(oz_i128_t a, long long bhi, unsigned
  * C(i, j) = fma(alpha, 2^(ea_i + eb_j + sh0) x, beta C(i, j)) where x is the integer of the residues
  * r_p = sum of the prime's residue bytes mod m_p, |x| < M / 2. Direct CRT: with c_p = r_p w_p mod m_p,
  * x_raw = sum_p c_p (M / m_p) = x (mod M) and x_raw / M = sum_p c_p / m_p, so x = x_raw - q M for
- * q = round(sum_p c_p / m_p) (FP64), then at most one exact step of M into [-M/2, M/2).
- * x_raw is summed in eight 16-bit limbs of 32-bit accumulators (c_p (M / m_p)_k < 2^24, 16 terms < 2^28:
- * no carries until the end). |x| is rounded to FP64 once (round to nearest even). Four consecutive
+ * q = round(sum_p c_p / m_p) (FP32: |x_raw - q M| < (1/2 + 2^-14) M), then at most one exact step of M
+ * into [-M/2, M/2). x_raw is summed in eight 16-bit limbs of 32-bit accumulators, two primes per
+ * v_dot2_u32_u16 (c_p (M / m_p)_k < 2^24, 16 terms < 2^28: no carries until the end). |x| is rounded to FP64 once (round to nearest even). Four consecutive
  * elements of a column per thread.
  */
 template <typename T>
@@ -645,38 +676,57 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
     const unsigned int jq = x / mq;
     const int i0 = (int)(4 * (x - jq * mq)), j = (int)jq;
     if (i0 >= M) return;
-    const uint8_t* const r8o = r8 + ((long)j * Mp + i0);
+    const unsigned int lo = (unsigned int)j * (unsigned int)Mp + (unsigned int)i0; /* < Mp Np < 2^32 */
     unsigned int acc[4][8];
-    double f[4];
+    float f[4];
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-      f[e] = 0.0;
+      f[e] = 0.0f;
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[e][k] = 0;
     }
 #pragma unroll
-    for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
-      if (p < np) {
-        const unsigned int mp = oz_crt.m[p];
-        const float rmp = oz_crt.rmf[p];
-        unsigned int s[4] = {0, 0, 0, 0};
-        for (int z = lv.zb[p]; z < lv.zb[p + 1]; ++z) { /* r_p = sum of the items' bytes (each < m_p) mod m_p */
-          const unsigned int w = *(const unsigned int*)(r8o + z * zs);
+    for (int p2 = 0; p2 < LXS_OZ_NPRIMES_MAX; p2 += 2) {
+      if (p2 < np) {
+        unsigned int cpair[4] = {0, 0, 0, 0}; /* c_p of primes p2 (low half) and p2 + 1 (high half) */
 #pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const unsigned int t = s[e] + ((w >> (8 * e)) & 0xFF);
-            s[e] = (t >= mp ? t - mp : t);
+        for (int h = 0; h < 2; ++h) {
+          const int p = p2 + h;
+          if (p < np) {
+            const unsigned int mp = oz_crt.m[p], wp = oz_rec.w[np][p];
+            const float rmp = oz_crt.rmf[p];
+            /* r_p = sum of the items' bytes (each < m_p) mod m_p: min(t, t - m) is t mod m for t < 2 m */
+            int z = lv.zb[p];
+            unsigned int w = *(const unsigned int*)(r8 + z * zs + lo), sv[4]; /* uniform base + 32-bit offset */
+#pragma unroll
+            for (int e = 0; e < 4; ++e) sv[e] = (w >> (8 * e)) & 0xFF;
+            for (++z; z < lv.zb[p + 1]; ++z) {
+              w = *(const unsigned int*)(r8 + z * zs + lo);
+#pragma unroll
+              for (int e = 0; e < 4; ++e) {
+                const unsigned int t = sv[e] + ((w >> (8 * e)) & 0xFF);
+                sv[e] = min(t, t - mp);
+              }
+            }
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+              /* cp = s w mod m_p, s w < 2^16: floor(s w / m) from FP32 is exact but at multiples of m (r = m) */
+              const unsigned int sw_ = __umul24(sv[e], wp);
+              const unsigned int r = (unsigned int)__mul24((int)((float)sw_ * rmp), -(int)mp) + sw_;
+              const unsigned int cp = min(r, r - mp);
+              f[e] = fmaf((float)cp, rmp, f[e]); /* sum_p c_p / m_p < np: absolute error below 2^-14 */
+              cpair[e] |= cp << (16 * h);
+            }
           }
         }
-        const unsigned int wp = oz_rec.w[np][p];
+        /* limb k += c_p (M / m_p)_k + c_p+1 (M / m_p+1)_k: products < 2^24, 16 terms < 2^28 */
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          /* cp = s w mod m_p, s w < 2^16: floor(s w / m) from FP32 is exact but at multiples of m (r = m) */
-          const unsigned int sw_ = __umul24(s[e], wp), r = sw_ - __umul24((unsigned int)((float)sw_ * rmp), mp);
-          const unsigned int cp = (r >= mp ? r - mp : r);
 #pragma unroll
-          for (int k = 0; k < 8; ++k) acc[e][k] = __umul24(cp, oz_rec.l16[np][p][k]) + acc[e][k];
-          f[e] = fma((double)cp, oz_rec.rm[p], f[e]);
+          for (int k = 0; k < 8; ++k) {
+            acc[e][k] = __builtin_amdgcn_udot2(__builtin_bit_cast(oz_u16x2_t, cpair[e]),
+              __builtin_bit_cast(oz_u16x2_t, oz_rec.l16p[np][p2 >> 1][k]), acc[e][k], false);
+          }
         }
       }
     }
@@ -684,7 +734,7 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
     for (int e = 0; e < 4; ++e) {
       const int i = i0 + e;
       if (i < M) {
-        const int q = (int)floor(f[e] + 0.5); /* 0 .. np */
+        const int q = (int)floorf(f[e] + 0.5f); /* 0 .. np; off by one at most, which the step below absorbs */
         /* x_raw - q M in 16-bit limbs, carries as arithmetic shifts; the top limb keeps the sign */
         unsigned long long lo = 0, hi = 0;
         int carry = 0;
@@ -853,16 +903,16 @@ __global__ void __launch_bounds__(256) oz_ccombine(const T* __restrict__ t1, con
 extern "C" int lxs_oz_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int s, int Rp, int Kp,
   int* e, int* occ, signed char* planes, void* hip_stream) {
   const hipStream_t st = (hipStream_t)hip_stream;
-  const dim3 grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
+  const dim3 egrid = oz_exp_grid(ks, Rp, Kp), grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
   if (0 != Rp % LT || 0 != Kp % LK || Rp < R || Kp < K || 0 >= Kp || 1 > s || 8 < s) return LXS_ELIMIT;
   if (!((1 == rs) || (1 == ks))) return LXS_ELIMIT;
   oz_init<<<(Rp + 255) / 256, 256, 0, st>>>(e, Rp, occ);
   if (3 == dtype) {
-    oz_exp<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
+    oz_exp<double><<<egrid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
     oz_slice<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e, s, Rp, Kp, planes, occ);
   }
   else if (1 == dtype) {
-    oz_exp<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
+    oz_exp<float><<<egrid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
     oz_slice<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e, s, Rp, Kp, planes, occ);
   }
   else {
@@ -940,17 +990,17 @@ extern "C" int lxs_oz_level_gemm(int crt, const signed char* pa, const signed ch
 extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int trim, int Rp,
   int Kp, int* e, signed char* planes, void* hip_stream) {
   const hipStream_t st = (hipStream_t)hip_stream;
-  const dim3 grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
+  const dim3 egrid = oz_exp_grid(ks, Rp, Kp), grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
   const int mb = (3 == dtype ? 53 : 24);
   if (0 != Rp % LT || 0 != Kp % LK || Rp < R || Kp < K || 0 >= Kp || 1 > np || LXS_OZ_NPRIMES_MAX < np) return LXS_ELIMIT;
   if (!((1 == rs) || (1 == ks)) || 0 > trim || mb <= trim) return LXS_ELIMIT;
   oz_init<<<(Rp + 255) / 256, 256, 0, st>>>(e, Rp, NULL);
   if (3 == dtype) {
-    oz_exp<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
+    oz_exp<double><<<egrid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
     oz_residues<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e, np, mb, trim, Rp, Kp, (int8_t*)planes);
   }
   else if (1 == dtype) {
-    oz_exp<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
+    oz_exp<float><<<egrid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
     oz_residues<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e, np, mb, trim, Rp, Kp, (int8_t*)planes);
   }
   else {
