@@ -146,6 +146,11 @@ LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* fil
  *  throughput (0..1); config->gflops = predicted GFLOP/s */
 LIBXSTREAM_API int libsmm_acc_mlp_predict_config(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
   int k, libsmm_acc_smm_config_t* config, double* confidence);
+/** the topk best valid configurations for the shape by the ensemble, best first: scores (may be NULL) =
+ *  predicted relative throughput, configs[i].gflops = predicted GFLOP/s; *count = min(topk, valid ones).
+ *  The learned-guided autotuner measures only these (tune_multiply -l model -k topk) */
+LIBXSTREAM_API int libsmm_acc_mlp_predict_topk(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
+  int k, int topk, libsmm_acc_smm_config_t* configs, double* scores, int* count);
 /** install (or remove: NULL) the model as predictor for shapes without a tuned configuration; a
  *  prediction below min_confidence leaves the built-in default in place */
 LIBXSTREAM_API int libsmm_acc_learned_set(libsmm_acc_mlp_t* model, double min_confidence);

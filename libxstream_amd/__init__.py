@@ -151,6 +151,8 @@ def _declare(L):
         "libsmm_acc_mlp_load": (i, [pvp, ctypes.c_char_p]),
         "libsmm_acc_mlp_predict_config": (i, [vp, i, i, i, i, ctypes.POINTER(SmmConfig),
                                               ctypes.POINTER(ctypes.c_double)]),
+        "libsmm_acc_mlp_predict_topk": (i, [vp, i, i, i, i, i, ctypes.POINTER(SmmConfig),
+                                            ctypes.POINTER(ctypes.c_double), pi]),
         "libsmm_acc_learned_set": (i, [vp, ctypes.c_double]),
         "ozaki_init": (i, [ctypes.POINTER(OzakiContext), i, i, i, i, i, i, i, i, i, i]),
         "ozaki_destroy": (None, [ctypes.POINTER(OzakiContext)]),
@@ -588,6 +590,15 @@ class Mlp:
         check(lib().libsmm_acc_mlp_predict_config(self.handle, tid, m, n, k, ctypes.byref(cfg), ctypes.byref(conf)),
               "libsmm_acc_mlp_predict_config")
         return cfg, conf.value
+
+    def predict_topk(self, tid, m, n, k, topk):
+        """the ensemble's topk configurations for the shape, best first, as [(SmmConfig, score)]"""
+        cfgs = (SmmConfig * topk)()
+        scores = (ctypes.c_double * topk)()
+        count = ctypes.c_int(0)
+        check(lib().libsmm_acc_mlp_predict_topk(self.handle, tid, m, n, k, topk, cfgs, scores, ctypes.byref(count)),
+              "libsmm_acc_mlp_predict_topk")
+        return [(cfgs[i], scores[i]) for i in range(count.value)]
 
     def save(self, path):
         check(lib().libsmm_acc_mlp_save(self.handle, os.fsencode(path)), "libsmm_acc_mlp_save")

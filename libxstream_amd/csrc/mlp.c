@@ -469,21 +469,28 @@ LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* fil
   return EXIT_SUCCESS;
 }
 
-/* shared by the public predictor call and the registry's predictor slot */
-static int mlp_predict(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n, int k,
-  libsmm_acc_smm_config_t* config, double* confidence) {
+/*
+ * The ensemble's ranking of every valid configuration of the shape (the set the exhaustive autotuner
+ * enumerates): the topk best by predicted relative throughput, best first, with their predicted
+ * relative throughput (scores) and predicted GFLOP/s (configs[i].gflops); *count of them.
+ */
+static int mlp_rank(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n, int k, int topk,
+  libsmm_acc_smm_config_t* configs, double* scores, int* count_out) {
   static const int bss[4] = {16, 32, 64, 128};
   const int nin = MLP_NSHAPE + mlp_nenc();
-  int cand[5 * 256], ncand, i, b, fl, count = 0, best = -1, result = EXIT_FAILURE;
+  int cand[5 * 256], ncand, i, j, b, fl, count = 0, result = EXIT_FAILURE;
   libsmm_acc_smm_config_t* cfgs;
   float *x, *out;
+  int* idx;
+  *count_out = 0;
   ncand = libsmm_acc_kernel_candidates(datatype, m, n, k, cand, 256);
-  if (0 >= ncand) return EXIT_FAILURE;
+  if (0 >= ncand || 0 >= topk) return EXIT_FAILURE;
   if (256 < ncand) ncand = 256;
   cfgs = (libsmm_acc_smm_config_t*)malloc(sizeof(libsmm_acc_smm_config_t) * (size_t)ncand * 8);
   x = (float*)malloc(sizeof(float) * (size_t)ncand * 8 * nin);
   out = (float*)malloc(sizeof(float) * (size_t)ncand * 8 * MLP_NOUT);
-  if (NULL != cfgs && NULL != x && NULL != out) {
+  idx = (int*)malloc(sizeof(int) * (size_t)ncand * 8);
+  if (NULL != cfgs && NULL != x && NULL != out && NULL != idx) {
     for (i = 0; i < ncand; ++i) { /* every valid configuration, as the exhaustive autotuner enumerates them */
       for (b = 0; b < 4; ++b) {
         for (fl = 1; fl <= 2; ++fl) {
@@ -509,22 +516,52 @@ static int mlp_predict(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int 
       }
     }
     if (0 < count && EXIT_SUCCESS == libsmm_acc_mlp_forward(model, x, count, out)) {
-      for (i = 0; i < count; ++i) {
-        if (0 > best || out[(size_t)i * MLP_NOUT] > out[(size_t)best * MLP_NOUT]) best = i;
+      const int nsel = (topk < count ? topk : count);
+      for (i = 0; i < count; ++i) idx[i] = i;
+      for (i = 0; i < nsel; ++i) { /* selection of the nsel best, ties to the earlier candidate */
+        int best = i;
+        for (j = i + 1; j < count; ++j) {
+          if (out[(size_t)idx[j] * MLP_NOUT] > out[(size_t)idx[best] * MLP_NOUT]) best = j;
+        }
+        j = idx[i];
+        idx[i] = idx[best];
+        idx[best] = j;
+        configs[i] = cfgs[idx[i]];
+        configs[i].gflops = exp2((double)out[(size_t)idx[i] * MLP_NOUT + 1]);
+        if (NULL != scores) scores[i] = (double)out[(size_t)idx[i] * MLP_NOUT];
       }
-      *config = cfgs[best];
-      config->gflops = exp2((double)out[(size_t)best * MLP_NOUT + 1]);
-      if (NULL != confidence) {
-        const double c = out[(size_t)best * MLP_NOUT];
-        *confidence = (c < 0 ? 0 : (c > 1 ? 1 : c));
-      }
+      *count_out = nsel;
       result = EXIT_SUCCESS;
     }
   }
   free(cfgs);
   free(x);
   free(out);
+  free(idx);
   return result;
+}
+
+/* shared by the public predictor call and the registry's predictor slot */
+static int mlp_predict(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n, int k,
+  libsmm_acc_smm_config_t* config, double* confidence) {
+  double score = 0;
+  int count = 0;
+  if (EXIT_SUCCESS != mlp_rank(model, datatype, m, n, k, 1, config, &score, &count) || 1 != count) {
+    return EXIT_FAILURE;
+  }
+  if (NULL != confidence) *confidence = (score < 0 ? 0 : (score > 1 ? 1 : score));
+  return EXIT_SUCCESS;
+}
+
+LIBXSTREAM_API int libsmm_acc_mlp_predict_topk(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
+  int k, int topk, libsmm_acc_smm_config_t* configs, double* scores, int* count) {
+  if (NULL == model || NULL == configs || NULL == count || 0 >= m || 0 >= n || 0 >= k || 0 >= topk) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_mlp_predict_topk: invalid argument");
+  }
+  if (EXIT_SUCCESS != mlp_rank(model, datatype, m, n, k, topk, configs, scores, count)) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_mlp_predict_topk: no valid configuration");
+  }
+  return EXIT_SUCCESS;
 }
 
 LIBXSTREAM_API int libsmm_acc_mlp_predict_config(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n, int k,

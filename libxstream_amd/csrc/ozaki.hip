@@ -685,6 +685,12 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[e][k] = 0;
     }
+    /* the first item's bytes of every prime, loaded before any is used (primes past np reload the last) */
+    unsigned int wv[LXS_OZ_NPRIMES_MAX];
+#pragma unroll
+    for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
+      wv[p] = *(const unsigned int*)(r8 + lv.zb[p < np ? p : np - 1] * zs + lo); /* uniform base + 32-bit offset */
+    }
 #pragma unroll
     for (int p2 = 0; p2 < LXS_OZ_NPRIMES_MAX; p2 += 2) {
       if (p2 < np) {
@@ -697,7 +703,7 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
             const float rmp = oz_crt.rmf[p];
             /* r_p = sum of the items' bytes (each < m_p) mod m_p: min(t, t - m) is t mod m for t < 2 m */
             int z = lv.zb[p];
-            unsigned int w = *(const unsigned int*)(r8 + z * zs + lo), sv[4]; /* uniform base + 32-bit offset */
+            unsigned int w = wv[p], sv[4];
 #pragma unroll
             for (int e = 0; e < 4; ++e) sv[e] = (w >> (8 * e)) & 0xFF;
             for (++z; z < lv.zb[p + 1]; ++z) {

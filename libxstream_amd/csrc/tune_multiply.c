@@ -13,8 +13,10 @@
  *     DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH (load with OPENCL_LIBSMM_SMM_PARAMS).
  *
  * usage: tune_multiply [-t f64|f32] [-s stack] [-r reps] [-o best.csv] [-j sweep.jsonl]
- *                      [-d device] [-T seconds] [-f triplets.txt | m n k [m n k ...]]
+ *                      [-d device] [-T seconds] [-l model.txt [-k topk]] [-f triplets.txt | m n k [m n k ...]]
  * -T stops before the next shape once the time budget is spent (a resumed run continues there).
+ * -l makes it learned-guided: only the model's topk (default 4) configurations are measured
+ * (libsmm_acc_mlp_predict_topk) instead of every valid one.
  */
 #include <math.h>
 #include <stdint.h>
@@ -27,6 +29,8 @@
 typedef struct tune_opts_t {
   int dtype, stack_size, reps;
   const char *csv, *jsonl, *device;
+  libsmm_acc_mlp_t* model; /* learned-guided: measure only the model's topk configurations */
+  int topk;
 } tune_opts_t;
 
 static uint64_t splitmix64(uint64_t* state) {
@@ -143,8 +147,9 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   const int dtype = o->dtype;
   tune_work_t w;
   void *da = NULL, *db = NULL, *dc = NULL, *ds = NULL, *stream = NULL;
-  int cand[5 * 256], ncand, i, j, b, f, result = EXIT_SUCCESS, nmeasured = 0;
+  int cand[5 * 256], ncand, i, j, b, f, result = EXIT_SUCCESS, nmeasured = 0, ntodo = 0;
   static const int bss[] = {16, 32, 64, 128};
+  static libsmm_acc_smm_config_t todo[2048];
   libsmm_acc_smm_config_t best, order[2048];
   double times[2048], flops;
   FILE* sweep = NULL;
@@ -169,57 +174,71 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
     result |= c_dbcsr_acc_memset_zero(dc, 0, (size_t)w.ts * w.nc * m * n, stream);
     result |= c_dbcsr_acc_stream_sync(stream);
   }
-  ncand = libsmm_acc_kernel_candidates((libsmm_acc_data_t)dtype, m, n, k, cand, 256);
-  if (NULL != o->jsonl) sweep = fopen(o->jsonl, "a");
-  libxstream_profile_enable(1);
-  for (i = 0; i < ncand && i < 256 && EXIT_SUCCESS == result; ++i) {
-    for (b = 0; b < 4; ++b) {
-      for (f = 1; f <= 2; ++f) {
-        libsmm_acc_smm_config_t cfg;
-        long cnt = 0;
-        double ms = 0;
-        int r;
-        memset(&cfg, 0, sizeof(cfg));
-        cfg.variant = cand[5 * i];
-        cfg.tm = cand[5 * i + 1];
-        cfg.tn = cand[5 * i + 2];
-        cfg.nt = cand[5 * i + 3];
-        cfg.kt = cand[5 * i + 4];
-        cfg.bs = bss[b];
-        cfg.flush = f;
-        if (EXIT_SUCCESS != libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) {
-          if (0 != cfg.variant) continue;
-          for (cfg.kt = (k > 8 ? (k - 1) / 8 * 8 : k - 1); 0 < cfg.kt; cfg.kt -= (cfg.kt > 8 ? 8 : 1)) {
-            if (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) break;
+  /* the configurations to measure: every valid one, or the learned model's topk */
+  if (NULL != o->model) {
+    if (EXIT_SUCCESS != libsmm_acc_mlp_predict_topk(o->model, (libsmm_acc_data_t)dtype, m, n, k,
+                          (o->topk < 2048 ? o->topk : 2048), todo, NULL, &ntodo)) {
+      ntodo = 0;
+    }
+  }
+  else {
+    ncand = libsmm_acc_kernel_candidates((libsmm_acc_data_t)dtype, m, n, k, cand, 256);
+    for (i = 0; i < ncand && i < 256; ++i) {
+      for (b = 0; b < 4; ++b) {
+        for (f = 1; f <= 2; ++f) {
+          libsmm_acc_smm_config_t cfg;
+          memset(&cfg, 0, sizeof(cfg));
+          cfg.variant = cand[5 * i];
+          cfg.tm = cand[5 * i + 1];
+          cfg.tn = cand[5 * i + 2];
+          cfg.nt = cand[5 * i + 3];
+          cfg.kt = cand[5 * i + 4];
+          cfg.bs = bss[b];
+          cfg.flush = f;
+          if (EXIT_SUCCESS != libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) {
+            if (0 != cfg.variant) continue;
+            for (cfg.kt = (k > 8 ? (k - 1) / 8 * 8 : k - 1); 0 < cfg.kt; cfg.kt -= (cfg.kt > 8 ? 8 : 1)) {
+              if (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) break;
+            }
+            if (0 >= cfg.kt) continue;
           }
-          if (0 >= cfg.kt) continue;
-        }
-        if (EXIT_SUCCESS != libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da,
-                              db, dc, m, n, k, stream)) {
-          (void)libxstream_error_consume(NULL, 0);
-          continue;
-        }
-        libxstream_profile_reset();
-        for (r = 0; r < o->reps; ++r) {
-          libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,
-            stream);
-        }
-        c_dbcsr_acc_stream_sync(stream);
-        libxstream_profile_get("smm", &cnt, &ms, NULL, NULL);
-        if (0 >= cnt || 0 >= ms) continue;
-        ms /= (double)cnt;
-        cfg.gflops = flops / (ms * 1e6);
-        if (nmeasured < 2048) {
-          order[nmeasured] = cfg;
-          times[nmeasured++] = ms;
-        }
-        if (NULL != sweep) {
-          fprintf(sweep,
-            "{\"dtype\": %i, \"m\": %i, \"n\": %i, \"k\": %i, \"S\": %i, \"variant\": %i, \"tm\": %i, \"tn\": %i, "
-            "\"nt\": %i, \"bs\": %i, \"kt\": %i, \"flush\": %i, \"ms\": %.6f, \"gflops\": %.2f}\n",
-            dtype, m, n, k, w.stack_size, cfg.variant, cfg.tm, cfg.tn, cfg.nt, cfg.bs, cfg.kt, cfg.flush, ms, cfg.gflops);
+          if (ntodo < 2048) todo[ntodo++] = cfg;
         }
       }
+    }
+  }
+  if (NULL != o->jsonl) sweep = fopen(o->jsonl, "a");
+  libxstream_profile_enable(1);
+  for (i = 0; i < ntodo && EXIT_SUCCESS == result; ++i) {
+    libsmm_acc_smm_config_t cfg = todo[i];
+    long cnt = 0;
+    double ms = 0;
+    int r;
+    cfg.gflops = 0;
+    if (EXIT_SUCCESS != libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da,
+                          db, dc, m, n, k, stream)) {
+      (void)libxstream_error_consume(NULL, 0);
+      continue;
+    }
+    libxstream_profile_reset();
+    for (r = 0; r < o->reps; ++r) {
+      libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,
+        stream);
+    }
+    c_dbcsr_acc_stream_sync(stream);
+    libxstream_profile_get("smm", &cnt, &ms, NULL, NULL);
+    if (0 >= cnt || 0 >= ms) continue;
+    ms /= (double)cnt;
+    cfg.gflops = flops / (ms * 1e6);
+    if (nmeasured < 2048) {
+      order[nmeasured] = cfg;
+      times[nmeasured++] = ms;
+    }
+    if (NULL != sweep) {
+      fprintf(sweep,
+        "{\"dtype\": %i, \"m\": %i, \"n\": %i, \"k\": %i, \"S\": %i, \"variant\": %i, \"tm\": %i, \"tn\": %i, "
+        "\"nt\": %i, \"bs\": %i, \"kt\": %i, \"flush\": %i, \"ms\": %.6f, \"gflops\": %.2f}\n",
+        dtype, m, n, k, w.stack_size, cfg.variant, cfg.tm, cfg.tn, cfg.nt, cfg.bs, cfg.kt, cfg.flush, ms, cfg.gflops);
     }
   }
   libxstream_profile_enable(0);
@@ -299,6 +318,7 @@ int main(int argc, char* argv[]) {
   tune_opts_t o;
   int i, result, ndevices = 0, ntuned = 0;
   FILE* triplets = NULL;
+  const char* model_path = NULL;
   double budget = 0;
   const double start = seconds_now();
   o.dtype = dbcsr_type_real_8;
@@ -307,6 +327,8 @@ int main(int argc, char* argv[]) {
   o.csv = NULL;
   o.jsonl = NULL;
   o.device = "MI355X";
+  o.model = NULL;
+  o.topk = 0;
   for (i = 1; i < argc && '-' == argv[i][0] && '\0' != argv[i][1] && i + 1 < argc; i += 2) {
     const char* const v = argv[i + 1];
     switch (argv[i][1]) {
@@ -318,6 +340,8 @@ int main(int argc, char* argv[]) {
       case 'd': o.device = v; break;
       case 'f': triplets = fopen(v, "r"); break;
       case 'T': budget = atof(v); break;
+      case 'l': model_path = v; break;
+      case 'k': o.topk = atoi(v); break;
       default: fprintf(stderr, "tune_multiply: unknown option %s\n", argv[i]); return EXIT_FAILURE;
     }
   }
@@ -327,6 +351,13 @@ int main(int argc, char* argv[]) {
   if (EXIT_SUCCESS != result || 0 >= ndevices) {
     fprintf(stderr, "tune_multiply: no device\n");
     return EXIT_FAILURE;
+  }
+  if (NULL != model_path) {
+    if (EXIT_SUCCESS != libsmm_acc_mlp_load(&o.model, model_path)) {
+      fprintf(stderr, "tune_multiply: cannot load the model %s\n", model_path);
+      return EXIT_FAILURE;
+    }
+    if (0 >= o.topk) o.topk = 4;
   }
   if (NULL != triplets) {
     int m, n, k;
@@ -343,6 +374,7 @@ int main(int argc, char* argv[]) {
   for (; i + 2 < argc + 0 && EXIT_SUCCESS == result; i += 3) {
     result = tune_shape(&o, atoi(argv[i]), atoi(argv[i + 1]), atoi(argv[i + 2]));
   }
+  if (NULL != o.model) libsmm_acc_mlp_destroy(o.model);
   libsmm_acc_finalize();
   return result;
 }

@@ -9,7 +9,8 @@ prepares arrays.
 
 Evaluation follows the exhaustive autotuner as the judge: the network's choice for a held-out shape
 is looked up in that shape's sweep (every configuration was measured there) and compared with the
-best measured one. The methods without the network are scored the same way: the built-in default
+best measured one. The learned-guided autotuner (tune_multiply -l model -k K) measures only the
+network's top K; its result is the best of those K measured configurations, looked up the same way. The methods without the network are scored the same way: the built-in default
 heuristic, and the reference's nearest tuned triplet (samples/smm/smm_kernel.c:116-148).
 """
 import gzip
@@ -22,6 +23,7 @@ import libxstream_amd as lx
 
 PARAM_KEYS = ("variant", "tm", "tn", "nt", "kt", "bs", "flush")
 OUT_WEIGHTS = (1.0, 0.01)  # loss weights of the outputs: relative throughput, log2 GFLOP/s
+TOPK = (2, 4, 8)  # learned-guided tuning: configurations measured per shape
 
 
 def triplets(count=500, seed=25071975, lo=4, hi=80):
@@ -142,8 +144,14 @@ def evaluate(model, sweep, keys, tuned_keys):
                 nearest = lookup(rows, ncfg)
         if nearest is None:  # not valid for this shape: the reference falls back to its default
             nearest = default
-        out.append({"key": list(key), "best": best, "learned": lookup(rows, cfg), "confidence": conf,
-                    "predicted_gflops": cfg.gflops, "default": default, "nearest": nearest})
+        entry = {"key": list(key), "best": best, "learned": lookup(rows, cfg), "confidence": conf,
+                 "predicted_gflops": cfg.gflops, "default": default, "nearest": nearest}
+        # learned-guided tuning: the best measured among the network's top K (K measurements per shape)
+        ranked = model.predict_topk(*key, max(TOPK))
+        for kk in TOPK:
+            got = [lookup(rows, c) for c, _ in ranked[:kk]]
+            entry["learned_top%d" % kk] = max([g for g in got if g is not None], default=None)
+        out.append(entry)
     return out
 
 
@@ -156,13 +164,14 @@ def summary(results, field):
 
 
 def report(results, tids):
+    fields = ("learned",) + tuple("learned_top%d" % kk for kk in TOPK) + ("default", "nearest")
     rep = {"heldout_shapes": len(results)}
-    for f in ("learned", "default", "nearest"):
+    for f in fields:
         rep[f] = summary(results, f)
     for tid in tids:
         sub = [x for x in results if x["key"][0] == tid]
         if sub:
-            rep["type%d" % tid] = {f: summary(sub, f) for f in ("learned", "default", "nearest")}
+            rep["type%d" % tid] = {f: summary(sub, f) for f in fields}
     return rep
 
 

@@ -109,6 +109,27 @@ def test_predicted_configs_valid_and_installed(dev, tid, tmp_path):
         model.close()
 
 
+@pytest.mark.parametrize("tid", [lx.REAL_8, lx.REAL_4])
+def test_predict_topk(dev, tid):
+    """the ranking behind learned-guided tuning: best first, distinct, valid, led by predict_config's choice"""
+    model = lx.Mlp(2, 64, 3, seed=5)
+    try:
+        for (m, n, k) in ((4, 4, 4), (23, 23, 23), (7, 60, 33), (45, 45, 13)):
+            ranked = model.predict_topk(tid, m, n, k, 8)
+            assert 1 <= len(ranked) <= 8
+            scores = [sc for _, sc in ranked]
+            assert scores == sorted(scores, reverse=True)
+            keys = {learned.config_key(c) for c, _ in ranked}
+            assert len(keys) == len(ranked)
+            assert all(lx.config_valid(tid, m, n, k, c) for c, _ in ranked)
+            top, conf = model.predict_config(tid, m, n, k)
+            assert learned.config_key(top) == learned.config_key(ranked[0][0])
+            assert conf == pytest.approx(min(max(scores[0], 0.0), 1.0))
+            assert len(model.predict_topk(tid, m, n, k, 10000)) >= len(ranked)
+    finally:
+        model.close()
+
+
 @pytest.fixture(scope="module")
 def shipped(dev):
     if not os.path.exists(MODEL):
@@ -133,6 +154,10 @@ def test_shipped_model_on_heldout(shipped):
     assert rep["learned"]["mean"] >= 0.9
     assert rep["learned"]["mean"] > rep["default"]["mean"] and rep["learned"]["mean"] > rep["nearest"]["mean"]
     assert rep["type3"]["learned"]["mean"] >= 0.9
+    # learned-guided tuning: measuring more of the network's ranking never loses
+    for f in ("learned_top2", "learned_top4", "learned_top8"):
+        assert all(r[f] is not None for r in results)
+    assert rep["learned"]["mean"] <= rep["learned_top2"]["mean"] <= rep["learned_top4"]["mean"] <= rep["learned_top8"]["mean"]
 
 
 def test_shipped_model_live(shipped):
