@@ -227,6 +227,16 @@ def batched(lx, stream, steps, warmup):
             r.process(stream)
     stream.sync()
     t3 = time.perf_counter()
+    per_stack = []  # kernel time of each stack's own launch: where the batch's time goes
+    for r in res:
+        lx.profile_reset()
+        for _ in range(steps):
+            r.process(stream)
+        stream.sync()
+        p = lx.profile_get("smm")
+        ms = p["ms"] / max(p["count"], 1)
+        per_stack.append({"shape": "%dx%dx%d" % (r.w.m, r.w.n, r.w.k), "entries": r.w.stack_size,
+                          "kernel_ms": round(ms, 4), "hbm_frac": round(r.w.bytes_alg() / (ms * 1e-3) / (HBM_PEAK_GBS * 1e9), 3)})
     lx.profile_enable(False)
     batch.close()
     for r in res:
@@ -244,6 +254,7 @@ def batched(lx, stream, steps, warmup):
         "alg_gbs": round(info["bytes"] / (prof["ms"] / max(prof["count"], 1) * 1e-3) * 1e-9, 1),
         "separate_launches_ms_per_step": round(single_ms, 4),
         "separate_launches_gflops": round(info["flops"] / (single_ms * 1e-3) * 1e-9, 1),
+        "separate_kernels": per_stack,
     }
 
 
