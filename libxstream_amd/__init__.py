@@ -39,6 +39,7 @@ class SmmConfig(ctypes.Structure):
         ("bs", ctypes.c_int),
         ("kt", ctypes.c_int),
         ("flush", ctypes.c_int),
+        ("s", ctypes.c_int),
         ("gflops", ctypes.c_double),
     ]
 

@@ -17,6 +17,10 @@
 #include <strings.h>
 
 #define LXS_SMM_MAXDIM_DEFAULT 80
+/* stack size of the autotuner's sweeps: the built-in and learned configurations are chosen for it */
+#define LXS_SMM_S 30000
+/* smallest bs the scaling for shorter stacks starts from (smm_kernel.c:49-51, 466-467) */
+#define LXS_SMM_BS_MIN 32
 
 typedef struct lxs_smm_slot_t {
   int type, m, n, k, used, source;
@@ -157,6 +161,7 @@ static int lxs_smm_default(int datatype, int m, int n, int k, libsmm_acc_smm_con
       config->nt = q[3];
       config->kt = q[4];
       config->bs = 32;
+      config->s = LXS_SMM_S;
       return EXIT_SUCCESS;
     }
   }
@@ -176,6 +181,7 @@ static int lxs_smm_default(int datatype, int m, int n, int k, libsmm_acc_smm_con
   }
   config->variant = 0;
   config->bs = 64;
+  config->s = LXS_SMM_S;
   if (0 == best_t && m * n <= 64) { /* tiny blocks: one element per thread, 64 threads */
     best_t = 1;
     best_nt = 64;
@@ -243,6 +249,7 @@ static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_co
   if (NULL == slot) {
     if (EXIT_SUCCESS == lxs_learned_predict(datatype, m, n, k, config)) { /* cached for the next lookup */
       src = libsmm_acc_config_learned;
+      config->s = LXS_SMM_S; /* the model learned from sweeps at this stack size */
       (void)lxs_smm_register(datatype, m, n, k, config, libsmm_acc_config_learned, 0);
     }
     else {
@@ -423,6 +430,7 @@ LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
         config.bs = LXS_GETI("BS");
         config.kt = LXS_GETI("KT");
         config.flush = LXS_GETI("FLUSH");
+        config.s = LXS_GETI("S");
 #undef LXS_GETI
         if (0 != with_perf) config.gflops = atof(values[lxs_column(names, ncolumns, "GFLOPS")]);
         ok = (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)key[0], key[1], key[2], key[3], &config));
@@ -435,6 +443,7 @@ LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
         /* only BS (stack entries per work-group) carries over to these kernels */
         ok = (EXIT_SUCCESS == lxs_smm_default(key[0], key[1], key[2], key[3], &config));
         config.bs = (0 < params[0] ? params[0] : config.bs);
+        config.s = (int)perf[0];
         config.gflops = perf[1];
       }
     }
@@ -460,7 +469,7 @@ LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* devi
     const lxs_smm_slot_t* const s = lxs_smm_slots + i;
     if (0 == s->used) continue;
     fprintf(file, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i\n", NULL != device ? device : "MI355X", s->type, s->m,
-      s->n, s->k, 0, s->config.gflops, s->config.variant, s->config.tm, s->config.tn, s->config.nt, s->config.bs,
+      s->n, s->k, s->config.s, s->config.gflops, s->config.variant, s->config.tm, s->config.tn, s->config.nt, s->config.bs,
       s->config.kt, s->config.flush);
     ++nrows;
   }
@@ -596,6 +605,13 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   args.nt = config->nt;
   args.bs = config->bs;
   args.kt = config->kt;
+  /* a stack shorter than the one bs was chosen for spreads over more work-groups (smm_kernel.c:463-473);
+     the result does not depend on bs: every C-run is summed by the work-group where it starts */
+  if (1 < config->bs && 1 < config->s && stack_size < config->s) {
+    const long cbs = (config->bs > LXS_SMM_BS_MIN ? config->bs : LXS_SMM_BS_MIN);
+    const long bs = ((long)stack_size * cbs + config->s - 2) / (config->s - 1);
+    args.bs = (int)(bs < config->bs ? (1 > bs ? 1 : bs) : config->bs);
+  }
   /* flush: 1 = read-modify-write (caller guarantees each C block is one run), else atomic add,
      which is exact-once per element when the stack is grouped by C and correct otherwise */
   args.atomic = (1 == config->flush ? 0 : 1);

@@ -275,6 +275,7 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
     fprintf(stderr, "tune_multiply: %ix%ix%i configuration %i failed the check (error %g)\n", m, n, k, i, err);
   }
   if (0 < best.nt) {
+    best.s = w.stack_size; /* bs was chosen for this stack size */
     libsmm_acc_config_set((libsmm_acc_data_t)dtype, m, n, k, &best);
     printf("%ix%ix%i %s: %.1f GFLOPS/s (variant=%i tm=%i tn=%i nt=%i bs=%i kt=%i flush=%i, %i configurations)\n", m,
       n, k, 8 == w.ts ? "f64" : "f32", best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt,

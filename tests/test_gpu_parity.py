@@ -72,8 +72,27 @@ def test_chunk_sizes(stream, bs):
     w = make_stack_workload(m, n, k, 4000, dtype=np.float64, seed=_seed("bs", bs), c_init="uniform", spread=12)
     cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
     cfg.bs = bs
+    cfg.s = 0  # launch with exactly this bs (no scaling for the short stack)
     out = run_stack(stream, w, config=cfg)
     assert oracle_error(w, out)[0] <= 1e-13
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", [(23, 23, 23), (5, 5, 5), (4, 9, 22), (45, 45, 13)], ids=lambda s: "%dx%dx%d" % s)
+def test_short_stack_bs_scaling(stream, shape, dtype):
+    """A stack shorter than the configuration's S launches with bs scaled down (smm_kernel.c:463-473):
+    more work-groups, the same C bit for bit (every C-run is summed where it starts)."""
+    m, n, k = shape
+    w = make_stack_workload(m, n, k, 1900, dtype=dtype, seed=_seed("short", shape), c_init="uniform")
+    tid = lx.typeid(dtype)
+    cfg, _ = lx.config_get(tid, m, n, k)
+    cfg.bs = 128
+    cfg.s = 30000  # scaled: ceil(1900 * 128 / 29999) = 9 entries per work-group
+    scaled = run_stack(stream, w, config=cfg, device_transpose=False)
+    cfg.s = 0
+    plain = run_stack(stream, w, config=cfg, device_transpose=False)
+    assert np.array_equal(scaled, plain)
+    assert oracle_error(w, scaled)[0] <= tol(dtype)
 
 
 @pytest.mark.parametrize("flush", [1, 2])
