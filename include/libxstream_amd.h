@@ -98,9 +98,21 @@ typedef struct libsmm_acc_batch_stack_t {
   void* dev_c_data;
 } libsmm_acc_batch_stack_t;
 typedef struct libsmm_acc_batch_t libsmm_acc_batch_t;
+/** the batch's schedule: entries per chunk sized to move about chunk_elements operand elements (8..512
+ *  entries), blocks_per_cu persistent work-groups per CU; 0 selects the default (32 entries of 23x23x23, 8) */
+typedef struct libsmm_acc_batch_params_t {
+  int chunk_elements, blocks_per_cu;
+} libsmm_acc_batch_params_t;
 /** plan the batch and upload the plan (allocates device memory; not for use under capture) */
 LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
   int nstacks, libsmm_acc_batch_t** batch);
+/** the same with an explicit schedule (NULL: default) */
+LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
+  int nstacks, const libsmm_acc_batch_params_t* params, libsmm_acc_batch_t** batch);
+/** autotune the schedule for these stacks: 18 candidates timed on the stacks and pools given, C redirected
+ *  to scratch (the caller's C is not touched); blocks until done (allocates; not under capture) */
+LIBXSTREAM_API int libsmm_acc_batch_tune(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
+  int nstacks, libsmm_acc_batch_params_t* best, double* best_ms);
 /** enqueue the batch as one kernel launch on stream */
 LIBXSTREAM_API int libsmm_acc_batch_process(libsmm_acc_batch_t* batch, void* stream);
 /** replay the batch from a hipGraph (captured on the first call) on stream */

@@ -65,6 +65,15 @@ class BatchStack(ctypes.Structure):
     ]
 
 
+class BatchParams(ctypes.Structure):
+    """libsmm_acc_batch_params_t (include/libxstream_amd.h): the batch's schedule."""
+
+    _fields_ = [("chunk_elements", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
+
+    def as_dict(self):
+        return {"chunk_elements": self.chunk_elements, "blocks_per_cu": self.blocks_per_cu}
+
+
 class OzakiContext(ctypes.Structure):
     """include/ozaki.h: ozaki_context_t (caller-allocated, filled by ozaki_init)."""
     _fields_ = [(name, ctypes.c_int) for name in ("use_double", "kind", "ndecomp", "nslices", "nprimes", "ozflags",
@@ -128,6 +137,9 @@ def _declare(L):
         "libsmm_acc_params_save": (i, [ctypes.c_char_p, ctypes.c_char_p]),
         "libsmm_acc_read_ref_params": (i, [ctypes.c_char_p, i, i, ctypes.c_char_p, sz, pi, pi, ctypes.POINTER(ctypes.c_double)]),
         "libsmm_acc_batch_create": (i, [i, ctypes.POINTER(BatchStack), i, pvp]),
+        "libsmm_acc_batch_create_ex": (i, [i, ctypes.POINTER(BatchStack), i, ctypes.POINTER(BatchParams), pvp]),
+        "libsmm_acc_batch_tune": (i, [i, ctypes.POINTER(BatchStack), i, ctypes.POINTER(BatchParams),
+                                      ctypes.POINTER(ctypes.c_double)]),
         "libsmm_acc_batch_process": (i, [vp, vp]),
         "libsmm_acc_batch_replay": (i, [vp, vp]),
         "libsmm_acc_batch_info": (i, [vp, pi, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]),
@@ -428,16 +440,32 @@ def profile_get(kernel):
 class Batch:
     """Several stacks of different shapes in one persistent launch (libsmm_acc_batch_*)."""
 
-    def __init__(self, tid, stacks):
-        """stacks: list of (stack DeviceBuffer, stack_size, m, n, k, a DeviceBuffer, b DeviceBuffer, c DeviceBuffer)."""
+    @staticmethod
+    def _stacks(stacks):
         arr = (BatchStack * len(stacks))()
         for x, (st, size, m, n, k, a, b, c) in zip(arr, stacks):
             x.dev_param_stack = st.ptr.value if st and st.ptr else None
             x.stack_size, x.m, x.n, x.k = size, m, n, k
             x.dev_a_data, x.dev_b_data, x.dev_c_data = a.ptr.value, b.ptr.value, c.ptr.value
+        return arr
+
+    def __init__(self, tid, stacks, params=None):
+        """stacks: list of (stack DeviceBuffer, stack_size, m, n, k, a DeviceBuffer, b DeviceBuffer, c DeviceBuffer);
+        params: a BatchParams schedule (None: the default)."""
+        arr = self._stacks(stacks)
         h = ctypes.c_void_p()
-        check(lib().libsmm_acc_batch_create(tid, arr, len(stacks), ctypes.byref(h)), "libsmm_acc_batch_create")
+        check(lib().libsmm_acc_batch_create_ex(tid, arr, len(stacks), ctypes.byref(params) if params else None,
+                                               ctypes.byref(h)), "libsmm_acc_batch_create_ex")
         self.handle = h
+
+    @staticmethod
+    def tune(tid, stacks):
+        """libsmm_acc_batch_tune: (BatchParams of the fastest schedule, its ms per launch)"""
+        arr = Batch._stacks(stacks)
+        best, ms = BatchParams(), ctypes.c_double(0)
+        check(lib().libsmm_acc_batch_tune(tid, arr, len(stacks), ctypes.byref(best), ctypes.byref(ms)),
+              "libsmm_acc_batch_tune")
+        return best, ms.value
 
     def process(self, stream=None):
         check(lib().libsmm_acc_batch_process(self.handle, stream.handle if stream else None), "libsmm_acc_batch_process")

@@ -36,6 +36,14 @@ static int lxs_chunk_cmp(const void* x, const void* y) {
 
 LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
   int nstacks, libsmm_acc_batch_t** batch) {
+  return libsmm_acc_batch_create_ex(datatype, stacks, nstacks, NULL, batch);
+}
+
+LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
+  int nstacks, const libsmm_acc_batch_params_t* params, libsmm_acc_batch_t** batch) {
+  const int chunk_elements = (NULL != params && 0 < params->chunk_elements ? params->chunk_elements
+                                                                            : LXS_BATCH_CHUNK_ELEMENTS);
+  const int blocks_per_cu = (NULL != params && 0 < params->blocks_per_cu ? params->blocks_per_cu : 8);
   const int ts = (dbcsr_type_real_8 == datatype ? 8 : (dbcsr_type_real_4 == datatype ? 4 : 0));
   lxs_batch_stack_t* desc = NULL;
   lxs_batch_task_t* tasks = NULL;
@@ -75,7 +83,7 @@ LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const lib
     d->k = s->k;
     d->ti = (s->m + 15) / 16;
     d->tj = (s->n + 15) / 16;
-    d->bs = LXS_BATCH_CHUNK_ELEMENTS / (s->m * s->k + s->k * s->n);
+    d->bs = chunk_elements / (s->m * s->k + s->k * s->n);
     if (d->bs < 8) d->bs = 8;
     if (d->bs > 512) d->bs = 512;
     if (4 < d->ti * d->tj) nbig += (s->stack_size + d->bs - 1) / d->bs;
@@ -127,7 +135,7 @@ LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const lib
     b->dtype = (int)datatype;
     b->nstacks = nstacks;
     b->ntasks = ntasks;
-    b->nblocks = (0 < lxs_cfg.ncus ? 8 * lxs_cfg.ncus : 2048);
+    b->nblocks = (0 < lxs_cfg.ncus ? blocks_per_cu * lxs_cfg.ncus : 256 * blocks_per_cu);
     if (b->nblocks > ntasks) b->nblocks = ntasks;
     result = lxs_hip(hipMalloc((void**)&b->desc, sizeof(*desc) * (size_t)nstacks), "hipMalloc batch");
   }
@@ -220,5 +228,89 @@ LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch) {
     if (NULL != batch->tasks) (void)hipFree(batch->tasks);
     free(batch);
   }
+  return result;
+}
+
+/*
+ * The batch's schedule autotuner: every candidate (chunk size, persistent work-groups per CU) is planned
+ * and timed with HIP events on the caller's stacks and pools, with C redirected to scratch buffers (the
+ * caller's C is neither read nor written), and the fastest is returned.
+ */
+LIBXSTREAM_API int libsmm_acc_batch_tune(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
+  int nstacks, libsmm_acc_batch_params_t* best, double* best_ms) {
+  static const int scales[] = {1, 2, 4, 8, 16, 32}; /* chunk size in 23x23x23 entries x 8 */
+  static const int wgs[] = {8, 16, 32};
+  const int ts = (dbcsr_type_real_8 == datatype ? 8 : (dbcsr_type_real_4 == datatype ? 4 : 0));
+  libsmm_acc_batch_stack_t* trial = NULL;
+  void** scratch = NULL;
+  hipStream_t hs = NULL;
+  hipEvent_t e0 = NULL, e1 = NULL;
+  double tbest = -1;
+  int i, result;
+  if (NULL == stacks || 0 >= nstacks || NULL == best || 0 == ts) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_batch_tune: invalid argument");
+  }
+  result = lxs_ensure_device();
+  if (EXIT_SUCCESS != result) return result;
+  trial = (libsmm_acc_batch_stack_t*)malloc(sizeof(*trial) * (size_t)nstacks);
+  scratch = (void**)calloc((size_t)nstacks, sizeof(void*));
+  if (NULL == trial || NULL == scratch) result = lxs_error(EXIT_FAILURE, "libsmm_acc_batch_tune: out of memory");
+  for (i = 0; i < nstacks && EXIT_SUCCESS == result; ++i) { /* scratch C covering every C offset of the stack */
+    const libsmm_acc_batch_stack_t* const s = stacks + i;
+    long cmax = 0, j;
+    trial[i] = *s;
+    if (0 < s->stack_size && NULL != s->dev_param_stack) {
+      int* const host = (int*)malloc(sizeof(int) * 3 * (size_t)s->stack_size);
+      if (NULL == host) {
+        result = lxs_error(EXIT_FAILURE, "libsmm_acc_batch_tune: out of memory");
+        break;
+      }
+      result = lxs_hip(hipMemcpy(host, s->dev_param_stack, sizeof(int) * 3 * (size_t)s->stack_size,
+        hipMemcpyDeviceToHost), "batch tune: stack download");
+      for (j = 0; j < s->stack_size && EXIT_SUCCESS == result; ++j) {
+        if (host[3 * j + 2] > cmax) cmax = host[3 * j + 2];
+      }
+      free(host);
+      if (EXIT_SUCCESS == result) {
+        const size_t nbytes = (size_t)ts * (size_t)(cmax - 1 + (long)s->m * s->n);
+        result = lxs_hip(hipMalloc(&scratch[i], nbytes), "batch tune: scratch C");
+        if (EXIT_SUCCESS == result) result = lxs_hip(hipMemset(scratch[i], 0, nbytes), "batch tune: scratch C");
+        trial[i].dev_c_data = scratch[i];
+      }
+    }
+  }
+  if (EXIT_SUCCESS == result) result = lxs_hip(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking), "hipStreamCreate");
+  if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreate(&e0), "hipEventCreate");
+  if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreate(&e1), "hipEventCreate");
+  for (i = 0; i < (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs)) && EXIT_SUCCESS == result; ++i) {
+    libsmm_acc_batch_params_t p;
+    libsmm_acc_batch_t* b = NULL;
+    float ms = 0;
+    int r;
+    p.chunk_elements = scales[i / 3] * 8 * 2 * 23 * 23;
+    p.blocks_per_cu = wgs[i % 3];
+    result = libsmm_acc_batch_create_ex(datatype, trial, nstacks, &p, &b);
+    if (EXIT_SUCCESS == result) result = lxs_batch_enqueue(b, hs); /* warm-up */
+    if (EXIT_SUCCESS == result) result = lxs_hip(hipEventRecord(e0, hs), "hipEventRecord");
+    for (r = 0; r < 3 && EXIT_SUCCESS == result; ++r) result = lxs_batch_enqueue(b, hs);
+    if (EXIT_SUCCESS == result) result = lxs_hip(hipEventRecord(e1, hs), "hipEventRecord");
+    if (EXIT_SUCCESS == result) result = lxs_hip(hipEventSynchronize(e1), "hipEventSynchronize");
+    if (EXIT_SUCCESS == result) result = lxs_hip(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+    if (NULL != b) libsmm_acc_batch_destroy(b);
+    if (EXIT_SUCCESS == result && (0 > tbest || ms / 3.0 < tbest)) {
+      tbest = ms / 3.0;
+      *best = p;
+    }
+  }
+  if (NULL != hs) (void)hipStreamSynchronize(hs);
+  if (NULL != e0) (void)hipEventDestroy(e0);
+  if (NULL != e1) (void)hipEventDestroy(e1);
+  if (NULL != hs) (void)hipStreamDestroy(hs);
+  for (i = 0; i < nstacks && NULL != scratch; ++i) {
+    if (NULL != scratch[i]) (void)hipFree(scratch[i]);
+  }
+  free(scratch);
+  free(trial);
+  if (EXIT_SUCCESS == result && NULL != best_ms) *best_ms = tbest;
   return result;
 }

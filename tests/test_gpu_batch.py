@@ -127,3 +127,26 @@ def test_batch_rejects_large_blocks(stream):
             lx.Batch(lx.REAL_8, [(d, 1, 65, 8, 8, d, d, d)])
     finally:
         d.free()
+
+
+def test_batch_tune_schedule(stream):
+    """The schedule autotuner leaves the caller's C alone, and every schedule gives the oracle's result."""
+    works = make_mixed_batch(12, seed=13, dtype=np.float64, len_range=(50, 4000), c_init="uniform")
+    res = Resident(works, stream)
+    try:
+        before = [res.download(i).copy() for i in range(len(works))]
+        best, ms = lx.Batch.tune(lx.REAL_8, res.items)
+        assert ms > 0 and best.chunk_elements > 0 and best.blocks_per_cu > 0
+        for i in range(len(works)):
+            assert np.array_equal(res.download(i), before[i]), "tuning wrote the caller's C (stack %d)" % i
+        expected = [oracle.stack_dd(w.m, w.n, w.k, w.stack, w.a, w.b, c_in=w.c.ravel()) for w in works]
+        for params in (best, lx.BatchParams(8 * 2 * 23 * 23, 4), lx.BatchParams(64 * 2 * 23 * 23, 16)):
+            for i, w in enumerate(works):
+                res.items[i][7].upload(w.c)
+            batch = lx.Batch(lx.REAL_8, res.items, params)
+            batch.replay(stream)
+            stream.sync()
+            check_all(res, expected)
+            batch.close()
+    finally:
+        res.free()
