@@ -105,10 +105,13 @@ __device__ __forceinline__ void ozs_dma16(const int8_t* src, int8_t* lds_dst) {
  * 4 chunks of 16 bytes with chunk c at position c ^ ((r >> 2) & 3). Wave w moves rows 16 w .. 16 w + 15 of
  * every plane (one DMA instruction per plane: lane l writes position l % 4 of row 16 w + l / 4).
  */
-template <typename T, int NS>
+template <typename T, int NS, int CUT>
 __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, int S, const int* __restrict__ info,
   const int8_t* __restrict__ pa, const int* __restrict__ ea, const int8_t* __restrict__ pb, const int* __restrict__ eb,
-  int cutoff, int m, int n, int k, int na, int nb, T* __restrict__ c) {
+  int cutoff_rt, int m, int n, int k, int na, int nb, T* __restrict__ c) {
+  /* the level cutoff: compiled in (CUT >= 0), so the pair loop below has no conditional matrix-core work,
+     or taken at run time (CUT < 0) */
+  const int cutoff = (0 <= CUT ? CUT : cutoff_rt);
   __shared__ __attribute__((aligned(1024))) int8_t lds0[2 * NS * SPL], lds1[2 * NS * SPL];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -239,18 +242,33 @@ __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, 
   }
 }
 
+/* NS slices with the level cutoff compiled in for the triangle (NS - 1) and all pairs (2 NS - 2) */
+template <typename T, int NS>
+void ozs_launch_ns(dim3 grid, hipStream_t st, const int* stack, int S, const int* info, const int8_t* pa,
+  const int* ea, const int8_t* pb, const int* eb, int cutoff, int m, int n, int k, int na, int nb, T* c) {
+  if (NS - 1 == cutoff) {
+    ozs_stack<T, NS, NS - 1><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c);
+  }
+  else if (2 * (NS - 1) == cutoff) {
+    ozs_stack<T, NS, 2 * (NS - 1)><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c);
+  }
+  else {
+    ozs_stack<T, NS, -1><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c);
+  }
+}
+
 template <typename T>
 int ozs_launch_stack(int s, dim3 grid, hipStream_t st, const int* stack, int S, const int* info,
   const int8_t* pa, const int* ea, const int8_t* pb, const int* eb, int cutoff, int m, int n, int k, int na, int nb, T* c) {
   switch (s) {
-    case 1: ozs_stack<T, 1><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 2: ozs_stack<T, 2><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 3: ozs_stack<T, 3><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 4: ozs_stack<T, 4><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 5: ozs_stack<T, 5><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 6: ozs_stack<T, 6><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 7: ozs_stack<T, 7><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
-    case 8: ozs_stack<T, 8><<<grid, 256, 0, st>>>(stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 1: ozs_launch_ns<T, 1>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 2: ozs_launch_ns<T, 2>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 3: ozs_launch_ns<T, 3>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 4: ozs_launch_ns<T, 4>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 5: ozs_launch_ns<T, 5>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 6: ozs_launch_ns<T, 6>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 7: ozs_launch_ns<T, 7>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
+    case 8: ozs_launch_ns<T, 8>(grid, st, stack, S, info, pa, ea, pb, eb, cutoff, m, n, k, na, nb, c); break;
     default: return LXS_ELIMIT;
   }
   return 0;
