@@ -99,9 +99,15 @@ __device__ __forceinline__ void ozs_dma16(const int8_t* src, int8_t* lds_dst) {
   __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
     (__attribute__((address_space(3))) void*)lds_dst, 16, 0, 0);
 }
+__device__ __forceinline__ void ozs_dma4(const int* src, int8_t* lds_dst) {
+  __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+    (__attribute__((address_space(3))) void*)lds_dst, 4, 0, 0);
+}
+constexpr int SEX = 512; /* LDS bytes per wavefront for an entry's exponent rows (A rows, then B columns) */
 
 /*
- * LDS: two entry buffers of 2 NS planes (A planes, then B planes), 4 KB each; row r of a plane holds its
+ * LDS: two entry buffers of 2 NS planes (A planes, then B planes), 4 KB each, and each wavefront's copy of the
+ * entry's exponent rows (2 x 256 bytes); row r of a plane holds its
  * 4 chunks of 16 bytes with chunk c at position c ^ ((r >> 2) & 3). Wave w moves rows 16 w .. 16 w + 15 of
  * every plane (one DMA instruction per plane: lane l writes position l % 4 of row 16 w + l / 4).
  */
@@ -112,7 +118,7 @@ __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, 
   /* the level cutoff: compiled in (CUT >= 0), so the pair loop below has no conditional matrix-core work,
      or taken at run time (CUT < 0) */
   const int cutoff = (0 <= CUT ? CUT : cutoff_rt);
-  __shared__ __attribute__((aligned(1024))) int8_t lds0[2 * NS * SPL], lds1[2 * NS * SPL];
+  __shared__ __attribute__((aligned(1024))) int8_t lds0[2 * NS * SPL + 4 * SEX], lds1[2 * NS * SPL + 4 * SEX];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wi = wave & 1, wj = wave >> 1;
@@ -170,14 +176,19 @@ __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, 
       ab = (0 <= ao && (unsigned int)ao == aq * mk && aq < (unsigned int)na) ? (int)aq : -1;
       bb = (0 <= bo && (unsigned int)bo == bq * kn && bq < (unsigned int)nb) ? (int)bq : -1;
     };
+    /* no branch around the DMA (an entry without valid blocks loads block 0, unused), so the compiler
+       can count the outstanding loads on every path */
     auto dma = [&](int8_t* buf, int ab, int bb) {
-      if (0 <= ab && 0 <= bb) {
+      ab = (0 <= ab ? ab : 0);
+      bb = (0 <= bb ? bb : 0);
 #pragma unroll
-        for (int pl = 0; pl < NS; ++pl) {
-          ozs_dma16(pa + ((long)ab * NS + pl) * SPL + drow * SB + 16 * dchunk, buf + pl * SPL + wave * 1024);
-          ozs_dma16(pb + ((long)bb * NS + pl) * SPL + drow * SB + 16 * dchunk, buf + (NS + pl) * SPL + wave * 1024);
-        }
+      for (int pl = 0; pl < NS; ++pl) {
+        ozs_dma16(pa + ((long)ab * NS + pl) * SPL + drow * SB + 16 * dchunk, buf + pl * SPL + wave * 1024);
+        ozs_dma16(pb + ((long)bb * NS + pl) * SPL + drow * SB + 16 * dchunk, buf + (NS + pl) * SPL + wave * 1024);
       }
+      /* the 64 row exponents of the A block and column exponents of the B block, one copy per wavefront */
+      ozs_dma4(ea + (long)ab * SB + lane, buf + 2 * NS * SPL + wave * SEX);
+      ozs_dma4(eb + (long)bb * SB + lane, buf + 2 * NS * SPL + wave * SEX + SB * 4);
     };
     int ab, bb;
     blocks(e0, ab, bb);
@@ -188,11 +199,17 @@ __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, 
       const int ac = ab, bc = bb;
       __builtin_amdgcn_s_waitcnt(0); /* own DMA of entry e, own reads of entry e - 1 */
       __syncthreads();              /* every wave's DMA of e has landed, every wave is done with e - 1 */
-      if (e + 1 < e1) {
-        blocks(e + 1, ab, bb);
-        dma(nxt, ab, bb);
-      }
+      /* entry e + 1 into the other buffer; the last entry of the run loads its own blocks again (unused).
+         The loop has no other global loads, so nothing waits for this DMA before the next entry */
+      blocks(e + 1 < e1 ? e + 1 : e, ab, bb);
+      dma(nxt, ab, bb);
       if (0 <= ac && 0 <= bc) {
+        /* exponents from LDS: row i of A (SNONE past m), columns 32 wj + 8 q + 4 (lane / 32) + 0..3 of B */
+        const int* const lex = (const int*)(cur + 2 * NS * SPL + wave * SEX);
+        const int exi = lex[i];
+        i4_t exj4[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) exj4[q] = *(const i4_t*)(lex + SB + 32 * wj + 8 * q + 4 * (lane >> 5));
         i4_t fa[NS][2], fb[NS][2];
 #pragma unroll
         for (int pl = 0; pl < NS; ++pl) {
@@ -216,7 +233,6 @@ __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, 
             }
           }
         }
-        const int exi = (i < m ? ea[(long)ac * SB + i] : SNONE);
 #pragma unroll
         for (int r = 0; r < 16; ++r) {
           const int j = 32 * wj + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
@@ -225,7 +241,7 @@ __global__ void __launch_bounds__(256) ozs_stack(const int* __restrict__ stack, 
           for (int L = 2 * NS - 2; 0 <= L; --L) {
             if (L <= cutoff) cd = fma(ldexp(1.0, -7 * (L + 2)), (double)acc[L][r], cd);
           }
-          const int exj = (j < n ? eb[(long)bc * SB + j] : SNONE);
+          const int exj = (j < n ? exj4[r >> 2][r & 3] : SNONE);
           const double v = (SNONE != exi && SNONE != exj) ? ldexp(cd, exi + exj) : 0.0;
           creg[r] = (double)(T)(creg[r] + v);
         }
