@@ -28,6 +28,7 @@ typedef struct lxs_config_t {
   int device;    /* active device, -1 if none */
   int verbosity;
   int profile;   /* LIBXSTREAM_PROFILE */
+  int split;     /* LIBXSTREAM_SMM_SPLIT: C-runs are cut past max(split*bs, 64) entries (0: never) */
   int ncus;      /* compute units of the active device */
   char arch[256]; /* gcnArchName of the active device */
   pthread_mutex_t lock;
@@ -58,5 +59,7 @@ void lxs_learned_load_default(const char* path);
 /** LIBSMM registry lifetime (smm.c) */
 int lxs_smm_init(void);
 void lxs_smm_finalize(void);
+/* C-runs of a launch with bs entries per work-group are cut past this many entries (0: never) */
+int lxs_smm_tsplit(int bs);
 
 #endif /*LXS_INTERNAL_H*/

@@ -29,6 +29,7 @@ typedef struct lxs_smm_args_t {
   int atomic;        /* 1: flush C-runs with atomic adds; 0: plain read-modify-write */
   int pfmt;          /* 0: 3-int records, 1-based offsets; else 6-int DBM task records (m, n, k, a, b, c),
                         0-based offsets (samples/smm/kernels/multiply.cl:96-99) */
+  int tsplit;        /* C-runs longer than this are cut at segment boundaries (<= 0: never) */
 } lxs_smm_args_t;
 
 /** number of LDS bytes the stack kernel needs for this configuration (0 if invalid) */
@@ -58,7 +59,7 @@ typedef struct lxs_batch_stack_t {
   int stack_size, m, n, k;
   int bs;     /* entries per chunk */
   int ti, tj; /* 16x16 output tiles covering the C block */
-  int pad;
+  int tsplit; /* C-runs longer than this are cut at chunk boundaries (<= 0: never) */
 } lxs_batch_stack_t;
 
 /** up to 4 chunks (one per wavefront), or one chunk split over the 4 wavefronts (team != 0) */

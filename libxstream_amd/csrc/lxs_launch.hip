@@ -278,14 +278,14 @@ extern "C" int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, 
 }
 
 extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
-  int kt, bs, atomic, ps, pz, stack_size = a->stack_size, m = a->m, n = a->n, k = a->k;
+  int kt, bs, atomic, ps, pz, tsplit = a->tsplit, stack_size = a->stack_size, m = a->m, n = a->n, k = a->k;
   size_t lds;
   unsigned int nblocks;
   const void* fn;
   void* c;
   const void *pa, *pb;
   const int* stack;
-  void* args[13];
+  void* args[14];
   if (1 == a->variant) {
     const mfma_entry_t* const e = mfma_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt, a->kt);
     if (NULL == e) return LXS_ENOVARIANT;
@@ -325,6 +325,7 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
   args[10] = &atomic;
   args[11] = &ps;
   args[12] = &pz;
+  args[13] = &tsplit;
   return (int)hipLaunchKernel(fn, dim3(nblocks), dim3((unsigned int)a->nt), args, lds, (hipStream_t)hip_stream);
 }
 

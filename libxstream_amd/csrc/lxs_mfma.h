@@ -55,6 +55,16 @@ __device__ __forceinline__ T gload(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
 
+/* Long C-runs are cut into pieces at the segment boundaries of the stack (multiples of the
+   segment size bs, where one work-group's entries begin). Boundary entry i starts a new piece
+   when the entry tsplit places earlier has the same C block ci: in a stack sorted by C, the run
+   through i has then gone on for more than tsplit entries. The two owners that meet at i both
+   evaluate this function of the stack alone, so each piece has exactly one owner; pieces of a cut
+   run update their C block with atomic adds. tsplit <= 0: runs are never cut. */
+__device__ __forceinline__ bool lxs_cut(const int* stack, int ps, int i, int ci, int tsplit) {
+  return 0 < tsplit && tsplit <= i && stack[ps * (i - tsplit) + 2] == ci;
+}
+
 } /* namespace lxs */
 
 #endif /*LXS_MFMA_H*/

@@ -21,6 +21,7 @@
 #define LXS_SMM_S 30000
 /* smallest bs the scaling for shorter stacks starts from (smm_kernel.c:49-51, 466-467) */
 #define LXS_SMM_BS_MIN 32
+#define LXS_SMM_TSPLIT_MIN 64 /* runs of up to this many entries are never cut into pieces */
 
 typedef struct lxs_smm_slot_t {
   int type, m, n, k, used, source;
@@ -582,6 +583,15 @@ LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t
   return result;
 }
 
+/* A C-run is summed by the work-group where it starts, unless it is longer than tsplit entries:
+   then it is cut at work-group boundaries and its pieces are added with atomics, so one long run
+   (many k-blocks into one C block) does not leave a single work-group working alone. Runs of up
+   to LXS_SMM_TSPLIT_MIN entries are never cut: their result is the same for every bs. */
+int lxs_smm_tsplit(int bs) {
+  const int t = lxs_cfg.split * (1 < bs ? bs : 1);
+  return (0 < lxs_cfg.split ? (LXS_SMM_TSPLIT_MIN < t ? t : LXS_SMM_TSPLIT_MIN) : 0);
+}
+
 static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,
   int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream, int param_format) {
   const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
@@ -605,8 +615,7 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   args.nt = config->nt;
   args.bs = config->bs;
   args.kt = config->kt;
-  /* a stack shorter than the one bs was chosen for spreads over more work-groups (smm_kernel.c:463-473);
-     the result does not depend on bs: every C-run is summed by the work-group where it starts */
+  /* a stack shorter than the one bs was chosen for spreads over more work-groups (smm_kernel.c:463-473) */
   if (1 < config->bs && 1 < config->s && stack_size < config->s) {
     const long cbs = (config->bs > LXS_SMM_BS_MIN ? config->bs : LXS_SMM_BS_MIN);
     const long bs = ((long)stack_size * cbs + config->s - 2) / (config->s - 1);
@@ -616,6 +625,7 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
      which is exact-once per element when the stack is grouped by C and correct otherwise */
   args.atomic = (1 == config->flush ? 0 : 1);
   args.pfmt = param_format;
+  args.tsplit = lxs_smm_tsplit(args.bs);
   token = lxs_profile_begin(hs);
   result = lxs_smm_launch(&args, (void*)hs);
   lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,

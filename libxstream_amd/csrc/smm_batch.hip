@@ -11,7 +11,7 @@
  * The per-chunk work is the matrix-core scheme of smm_mfma.hip with runtime shapes: fragments
  * loaded straight into registers and refilled as soon as they have been consumed, C-runs summed in
  * registers (segments of 16 entries) and flushed once with an atomic add (stacks of a batch may
- * share C pools).
+ * share C pools); runs longer than the stack's tsplit are cut at chunk boundaries (lxs_cut).
  */
 #include <hip/hip_runtime.h>
 #include "lxs_kernels.h"
@@ -43,7 +43,10 @@ __device__ void run_chunk(const lxs_batch_stack_t& d, int chunk, int q0, int qst
   for (int base = begin; base < end; base += 64) {
     const int i = base + lane;
     bool head = false;
-    if (i < end) head = (0 == i) || (stack[3 * i + 2] != stack[3 * i - 1]);
+    if (i < end) {
+      head = (0 == i) || (stack[3 * i + 2] != stack[3 * i - 1]) ||
+             (i == begin && lxs::lxs_cut(stack, 3, i, stack[3 * i + 2], d.tsplit));
+    }
     const unsigned long long mask = __ballot(head);
     if (0 != mask) {
       h = base + __ffsll((long long)mask) - 1;
@@ -84,18 +87,28 @@ __device__ void run_chunk(const lxs_batch_stack_t& d, int chunk, int q0, int qst
   int c1 = (have1 ? stack[3 * e + 5] : -1);
   int a2 = (have2 ? stack[3 * e + 6] - 1 : a0), b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
   int c2 = (have2 ? stack[3 * e + 8] : -1);
+  /* a long run is cut at the chunk boundaries past end (lxs_cut); nbd: the next one */
+  int nbd = end;
+  const auto cut_at = [&](int i, int ci) -> bool {
+    if (i != nbd) return false;
+    nbd += d.bs;
+    if (i >= stack_size) return false;
+    return lxs::lxs_cut(stack, 3, i, ci, d.tsplit);
+  };
+  bool cut1 = cut_at(e + 1, c1);
+  bool cut2 = cut_at(e + 2, c2);
   int s = 0;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
     if (kb < KS) LXS_LOAD_KSTEP(kb, a0, b0, kb);
   }
-  for (;;) { /* one C-run per iteration */
+  for (;;) { /* one C-run (or piece of one) per iteration */
     int nent = 0;
     bool run_ends, own_next;
     for (;;) { /* one unit (entry, k-stage) per iteration */
       const bool same_entry = (s + 1 < NS);
-      run_ends = !same_entry && (!have1 || c1 != c0);
-      own_next = have1 && (c1 == c0 || e + 1 < end);
+      run_ends = !same_entry && (!have1 || c1 != c0 || cut1);
+      own_next = have1 && ((c1 == c0 && !cut1) || e + 1 < end);
       const int na = ((same_entry || !own_next) ? a0 : a1), nb = ((same_entry || !own_next) ? b0 : b1);
       const int ns = (same_entry ? s + 1 : 0);
 #pragma unroll
@@ -126,11 +139,13 @@ __device__ void run_chunk(const lxs_batch_stack_t& d, int chunk, int q0, int qst
       b1 = b2;
       c1 = c2;
       have1 = have2;
+      cut1 = cut2;
       s = 0;
       have2 = (e + 2 < stack_size);
       a2 = (have2 ? stack[3 * e + 6] - 1 : a0);
       b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
       c2 = (have2 ? stack[3 * e + 8] : -1);
+      cut2 = cut_at(e + 2, c2);
     }
     {
       T* const cblk = cdata + (c0 - 1);
@@ -155,11 +170,13 @@ __device__ void run_chunk(const lxs_batch_stack_t& d, int chunk, int q0, int qst
     b1 = b2;
     c1 = c2;
     have1 = have2;
+    cut1 = cut2;
     s = 0;
     have2 = (e + 2 < stack_size);
     a2 = (have2 ? stack[3 * e + 6] - 1 : a0);
     b2 = (have2 ? stack[3 * e + 7] - 1 : b0);
     c2 = (have2 ? stack[3 * e + 8] : -1);
+    cut2 = cut_at(e + 2, c2);
   }
 #undef LXS_LOAD_KSTEP
 }

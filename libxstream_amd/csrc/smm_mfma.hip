@@ -18,7 +18,8 @@
  *   - stack indices are wave-uniform scalar loads, fetched one entry ahead;
  *   - C-run ownership as in smm_stack.hip: a work-group owns the runs whose first entry lies in
  *     [w*bs, (w+1)*bs), sums the run in registers (segments of 16 entries in the matrix-core
- *     accumulator, added to a run total) and flushes it once;
+ *     accumulator, added to a run total) and flushes it once; a run longer than tsplit entries is
+ *     cut into pieces at the segment boundaries (lxs_cut), so no work-group walks a long run alone;
  *   - fragment loads carry no branches and no selects on loaded values (padding lanes pick their
  *     address instead), so the compiler can count outstanding loads exactly.
  *
@@ -44,12 +45,13 @@ struct mfma_entry_t {
 };
 
 using lxs::gload;
+using lxs::lxs_cut;
 using lxs::mfma;
 
 template <typename T, int TI, int TJ, int NWI, int KB, int MC, int NC, int KC>
 __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
-  int kt_unused, int atomic, int ps, int pz) {
+  int kt_unused, int atomic, int ps, int pz, int tsplit) {
   typedef typename mfma<T>::acc_t acc_t;
   constexpr int RI = TI / NWI; /* tile rows per wavefront */
   /* summation order: the matrix core accumulates up to SEG_LEN entries in prod, which is then
@@ -66,12 +68,15 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
 
   const int begin = blockIdx.x * bs;
   const int end = min(begin + bs, stack_size);
-  /* first run head in [begin, end) */
+  /* first piece head in [begin, end): a run head, or begin where it cuts a long run */
   int h = end;
   for (int base = begin; base < end; base += 64) {
     const int i = base + lane;
     bool head = false;
-    if (i < end) head = (0 == i) || (stack[ps * i + 2] != stack[ps * (i - 1) + 2]);
+    if (i < end) {
+      head = (0 == i) || (stack[ps * i + 2] != stack[ps * (i - 1) + 2]) ||
+             (i == begin && lxs_cut(stack, ps, i, stack[ps * i + 2], tsplit));
+    }
     const unsigned long long mask = __ballot(head);
     if (0 != mask) {
       h = base + __ffsll((long long)mask) - 1;
@@ -120,6 +125,19 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
   int c1 = (have1 ? stack[ps * (e + 1) + 2] : -1);
   int a2 = (have2 ? stack[ps * (e + 2) + 0] - pz : a0), b2 = (have2 ? stack[ps * (e + 2) + 1] - pz : b0);
   int c2 = (have2 ? stack[ps * (e + 2) + 2] : -1);
+  /* cuts past end: entry i >= end starts a new piece of a run when i is a segment boundary (next
+     boundary: nbd) and lxs_cut holds; entries are checked once each, in order (cut1: e+1, cut2: e+2) */
+  int nbd = end;
+  const auto cut_at = [&](int i, int ci) -> bool {
+    if (i != nbd) return false;
+    nbd += bs;
+    if (i >= stack_size) return false;
+    return lxs_cut(stack, ps, i, ci, tsplit);
+  };
+  bool cut1 = cut_at(e + 1, c1);
+  bool cut2 = cut_at(e + 2, c2);
+  /* a piece that starts or ends inside a run shares its C block with another work-group */
+  bool split = (0 < e && stack[ps * (e - 1) + 2] == c0);
   int s = 0;
 #pragma unroll
   for (int kb = 0; kb < KB; ++kb) {
@@ -131,8 +149,8 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
     bool run_ends, own_next;
     for (;;) { /* one unit (entry, k-stage) per iteration; no memory traffic but the fragment loads */
       const bool same_entry = (s + 1 < NS);
-      run_ends = !same_entry && (!have1 || c1 != c0);
-      own_next = have1 && (c1 == c0 || e + 1 < end);
+      run_ends = !same_entry && (!have1 || c1 != c0 || cut1);
+      own_next = have1 && ((c1 == c0 && !cut1) || e + 1 < end);
       /* the next unit of work; without one, reload the current entry (in bounds, never used) */
       const int na = ((same_entry || !own_next) ? a0 : a1), nb = ((same_entry || !own_next) ? b0 : b1);
       const int ns = (same_entry ? s + 1 : 0);
@@ -172,15 +190,19 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
       b1 = b2;
       c1 = c2;
       have1 = have2;
+      cut1 = cut2;
       s = 0;
       have2 = (e + 2 < stack_size);
       a2 = (have2 ? stack[ps * (e + 2) + 0] - pz : a0);
       b2 = (have2 ? stack[ps * (e + 2) + 1] - pz : b0);
       c2 = (have2 ? stack[ps * (e + 2) + 2] : -1);
+      cut2 = cut_at(e + 2, c2);
     }
-    /* run done: fold and flush C once (read-modify-write, or atomic add for unsorted stacks) */
+    /* piece done: fold and flush C once (read-modify-write; atomic add for unsorted stacks and for
+       pieces of a cut run) */
     {
       T* const cblk = cdata + (c0 - pz);
+      const bool flush_atomic = atomic || split || (have1 && c1 == c0);
 #pragma unroll
       for (int ri = 0; ri < RI; ++ri) {
 #pragma unroll
@@ -193,7 +215,7 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
             const int i = 16 * (ti0 + ri) + mfma<T>::row(lane, r);
             if (i < M && j < N) {
               T* const p = cblk + j * M + i;
-              if (atomic) {
+              if (flush_atomic) {
                 unsafeAtomicAdd(p, sum[r]);
               }
               else {
@@ -213,11 +235,14 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
     b1 = b2;
     c1 = c2;
     have1 = have2;
+    cut1 = cut2;
+    split = false;
     s = 0;
     have2 = (e + 2 < stack_size);
     a2 = (have2 ? stack[ps * (e + 2) + 0] - pz : a0);
     b2 = (have2 ? stack[ps * (e + 2) + 1] - pz : b0);
     c2 = (have2 ? stack[ps * (e + 2) + 2] : -1);
+    cut2 = cut_at(e + 2, c2);
   }
 #undef LXS_LOAD_KSTEP
 }

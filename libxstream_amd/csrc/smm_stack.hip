@@ -9,7 +9,8 @@
  *
  * Work decomposition (new design, not the reference's mini-batch + atomic flush):
  *   - work-group w owns every C-run (maximal sequence of equal c) whose first entry lies in
- *     [w*bs, (w+1)*bs) and runs it to its end, so a C block is flushed once per run;
+ *     [w*bs, (w+1)*bs) and runs it to its end, so a C block is flushed once per run; a run longer
+ *     than tsplit entries is cut at segment boundaries (lxs_cut) and its pieces flush atomically;
  *   - the threads cover one C block with TM x TN microtiles (ntiles threads); NT/ntiles such
  *     groups take the run's entries round-robin, each staging its A and B block in LDS;
  *   - partial C sums stay in registers for the whole run and are reduced across groups through
@@ -20,6 +21,7 @@
  */
 #include <hip/hip_runtime.h>
 #include "lxs_kernels.h"
+#include "lxs_mfma.h"
 
 #if !defined(LXS_REAL)
 #  define LXS_REAL double
@@ -51,7 +53,7 @@ __device__ __forceinline__ void flush_one(T* p, T v, int atomic) {
 template <typename T, int TM, int TN, int NT, int MC, int NC, int KC, int KTC>
 __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
-  int kt_rt, int atomic, int ps, int pz) {
+  int kt_rt, int atomic, int ps, int pz, int tsplit) {
   const int M = (0 < MC ? MC : m_rt), N = (0 < NC ? NC : n_rt), K = (0 < KC ? KC : k_rt);
   const int KT = (0 < KTC ? KTC : kt_rt);
   constexpr bool KEVEN = (0 < KC && 0 < KTC && 0 == (KC % KTC));
@@ -77,9 +79,13 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
   for (int x = tid; x < G * slab; x += NT) smem[x] = T(0);
   if (0 == tid) s_idx = end;
   __syncthreads();
-  /* first run head in [begin, end): entry i starts a run if its c differs from entry i-1 */
+  /* first piece head in [begin, end): entry i starts a run if its c differs from entry i-1, and
+     begin starts a piece where it cuts a long run */
   for (int i = begin + tid; i < end; i += NT) {
-    if (0 == i || stack[ps * i + 2] != stack[ps * (i - 1) + 2]) atomicMin(&s_idx, i);
+    const int ci = stack[ps * i + 2];
+    if (0 == i || ci != stack[ps * (i - 1) + 2] || (i == begin && lxs::lxs_cut(stack, ps, i, ci, tsplit))) {
+      atomicMin(&s_idx, i);
+    }
   }
   __syncthreads();
   int h = s_idx;
@@ -88,13 +94,17 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
 
   for (;;) {
     const int c0 = stack[ps * h + 2];
-    /* end of this run: first entry after h with a different c (or stack_size) */
+    /* end of this piece: first entry after h with a different c, or a segment boundary past end
+       that cuts the run (or stack_size) */
     if (0 == tid) s_idx = stack_size;
     __syncthreads();
     int r_end;
     for (int base = h + 1;; base += NT) {
       const int i = base + tid;
-      if (i < stack_size && stack[ps * i + 2] != c0) atomicMin(&s_idx, i);
+      if (i < stack_size && (stack[ps * i + 2] != c0 ||
+                              (end <= i && 0 == (i - end) % bs && lxs::lxs_cut(stack, ps, i, c0, tsplit)))) {
+        atomicMin(&s_idx, i);
+      }
       __syncthreads();
       r_end = s_idx;
       __syncthreads();
@@ -196,6 +206,9 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
     }
 
     T* const cblk = cdata + (c0 - pz);
+    /* a piece of a cut run shares its C block with another work-group */
+    const int fl = (atomic || (0 < h && stack[ps * (h - 1) + 2] == c0) ||
+                    (r_end < stack_size && stack[ps * r_end + 2] == c0)) ? 1 : 0;
     if (1 == G) {
       if (active) {
 #pragma unroll
@@ -203,7 +216,7 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
 #pragma unroll
           for (int r = 0; r < TM; ++r) {
             const int m = m0 + r, n = n0 + c;
-            if (m < M && n < N) flush_one(cblk + n * M + m, acc[r][c], atomic);
+            if (m < M && n < N) flush_one(cblk + n * M + m, acc[r][c], fl);
           }
         }
       }
@@ -225,7 +238,7 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
         const int tt = x / (TM * TN), rc = x - tt * (TM * TN);
         const int r = rc / TN, c = rc - r * TN;
         const int m = (tt % tiles_m) * TM + r, n = (tt / tiles_m) * TN + c;
-        if (m < M && n < N) flush_one(cblk + n * M + m, s, atomic);
+        if (m < M && n < N) flush_one(cblk + n * M + m, s, fl);
       }
       __syncthreads();
     }
@@ -245,7 +258,7 @@ __global__ void __launch_bounds__(NT) smm_stack(T* __restrict__ cdata, const T* 
 template <typename T, int NT, int MC, int NC, int KC>
 __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
-  int kt_rt, int atomic, int ps, int pz) {
+  int kt_rt, int atomic, int ps, int pz, int tsplit) {
   constexpr int MN = MC * NC, P = 64 / MN, W = NT / 64;
   static_assert(0 < MN && MN <= 64 && 0 == NT % 64, "one C block per wavefront slot");
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -257,10 +270,16 @@ __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* 
   const int wend = min(min(wbeg + bsw, (int)(blockIdx.x + 1) * bs), stack_size);
   (void)m_rt, (void)n_rt, (void)k_rt, (void)kt_rt;
   if (wbeg >= wend) return;
-  int h = wbeg; /* the first run that starts in [wbeg, wend), 64 entries per step */
+  /* pieces: runs are cut at the wavefront segment starts q*bs + w*bsw (lxs_cut) */
+  const auto boundary = [&](int x) { const int r = x % bs; return 0 == r % bsw; };
+  int h = wbeg; /* the first piece that starts in [wbeg, wend), 64 entries per step */
   for (;;) {
     const int x = h + lane;
-    const bool start = (x < wend) && (0 == x || stack[ps * x + 2] != stack[ps * (x - 1) + 2]);
+    bool start = false;
+    if (x < wend) {
+      const int cx = stack[ps * x + 2];
+      start = (0 == x || cx != stack[ps * (x - 1) + 2] || (x == wbeg && lxs::lxs_cut(stack, ps, x, cx, tsplit)));
+    }
     const unsigned long long bal = __ballot(start);
     if (0 != bal) {
       h += __ffsll((long long)bal) - 1;
@@ -271,10 +290,14 @@ __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* 
   }
   while (h < wend) {
     const int c0 = stack[ps * h + 2];
-    int r_end = h + 1; /* the run's end, 64 entries per step */
+    int r_end = h + 1; /* the piece's end, 64 entries per step */
     for (;;) {
       const int x = r_end + lane;
-      const unsigned long long bal = __ballot(x >= stack_size || stack[ps * x + 2] != c0);
+      bool stop = true;
+      if (x < stack_size) {
+        stop = (stack[ps * x + 2] != c0) || (wend <= x && boundary(x) && lxs::lxs_cut(stack, ps, x, c0, tsplit));
+      }
+      const unsigned long long bal = __ballot(stop);
       if (0 != bal) {
         r_end += __ffsll((long long)bal) - 1;
         break;
@@ -316,7 +339,9 @@ __global__ void __launch_bounds__(NT) smm_lanes(T* __restrict__ cdata, const T* 
     T tot = acc;
 #pragma unroll
     for (int s = 1; s < P; ++s) tot += __shfl(acc, (lane + s * MN) & 63);
-    if (0 == slot) flush_one(cdata + (c0 - pz) + i + j * MC, tot, atomic);
+    const int fl = (atomic || (0 < h && stack[ps * (h - 1) + 2] == c0) ||
+                    (r_end < stack_size && stack[ps * r_end + 2] == c0)) ? 1 : 0;
+    if (0 == slot) flush_one(cdata + (c0 - pz) + i + j * MC, tot, fl);
     h = r_end;
   }
 }

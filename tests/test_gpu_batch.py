@@ -119,6 +119,29 @@ def test_batch_matches_process(stream):
         res.free()
 
 
+def test_batch_long_runs(stream):
+    """Runs much longer than a chunk are cut at chunk boundaries; pieces add atomically."""
+    from libxstream_amd.workload import make_stack_workload
+
+    lengths = [[2500, 3, 700, 40], [1, 1200, 64, 65, 1], [3000]]
+    works = []
+    for j, (m, n, k) in enumerate([(23, 23, 23), (64, 64, 64), (5, 7, 9)]):
+        w = make_stack_workload(m, n, k, sum(lengths[j]), nc=len(lengths[j]), dtype=np.float64, seed=31 + j,
+                                c_init="uniform")
+        w.stack[:, 2] = np.repeat(np.arange(len(lengths[j]), dtype=np.int32), lengths[j]) * (m * n) + 1
+        works.append(w)
+    res = Resident(works, stream)
+    try:
+        batch = lx.Batch(lx.REAL_8, res.items, lx.BatchParams(chunk_elements=8 * 2 * 23 * 23, blocks_per_cu=8))
+        batch.process(stream)
+        stream.sync()
+        expected = [oracle.stack_dd(w.m, w.n, w.k, w.stack, w.a, w.b, c_in=w.c.ravel()) for w in works]
+        check_all(res, expected)
+        batch.close()
+    finally:
+        res.free()
+
+
 def test_batch_rejects_large_blocks(stream):
     w = np.zeros(8)
     d = lx.DeviceBuffer.from_array(w)

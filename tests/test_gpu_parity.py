@@ -81,7 +81,7 @@ def test_chunk_sizes(stream, bs):
 @pytest.mark.parametrize("shape", [(23, 23, 23), (5, 5, 5), (4, 9, 22), (45, 45, 13)], ids=lambda s: "%dx%dx%d" % s)
 def test_short_stack_bs_scaling(stream, shape, dtype):
     """A stack shorter than the configuration's S launches with bs scaled down (smm_kernel.c:463-473):
-    more work-groups, the same C bit for bit (every C-run is summed where it starts)."""
+    more work-groups, the same C bit for bit (every run of up to 64 entries is summed where it starts)."""
     m, n, k = shape
     w = make_stack_workload(m, n, k, 1900, dtype=dtype, seed=_seed("short", shape), c_init="uniform")
     tid = lx.typeid(dtype)
@@ -110,6 +110,7 @@ def test_flush_modes_bit_identical(stream, flush):
 
 
 def test_deterministic(stream):
+    """Runs of at most 64 entries are never cut, so each C block is flushed once: the same bits every time."""
     m = n = k = 32
     w = make_stack_workload(m, n, k, 5000, dtype=np.float64, seed=_seed("det"), c_init="uniform")
     a = run_stack(stream, w)
@@ -143,6 +144,46 @@ def test_run_structure(stream, case):
     assert oracle_error(w, out)[0] <= 1e-13
 
 
+def _with_runs(w, lengths):
+    """w's stack re-cut into C-runs of the given lengths (sorted by C, lengths sum to the stack size)."""
+    assert sum(lengths) == w.stack_size and len(lengths) <= w.nc
+    stack = w.stack.copy()
+    stack[:, 2] = np.repeat(np.arange(len(lengths), dtype=np.int32), lengths) * (w.m * w.n) + 1
+    return stack
+
+
+LONG_RUNS = [1500, 3, 7, 40, 1, 64, 65, 33, 900] + [17] * 40 + [707]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", [(23, 23, 23), (5, 5, 5), (13, 13, 13), (4, 9, 22)], ids=lambda s: "%dx%dx%d" % s)
+def test_long_runs_cut(stream, shape, dtype):
+    """Runs far longer than bs (one C block fed by many k-blocks, or the last C block of
+    acc_bench's INIT_STACK taking the remainder) are cut into pieces at work-group boundaries and
+    the pieces added atomically: every kernel variant, both flush modes, several bs."""
+    m, n, k = shape
+    tid = lx.typeid(dtype)
+    S = sum(LONG_RUNS)
+    w = make_stack_workload(m, n, k, S, nc=len(LONG_RUNS), dtype=dtype, seed=_seed("long", shape), c_init="uniform")
+    stack = _with_runs(w, LONG_RUNS)
+    checked = 0
+    for variant, tm, tn, nt, vkt in lx.kernel_variants(tid):
+        for bs in (5, 16, 64):
+            for flush in (1, 2):
+                cfg = lx.SmmConfig(variant=variant, tm=tm, tn=tn, nt=nt, bs=bs, kt=(vkt if variant else 0),
+                                   flush=flush)
+                if not lx.config_valid(tid, m, n, k, cfg):
+                    continue
+                out = run_stack(stream, w, config=cfg, device_transpose=False, stack=stack)
+                err = oracle_error(w, out, stack=stack)[0]
+                assert err <= tol(dtype), "v=%d tm=%d tn=%d nt=%d bs=%d flush=%d: %g" % (
+                    variant, tm, tn, nt, bs, flush, err)
+                checked += 1
+    assert checked >= 6
+    out = run_stack(stream, w, stack=stack)  # the tuned or default configuration
+    assert oracle_error(w, out, stack=stack)[0] <= tol(dtype)
+
+
 def test_long_k_slices(stream):
     """k beyond max_kernel_dim (allowed: the gate bounds m*n only) runs in k-slices."""
     m, n, k = 20, 24, 300
@@ -172,7 +213,8 @@ def test_full_size_23_30k_oracle(stream):
 @pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
 @pytest.mark.parametrize("stack_size", [100000, 500000])
 def test_full_size_checksum(stream, dtype, stack_size):
-    """Large stacks: size-independent checks (checksum of the update, bitwise determinism)."""
+    """Large stacks: size-independent checks (checksum of the update; bitwise determinism for every
+    C block whose run has at most 64 entries, the others are cut into pieces added atomically)."""
     m = n = k = 23
     w = make_stack_workload(m, n, k, stack_size, dtype=dtype, seed=_seed("big", stack_size))
     out = run_stack(stream, w)
@@ -181,7 +223,10 @@ def test_full_size_checksum(stream, dtype, stack_size):
     eps = 1e-12 if dtype == np.float64 else 1e-5
     assert abs(got - total) <= eps * scale, (got, total, scale)
     again = run_stack(stream, w)
-    assert np.array_equal(out, again)
+    runs = np.bincount((w.stack[:, 2] - 1) // (m * n), minlength=w.nc)
+    short = runs <= 64
+    assert np.array_equal(out[short], again[short])
+    assert np.allclose(out[~short], again[~short], rtol=0, atol=eps * max(1.0, float(np.abs(out).max())) * 64)
 
 
 def test_transpose_parity(stream):
