@@ -140,6 +140,49 @@ def copyin_section(lx, w, stream, reps=5):
             "gibs": round(size / (sec * (1 << 30)), 2), "pinned": True}
 
 
+def transfer_section(lx, m, n, k, dtype, stream, nstacks=16, rounds=3):
+    """Per-call stack copy (DBCSR's pattern, libxstream_mem.c:1094-1127): nstacks different 30000-entry
+    stacks over resident 2000/2000/1000-block pools, each copied from pinned staging memory and then
+    multiplied. Serialised: one stream, one slot (copy, multiply, copy, ...). Overlapped: the copies on
+    their own stream through 2 slots (and 4, for comparison), so the copy of stack i+1 runs during the
+    multiply of stack i."""
+    from libxstream_amd.transfer import StackPipeline
+    from libxstream_amd.workload import make_stack_workload
+
+    w = make_stack_workload(m, n, k, 30000, nc=1000, na=2000, nb=2000, dtype=dtype, seed=25071975)
+    stacks = [make_stack_workload(m, n, k, 30000, nc=1000, na=2000, nb=2000, dtype=dtype, seed=25071975 + i).stack
+              for i in range(nstacks)]
+    res = Resident(lx, w, stream)
+    upload = lx.Stream("upload")
+    out = {"stacks": nstacks, "entries": 30000, "stack_bytes": int(stacks[0].nbytes)}
+    try:
+        def launch(dev, entries):
+            rc = lx.process(dev, entries, res.tid, res.a, res.b, res.c, m, n, k, stream)
+            assert rc == 0, "libsmm_acc_process returned %d" % rc
+
+        for name, pipe in (("serialised", StackPipeline(30000, stream, None, nslots=1)),
+                           ("overlapped", StackPipeline(30000, stream, upload, nslots=2)),
+                           ("overlapped_4slots", StackPipeline(30000, stream, upload, nslots=4))):
+            try:
+                for st in stacks[:2]:  # warm-up
+                    pipe.submit(st, launch)
+                pipe.sync()
+                t0 = time.perf_counter()
+                for _ in range(rounds):
+                    for st in stacks:
+                        pipe.submit(st, launch)
+                pipe.sync()
+                out[name + "_ms_per_stack"] = round((time.perf_counter() - t0) * 1e3 / (rounds * nstacks), 4)
+            finally:
+                pipe.close()
+        out["kernel_gflops_overlapped"] = round(w.flops() / (out["overlapped_ms_per_stack"] * 1e-3) * 1e-9, 1)
+        out["overlap_gain"] = round(out["serialised_ms_per_stack"] / out["overlapped_ms_per_stack"], 3)
+    finally:
+        upload.close()
+        res.free()
+    return out
+
+
 def transpose_section(lx, res, stream, pairs=10):
     """libsmm_acc_transpose over the whole B pool (smm_trans.c:48-228): kernel time from the launch records
     and HBM rate over the reference's bytes, 2 S m n ts per launch (smm_trans.c:208-212). Transposes go
@@ -684,6 +727,7 @@ def main():
         small = {"stack_size": 30000, "blocks_a_b_c": [2000, 2000, 1000], "ms_per_step": round(s_step, 5),
                  "gflops": round(ws.flops() / (s_step * 1e-3) * 1e-9, 1), "kernel_ms": round(s_kernel, 5),
                  "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
+    transfer = transfer_section(lx, m, n, k, w.dtype, stream)
     batch = None if args.no_batch else batched(lx, stream, args.steps, args.warmup)
     ozaki = None if args.no_ozaki else ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
     if ozaki is not None:
@@ -742,6 +786,7 @@ def main():
         "stack_30000": small,
         "transpose_b_pool": trans,
         "copy_in_pinned": copyin,
+        "stack_transfer": transfer,
         "batched_mixed": batch,
         "ozaki": ozaki,
         "learned": learned_rep,

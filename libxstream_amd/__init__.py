@@ -101,6 +101,7 @@ def _declare(L):
         "libxstream_stream_destroy": (i, [vp]),
         "libxstream_stream_sync": (i, [vp]),
         "libxstream_stream_native": (i, [vp, pvp]),
+        "libxstream_stream_wait_event": (i, [vp, vp]),
         "libxstream_event_create": (i, [pvp]),
         "libxstream_event_destroy": (i, [vp]),
         "libxstream_event_record": (i, [vp, vp]),
@@ -246,6 +247,10 @@ class Stream:
     def sync(self):
         check(lib().libxstream_stream_sync(self.handle), "stream_sync")
 
+    def wait(self, event):
+        """Later work on this stream waits for event (libxstream_stream_wait_event)."""
+        check(lib().libxstream_stream_wait_event(self.handle, event.handle), "stream_wait_event")
+
     def native(self):
         h = ctypes.c_void_p()
         check(lib().libxstream_stream_native(self.handle, ctypes.byref(h)))
@@ -334,6 +339,12 @@ class HostBuffer:
         self.ptr = p.value
         self.array = np.ctypeslib.as_array((ctypes.c_char * max(self.nbytes, 1)).from_address(self.ptr))
         self.array = self.array.view(np.uint8)[:self.nbytes].view(dtype)
+
+    def copy_to(self, dev, stream, nbytes=None):
+        """Asynchronous host-to-device copy on stream (pinned source, libxstream_mem_copy_h2d)."""
+        n = self.nbytes if nbytes is None else int(nbytes)
+        assert n <= self.nbytes and n <= dev.nbytes
+        check(lib().libxstream_mem_copy_h2d(self.ptr, dev.ptr, n, stream.handle), "copy_h2d")
 
     def free(self):
         if self.ptr:

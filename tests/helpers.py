@@ -69,3 +69,11 @@ def checksum(w, stack=None):
     scale = np.einsum("ij,ij->i", np.abs(a64).reshape(w.na, k, m).sum(axis=2)[ai],
                       np.abs(b64).reshape(w.nb, n, k).sum(axis=1)[bi])
     return float(np.sum(total)), float(np.sum(scale))
+
+
+def uncut_blocks(w, stack=None, limit=64):
+    """C blocks whose run has at most `limit` entries: never cut into pieces, so their bits do not
+    depend on bs or on the order in which atomics land."""
+    stack = w.stack if stack is None else stack
+    runs = np.bincount((np.asarray(stack)[:, 2] - 1) // (w.m * w.n), minlength=w.nc)
+    return runs <= limit

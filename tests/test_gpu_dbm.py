@@ -10,7 +10,7 @@ import pytest
 
 import libxstream_amd as lx
 from libxstream_amd.workload import make_stack_workload
-from tests.helpers import oracle_error, tol
+from tests.helpers import oracle_error, tol, uncut_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -67,13 +67,21 @@ def run_both(stream, w, stack=None, event=None):
             buf.free()
 
 
+def same_bits(w, out3, out6):
+    """Both record formats run the same kernels: identical bits in every C block whose run is never cut
+    (at most 64 entries); a cut run's pieces add atomically in either order."""
+    short = uncut_blocks(w)
+    bound = (1e-12 if out3.dtype == np.float64 else 1e-5) * max(1.0, float(np.abs(out3).max()))
+    return np.array_equal(out3[short], out6[short]) and np.allclose(out3[~short], out6[~short], rtol=0, atol=bound)
+
+
 @pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
 @pytest.mark.parametrize("shape", SHAPES, ids=["%dx%dx%d" % s for s in SHAPES])
 def test_dbm_records(stream, shape, dtype):
     m, n, k = shape
     w = make_stack_workload(m, n, k, 2000, dtype=dtype, seed=hash((shape, "dbm")) % 2**31, c_init="uniform")
     out3, out6 = run_both(stream, w)
-    assert np.array_equal(out3, out6), "DBM records differ from triplets"
+    assert same_bits(w, out3, out6), "DBM records differ from triplets"
     err = oracle_error(w, out6)
     assert err[0] <= tol(dtype), "max normalised error %g" % err[0]
 
@@ -96,12 +104,12 @@ def test_dbm_every_variant(stream, dtype):
                         continue
                     lx.config_set(tid, m, n, k, cfg)
                     out3, out6 = run_both(stream, w)
-                    assert np.array_equal(out3, out6), "%s %r" % (shape, cfg)
+                    assert same_bits(w, out3, out6), "%s %r" % (shape, cfg)
                     checked += 1
             # the kernel compiled for this shape (5^3: wavefront-per-run; others: the tuned one)
             lx.config_set(tid, m, n, k, saved)
             out3, out6 = run_both(stream, w)
-            assert np.array_equal(out3, out6)
+            assert same_bits(w, out3, out6)
             assert oracle_error(w, out6)[0] <= tol(dtype)
         finally:
             lx.config_set(tid, m, n, k, saved)
@@ -128,7 +136,7 @@ def test_dbm_event(stream):
         out3, out6 = run_both(stream, w, event=ev)
     finally:
         ev.close()
-    assert np.array_equal(out3, out6)
+    assert same_bits(w, out3, out6)
 
 
 def test_dbm_fallback_codes(stream):
