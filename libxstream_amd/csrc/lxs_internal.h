@@ -28,7 +28,8 @@ typedef struct lxs_config_t {
   int device;    /* active device, -1 if none */
   int verbosity;
   int profile;   /* LIBXSTREAM_PROFILE */
-  int split;     /* LIBXSTREAM_SMM_SPLIT: C-runs are cut past max(split*bs, 64) entries (0: never) */
+  int split;     /* LIBXSTREAM_SMM_SPLIT: C-runs are cut past max(split*bs, split_min) entries (0: never) */
+  int split_min; /* LIBXSTREAM_SMM_SPLIT_MIN: runs of up to this many entries are never cut (default 1) */
   int ncus;      /* compute units of the active device */
   char arch[256]; /* gcnArchName of the active device */
   pthread_mutex_t lock;

@@ -21,7 +21,6 @@
 #define LXS_SMM_S 30000
 /* smallest bs the scaling for shorter stacks starts from (smm_kernel.c:49-51, 466-467) */
 #define LXS_SMM_BS_MIN 32
-#define LXS_SMM_TSPLIT_MIN 64 /* runs of up to this many entries are never cut into pieces */
 
 typedef struct lxs_smm_slot_t {
   int type, m, n, k, used, source;
@@ -583,13 +582,15 @@ LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t
   return result;
 }
 
-/* A C-run is summed by the work-group where it starts, unless it is longer than tsplit entries:
-   then it is cut at work-group boundaries and its pieces are added with atomics, so one long run
-   (many k-blocks into one C block) does not leave a single work-group working alone. Runs of up
-   to LXS_SMM_TSPLIT_MIN entries are never cut: their result is the same for every bs. */
+/* A C-run is summed by the work-group where it starts, unless it is longer than tsplit = 2 bs entries
+   (LIBXSTREAM_SMM_SPLIT x bs): then it is cut at work-group boundaries and its pieces are added with
+   atomics, so one long run (many k-blocks into one C block) does not leave a single work-group working
+   alone, and a short stack (bs scaled down) spreads its runs over many work-groups. Runs of up to
+   LIBXSTREAM_SMM_SPLIT_MIN entries are never cut (default 1; 64 keeps such runs bit-identical for
+   every bs, at the price of short stacks). */
 int lxs_smm_tsplit(int bs) {
-  const int t = lxs_cfg.split * (1 < bs ? bs : 1);
-  return (0 < lxs_cfg.split ? (LXS_SMM_TSPLIT_MIN < t ? t : LXS_SMM_TSPLIT_MIN) : 0);
+  const int t = lxs_cfg.split * (1 < bs ? bs : 1), floor = (0 < lxs_cfg.split_min ? lxs_cfg.split_min : 1);
+  return (0 < lxs_cfg.split ? (floor < t ? t : floor) : 0);
 }
 
 static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,

@@ -71,9 +71,9 @@ def checksum(w, stack=None):
     return float(np.sum(total)), float(np.sum(scale))
 
 
-def uncut_blocks(w, stack=None, limit=64):
-    """C blocks whose run has at most `limit` entries: never cut into pieces, so their bits do not
-    depend on bs or on the order in which atomics land."""
+def uncut_blocks(w, stack=None, limit=32):
+    """C blocks whose run has at most `limit` entries (2 bs of the launch: never cut into pieces), so
+    their bits do not depend on the order in which atomics land."""
     stack = w.stack if stack is None else stack
     runs = np.bincount((np.asarray(stack)[:, 2] - 1) // (w.m * w.n), minlength=w.nc)
     return runs <= limit

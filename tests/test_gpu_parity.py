@@ -81,8 +81,8 @@ def test_chunk_sizes(stream, bs):
 @pytest.mark.parametrize("shape", [(23, 23, 23), (5, 5, 5), (4, 9, 22), (45, 45, 13)], ids=lambda s: "%dx%dx%d" % s)
 def test_short_stack_bs_scaling(stream, shape, dtype):
     """A stack shorter than the configuration's S launches with bs scaled down (smm_kernel.c:463-473):
-    more work-groups, the same C bit for bit in every block whose run has at most 64 entries (summed where
-    it starts); the generator's tail run may be longer, and is cut into pieces."""
+    more work-groups, and runs longer than twice the scaled bs are cut into pieces; the same C within
+    the parity bar."""
     m, n, k = shape
     w = make_stack_workload(m, n, k, 1900, dtype=dtype, seed=_seed("short", shape), c_init="uniform")
     tid = lx.typeid(dtype)
@@ -92,34 +92,36 @@ def test_short_stack_bs_scaling(stream, shape, dtype):
     scaled = run_stack(stream, w, config=cfg, device_transpose=False)
     cfg.s = 0
     plain = run_stack(stream, w, config=cfg, device_transpose=False)
-    short = uncut_blocks(w)
-    assert np.array_equal(scaled[short], plain[short])
     assert oracle_error(w, scaled)[0] <= tol(dtype)
     assert oracle_error(w, plain)[0] <= tol(dtype)
 
 
 @pytest.mark.parametrize("flush", [1, 2])
 def test_flush_modes_bit_identical(stream, flush):
-    """Sorted stack: the read-modify-write and atomic flush give the same bits (one add per element)."""
+    """Sorted stack whose runs are never cut (bs 16 unscaled: runs of up to 32 entries stay whole): the
+    read-modify-write and atomic flush give the same bits (one add per element)."""
     m = n = k = 23
-    w = make_stack_workload(m, n, k, 6000, dtype=np.float64, seed=_seed("flush"), c_init="uniform")
+    w = make_stack_workload(m, n, k, 6000, dtype=np.float64, seed=_seed("flush"), c_init="uniform", spread=2)
     cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
+    cfg.bs, cfg.s = 16, 0
     cfg.flush = 1
     ref = run_stack(stream, w, config=cfg)
     cfg.flush = flush
     out = run_stack(stream, w, config=cfg)
-    short = uncut_blocks(w)  # the generator's tail run may be long enough to be cut
+    short = uncut_blocks(w, limit=32)  # the generator's tail run may be long enough to be cut
     assert np.array_equal(ref[short].view(np.uint64), out[short].view(np.uint64))
     assert oracle_error(w, out)[0] <= 1e-13
 
 
 def test_deterministic(stream):
-    """Runs of at most 64 entries are never cut, so each C block is flushed once: the same bits every time."""
+    """Runs of at most 2 bs entries are never cut, so each C block is flushed once: the same bits every time."""
     m = n = k = 32
-    w = make_stack_workload(m, n, k, 5000, dtype=np.float64, seed=_seed("det"), c_init="uniform")
-    a = run_stack(stream, w)
-    b = run_stack(stream, w)
-    short = uncut_blocks(w)
+    w = make_stack_workload(m, n, k, 5000, dtype=np.float64, seed=_seed("det"), c_init="uniform", spread=2)
+    cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
+    cfg.bs, cfg.s = 16, 0
+    a = run_stack(stream, w, config=cfg)
+    b = run_stack(stream, w, config=cfg)
+    short = uncut_blocks(w, limit=32)
     assert np.array_equal(a[short].view(np.uint64), b[short].view(np.uint64))
     assert oracle_error(w, a)[0] <= 1e-13 and oracle_error(w, b)[0] <= 1e-13
 
@@ -220,7 +222,7 @@ def test_full_size_23_30k_oracle(stream):
 @pytest.mark.parametrize("stack_size", [100000, 500000])
 def test_full_size_checksum(stream, dtype, stack_size):
     """Large stacks: size-independent checks (checksum of the update; bitwise determinism for every
-    C block whose run has at most 64 entries, the others are cut into pieces added atomically)."""
+    C block whose run has at most 2 bs = 32 entries, the others are cut into pieces added atomically)."""
     m = n = k = 23
     w = make_stack_workload(m, n, k, stack_size, dtype=dtype, seed=_seed("big", stack_size))
     out = run_stack(stream, w)
@@ -229,7 +231,7 @@ def test_full_size_checksum(stream, dtype, stack_size):
     eps = 1e-12 if dtype == np.float64 else 1e-5
     assert abs(got - total) <= eps * scale, (got, total, scale)
     again = run_stack(stream, w)
-    short = uncut_blocks(w)
+    short = uncut_blocks(w, limit=32)
     assert np.array_equal(out[short], again[short])
     assert np.allclose(out[~short], again[~short], rtol=0, atol=eps * max(1.0, float(np.abs(out).max())) * 64)
 
