@@ -34,6 +34,8 @@ typedef struct libsmm_acc_smm_config_t {
   int variant, tm, tn, nt, bs, kt, flush;
   int s;         /* stack size bs was chosen for (CSV column S); shorter stacks launch with bs scaled down
                     (samples/smm/smm_kernel.c:463-473); 0: bs as given */
+  int split;     /* C-runs longer than split x bs entries are cut into pieces added atomically (CSV column
+                    SPLIT); 0: LIBXSTREAM_SMM_SPLIT (default 2) */
   double gflops; /* measured (autotuner), predicted (learned model) or 0 */
 } libsmm_acc_smm_config_t;
 

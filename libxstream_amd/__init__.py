@@ -40,6 +40,7 @@ class SmmConfig(ctypes.Structure):
         ("kt", ctypes.c_int),
         ("flush", ctypes.c_int),
         ("s", ctypes.c_int),
+        ("split", ctypes.c_int),
         ("gflops", ctypes.c_double),
     ]
 

@@ -60,7 +60,8 @@ void lxs_learned_load_default(const char* path);
 /** LIBSMM registry lifetime (smm.c) */
 int lxs_smm_init(void);
 void lxs_smm_finalize(void);
-/* C-runs of a launch with bs entries per work-group are cut past this many entries (0: never) */
-int lxs_smm_tsplit(int bs);
+/* C-runs of a launch with bs entries per work-group are cut past this many entries (0: never);
+   split: the configuration's factor (0: the library default) */
+int lxs_smm_tsplit(int bs, int split);
 
 #endif /*LXS_INTERNAL_H*/

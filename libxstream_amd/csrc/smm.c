@@ -95,7 +95,7 @@ LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, in
   lxs_smm_args_t args;
   if (NULL == config || 0 >= m || 0 >= n || 0 >= k || 0 == lxs_typesize(datatype)) return EXIT_FAILURE;
   if ((0 != config->variant && 1 != config->variant) || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt ||
-      0 > config->bs || 0 > config->kt) {
+      0 > config->bs || 0 > config->kt || 0 > config->split) {
     return EXIT_FAILURE;
   }
   if (0 == lxs_smm_available((int)datatype, config->variant, m, n, k, config->tm, config->tn, config->nt,
@@ -431,6 +431,7 @@ LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
         config.kt = LXS_GETI("KT");
         config.flush = LXS_GETI("FLUSH");
         config.s = LXS_GETI("S");
+        config.split = LXS_GETI("SPLIT");
 #undef LXS_GETI
         if (0 != with_perf) config.gflops = atof(values[lxs_column(names, ncolumns, "GFLOPS")]);
         ok = (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)key[0], key[1], key[2], key[3], &config));
@@ -463,14 +464,14 @@ LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* devi
   FILE* const file = (NULL != filename ? fopen(filename, "w") : NULL);
   int i, nrows = 0;
   if (NULL == file) return -1;
-  fprintf(file, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH\n");
+  fprintf(file, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT\n");
   pthread_mutex_lock(&lxs_smm_lock);
   for (i = 0; i < lxs_smm_cap; ++i) {
     const lxs_smm_slot_t* const s = lxs_smm_slots + i;
     if (0 == s->used) continue;
-    fprintf(file, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i\n", NULL != device ? device : "MI355X", s->type, s->m,
+    fprintf(file, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i;%i\n", NULL != device ? device : "MI355X", s->type, s->m,
       s->n, s->k, s->config.s, s->config.gflops, s->config.variant, s->config.tm, s->config.tn, s->config.nt, s->config.bs,
-      s->config.kt, s->config.flush);
+      s->config.kt, s->config.flush, s->config.split);
     ++nrows;
   }
   pthread_mutex_unlock(&lxs_smm_lock);
@@ -582,14 +583,16 @@ LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t
   return result;
 }
 
-/* A C-run is summed by the work-group where it starts, unless it is longer than tsplit = 2 bs entries
-   (LIBXSTREAM_SMM_SPLIT x bs): then it is cut at work-group boundaries and its pieces are added with
+/* A C-run is summed by the work-group where it starts, unless it is longer than tsplit = split x bs
+   entries (the configuration's split, else LIBXSTREAM_SMM_SPLIT, default 2; LIBXSTREAM_SMM_SPLIT=0 never
+   cuts): then it is cut at work-group boundaries and its pieces are added with
    atomics, so one long run (many k-blocks into one C block) does not leave a single work-group working
    alone, and a short stack (bs scaled down) spreads its runs over many work-groups. Runs of up to
    LIBXSTREAM_SMM_SPLIT_MIN entries are never cut (default 1; 64 keeps such runs bit-identical for
    every bs, at the price of short stacks). */
-int lxs_smm_tsplit(int bs) {
-  const int t = lxs_cfg.split * (1 < bs ? bs : 1), floor = (0 < lxs_cfg.split_min ? lxs_cfg.split_min : 1);
+int lxs_smm_tsplit(int bs, int split) {
+  const int factor = (0 < split ? split : lxs_cfg.split);
+  const int t = factor * (1 < bs ? bs : 1), floor = (0 < lxs_cfg.split_min ? lxs_cfg.split_min : 1);
   return (0 < lxs_cfg.split ? (floor < t ? t : floor) : 0);
 }
 
@@ -626,7 +629,7 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
      which is exact-once per element when the stack is grouped by C and correct otherwise */
   args.atomic = (1 == config->flush ? 0 : 1);
   args.pfmt = param_format;
-  args.tsplit = lxs_smm_tsplit(args.bs);
+  args.tsplit = lxs_smm_tsplit(args.bs, config->split);
   token = lxs_profile_begin(hs);
   result = lxs_smm_launch(&args, (void*)hs);
   lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,

@@ -143,6 +143,44 @@ static int shape_done(const char* jsonl, int dtype, int m, int n, int k, int sta
   return done;
 }
 
+/* average launch time of a configuration in ms (profile records), or a negative value if it does not run */
+static double tune_time(const libsmm_acc_smm_config_t* cfg, const tune_work_t* w, const void* ds, const void* da,
+  const void* db, void* dc, int m, int n, int k, int reps, void* stream) {
+  long cnt = 0;
+  double ms = 0;
+  int r;
+  if (EXIT_SUCCESS != libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, (libsmm_acc_data_t)(8 == w->ts ? 3 : 1), da, db,
+                        dc, m, n, k, stream)) {
+    (void)libxstream_error_consume(NULL, 0);
+    return -1;
+  }
+  libxstream_profile_reset();
+  for (r = 0; r < reps; ++r) {
+    libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, (libsmm_acc_data_t)(8 == w->ts ? 3 : 1), da, db, dc, m, n, k,
+      stream);
+  }
+  c_dbcsr_acc_stream_sync(stream);
+  libxstream_profile_get("smm", &cnt, &ms, NULL, NULL);
+  return (0 < cnt && 0 < ms) ? ms / (double)cnt : -1;
+}
+
+/* max normalised error of one launch of cfg from zeroed C, over the first C blocks */
+static double tune_error(const libsmm_acc_smm_config_t* cfg, const tune_work_t* w, const void* ds, const void* da,
+  const void* db, void* dc, int m, int n, int k, void* stream) {
+  const int nblocks = (w->nc < 40 ? w->nc : 40);
+  double err = 1;
+  void* host = malloc((size_t)w->ts * nblocks * m * n);
+  if (NULL == host) return err;
+  c_dbcsr_acc_memset_zero(dc, 0, (size_t)w->ts * w->nc * m * n, stream);
+  libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, (libsmm_acc_data_t)(8 == w->ts ? 3 : 1), da, db, dc, m, n, k,
+    stream);
+  c_dbcsr_acc_memcpy_d2h(dc, host, (size_t)w->ts * nblocks * m * n, stream);
+  c_dbcsr_acc_stream_sync(stream);
+  err = host_check(w, host, nblocks);
+  free(host);
+  return err;
+}
+
 static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   const int dtype = o->dtype;
   tune_work_t w;
@@ -274,19 +312,83 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
     }
     fprintf(stderr, "tune_multiply: %ix%ix%i configuration %i failed the check (error %g)\n", m, n, k, i, err);
   }
+  if (0 < best.nt && EXIT_SUCCESS == result) {
+    /* refinement around the winner: entries per work-group and the long-run cut (C-runs longer than
+       split x bs entries are cut into pieces); the sweep file keeps the plain configurations only */
+    static const int rbs[] = {4, 8, 16, 32}, rsplit[] = {1, 2};
+    libsmm_acc_smm_config_t cand_best = best;
+    double t_best;
+    libxstream_profile_enable(1);
+    t_best = tune_time(&best, &w, ds, da, db, dc, m, n, k, o->reps, stream);
+    for (i = 0; i < 4 && 0 < t_best; ++i) {
+      for (j = 0; j < 2; ++j) {
+        libsmm_acc_smm_config_t cfg = best;
+        double t;
+        cfg.bs = rbs[i];
+        cfg.split = rsplit[j];
+        if (EXIT_SUCCESS != libsmm_acc_config_valid((libsmm_acc_data_t)dtype, m, n, k, &cfg)) continue;
+        t = tune_time(&cfg, &w, ds, da, db, dc, m, n, k, o->reps, stream);
+        if (0 < t && t < 0.97 * t_best) {
+          cfg.gflops = flops / (t * 1e6);
+          cand_best = cfg;
+          t_best = t;
+        }
+      }
+    }
+    if (cand_best.bs != best.bs || cand_best.split != best.split) {
+      /* a refinement must not lose on a long stack: bs is scaled down for shorter stacks only, so the
+         configuration also serves stacks of many times the tuned size (the same shape at 16x S) */
+      tune_work_t w2;
+      void *da2 = NULL, *db2 = NULL, *dc2 = NULL, *ds2 = NULL;
+      const int s2 = (16 * w.stack_size < 500000 ? 16 * w.stack_size : 500000);
+      int ok = (s2 > w.stack_size && EXIT_SUCCESS == work_init(&w2, dtype, m, n, k, s2, 0x5eed2507ull ^ (uint64_t)m));
+      if (0 != ok) {
+        ok = (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&da2, (size_t)w2.ts * w2.na * m * k)) &&
+             (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&db2, (size_t)w2.ts * w2.nb * k * n)) &&
+             (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&dc2, (size_t)w2.ts * w2.nc * m * n)) &&
+             (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&ds2, sizeof(int) * 3 * (size_t)w2.stack_size));
+        if (0 != ok) {
+          c_dbcsr_acc_memcpy_h2d(w2.a, da2, (size_t)w2.ts * w2.na * m * k, stream);
+          c_dbcsr_acc_memcpy_h2d(w2.b, db2, (size_t)w2.ts * w2.nb * k * n, stream);
+          c_dbcsr_acc_memcpy_h2d(w2.stack, ds2, sizeof(int) * 3 * (size_t)w2.stack_size, stream);
+          c_dbcsr_acc_memset_zero(dc2, 0, (size_t)w2.ts * w2.nc * m * n, stream);
+          c_dbcsr_acc_stream_sync(stream);
+          {
+            const double t_plain = tune_time(&best, &w2, ds2, da2, db2, dc2, m, n, k, o->reps, stream);
+            const double t_cand = tune_time(&cand_best, &w2, ds2, da2, db2, dc2, m, n, k, o->reps, stream);
+            ok = (0 < t_plain && 0 < t_cand && t_cand <= 1.01 * t_plain);
+            if (0 == ok) {
+              printf("%ix%ix%i: bs=%i split=%i is faster at S=%i but not at S=%i (%.4f vs %.4f ms): not taken\n", m, n,
+                k, cand_best.bs, cand_best.split, w.stack_size, s2, t_cand, t_plain);
+            }
+          }
+        }
+        c_dbcsr_acc_dev_mem_deallocate(da2);
+        c_dbcsr_acc_dev_mem_deallocate(db2);
+        c_dbcsr_acc_dev_mem_deallocate(dc2);
+        c_dbcsr_acc_dev_mem_deallocate(ds2);
+        work_free(&w2);
+      }
+      if (0 != ok) {
+        const double err = tune_error(&cand_best, &w, ds, da, db, dc, m, n, k, stream);
+        if (err <= (8 == w.ts ? 1e-13 : 1e-5)) best = cand_best;
+      }
+    }
+    libxstream_profile_enable(0);
+  }
   if (0 < best.nt) {
     best.s = w.stack_size; /* bs was chosen for this stack size */
     libsmm_acc_config_set((libsmm_acc_data_t)dtype, m, n, k, &best);
-    printf("%ix%ix%i %s: %.1f GFLOPS/s (variant=%i tm=%i tn=%i nt=%i bs=%i kt=%i flush=%i, %i configurations)\n", m,
-      n, k, 8 == w.ts ? "f64" : "f32", best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt,
-      best.flush, nmeasured);
+    printf("%ix%ix%i %s: %.1f GFLOPS/s (variant=%i tm=%i tn=%i nt=%i bs=%i kt=%i flush=%i split=%i, %i configurations)\n",
+      m, n, k, 8 == w.ts ? "f64" : "f32", best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt,
+      best.flush, best.split, nmeasured);
     if (NULL != o->csv) {
       FILE* const csv = fopen(o->csv, "a+");
       if (NULL != csv) {
         fseek(csv, 0, SEEK_END);
-        if (0 == ftell(csv)) fprintf(csv, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH\n");
-        fprintf(csv, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i\n", o->device, dtype, m, n, k, w.stack_size,
-          best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt, best.flush);
+        if (0 == ftell(csv)) fprintf(csv, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT\n");
+        fprintf(csv, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i;%i\n", o->device, dtype, m, n, k, w.stack_size,
+          best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt, best.flush, best.split);
         fclose(csv);
       }
     }

@@ -77,3 +77,13 @@ def uncut_blocks(w, stack=None, limit=32):
     stack = w.stack if stack is None else stack
     runs = np.bincount((np.asarray(stack)[:, 2] - 1) // (w.m * w.n), minlength=w.nc)
     return runs <= limit
+
+
+def cut_limit(tid, m, n, k, stack_size):
+    """Longest C-run that libsmm_acc_process keeps whole for this shape and stack size: split x bs of the
+    launch, bs scaled down for stacks shorter than the configuration's S (smm_kernel.c:463-473)."""
+    cfg, _ = lx.config_get(tid, m, n, k)
+    bs = cfg.bs if cfg.bs > 0 else 32
+    if bs > 1 and cfg.s > 1 and stack_size < cfg.s:
+        bs = min(bs, max(1, -(-stack_size * max(bs, 32) // (cfg.s - 1))))
+    return (cfg.split if cfg.split > 0 else 2) * bs

@@ -10,7 +10,7 @@ import pytest
 import libxstream_amd as lx
 import oracle
 from libxstream_amd.workload import make_stack_workload
-from tests.helpers import checksum, oracle_error, run_stack, tol, uncut_blocks
+from tests.helpers import checksum, cut_limit, oracle_error, run_stack, tol, uncut_blocks
 
 pytestmark = pytest.mark.gpu
 
@@ -103,7 +103,7 @@ def test_flush_modes_bit_identical(stream, flush):
     m = n = k = 23
     w = make_stack_workload(m, n, k, 6000, dtype=np.float64, seed=_seed("flush"), c_init="uniform", spread=2)
     cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
-    cfg.bs, cfg.s = 16, 0
+    cfg.bs, cfg.s, cfg.split = 16, 0, 2
     cfg.flush = 1
     ref = run_stack(stream, w, config=cfg)
     cfg.flush = flush
@@ -118,7 +118,7 @@ def test_deterministic(stream):
     m = n = k = 32
     w = make_stack_workload(m, n, k, 5000, dtype=np.float64, seed=_seed("det"), c_init="uniform", spread=2)
     cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
-    cfg.bs, cfg.s = 16, 0
+    cfg.bs, cfg.s, cfg.split = 16, 0, 2
     a = run_stack(stream, w, config=cfg)
     b = run_stack(stream, w, config=cfg)
     short = uncut_blocks(w, limit=32)
@@ -222,7 +222,8 @@ def test_full_size_23_30k_oracle(stream):
 @pytest.mark.parametrize("stack_size", [100000, 500000])
 def test_full_size_checksum(stream, dtype, stack_size):
     """Large stacks: size-independent checks (checksum of the update; bitwise determinism for every
-    C block whose run has at most 2 bs = 32 entries, the others are cut into pieces added atomically)."""
+    C block whose run is not longer than split x bs of the launch, the others are cut into pieces added
+    atomically)."""
     m = n = k = 23
     w = make_stack_workload(m, n, k, stack_size, dtype=dtype, seed=_seed("big", stack_size))
     out = run_stack(stream, w)
@@ -231,7 +232,7 @@ def test_full_size_checksum(stream, dtype, stack_size):
     eps = 1e-12 if dtype == np.float64 else 1e-5
     assert abs(got - total) <= eps * scale, (got, total, scale)
     again = run_stack(stream, w)
-    short = uncut_blocks(w, limit=32)
+    short = uncut_blocks(w, limit=cut_limit(lx.typeid(dtype), m, n, k, stack_size))
     assert np.array_equal(out[short], again[short])
     assert np.allclose(out[~short], again[~short], rtol=0, atol=eps * max(1.0, float(np.abs(out).max())) * 64)
 
