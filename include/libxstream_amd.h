@@ -36,6 +36,8 @@ typedef struct libsmm_acc_smm_config_t {
                     (samples/smm/smm_kernel.c:463-473); 0: bs as given */
   int split;     /* C-runs longer than split x bs entries are cut into pieces added atomically (CSV column
                     SPLIT); 0: LIBXSTREAM_SMM_SPLIT (default 2) */
+  int bs_short, split_short; /* used instead of bs and split for stacks of at most s entries (CSV columns
+                    BSS, SPLITS); 0: none. Tuned where they win at s but lose on longer stacks. */
   double gflops; /* measured (autotuner), predicted (learned model) or 0 */
 } libsmm_acc_smm_config_t;
 

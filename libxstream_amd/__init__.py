@@ -41,6 +41,8 @@ class SmmConfig(ctypes.Structure):
         ("flush", ctypes.c_int),
         ("s", ctypes.c_int),
         ("split", ctypes.c_int),
+        ("bs_short", ctypes.c_int),
+        ("split_short", ctypes.c_int),
         ("gflops", ctypes.c_double),
     ]
 

@@ -95,7 +95,7 @@ LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, in
   lxs_smm_args_t args;
   if (NULL == config || 0 >= m || 0 >= n || 0 >= k || 0 == lxs_typesize(datatype)) return EXIT_FAILURE;
   if ((0 != config->variant && 1 != config->variant) || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt ||
-      0 > config->bs || 0 > config->kt || 0 > config->split) {
+      0 > config->bs || 0 > config->kt || 0 > config->split || 0 > config->bs_short || 0 > config->split_short) {
     return EXIT_FAILURE;
   }
   if (0 == lxs_smm_available((int)datatype, config->variant, m, n, k, config->tm, config->tn, config->nt,
@@ -432,6 +432,8 @@ LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
         config.flush = LXS_GETI("FLUSH");
         config.s = LXS_GETI("S");
         config.split = LXS_GETI("SPLIT");
+        config.bs_short = LXS_GETI("BSS");
+        config.split_short = LXS_GETI("SPLITS");
 #undef LXS_GETI
         if (0 != with_perf) config.gflops = atof(values[lxs_column(names, ncolumns, "GFLOPS")]);
         ok = (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)key[0], key[1], key[2], key[3], &config));
@@ -464,14 +466,14 @@ LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* devi
   FILE* const file = (NULL != filename ? fopen(filename, "w") : NULL);
   int i, nrows = 0;
   if (NULL == file) return -1;
-  fprintf(file, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT\n");
+  fprintf(file, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT;BSS;SPLITS\n");
   pthread_mutex_lock(&lxs_smm_lock);
   for (i = 0; i < lxs_smm_cap; ++i) {
     const lxs_smm_slot_t* const s = lxs_smm_slots + i;
     if (0 == s->used) continue;
-    fprintf(file, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i;%i\n", NULL != device ? device : "MI355X", s->type, s->m,
-      s->n, s->k, s->config.s, s->config.gflops, s->config.variant, s->config.tm, s->config.tn, s->config.nt, s->config.bs,
-      s->config.kt, s->config.flush, s->config.split);
+    fprintf(file, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i;%i;%i;%i\n", NULL != device ? device : "MI355X", s->type,
+      s->m, s->n, s->k, s->config.s, s->config.gflops, s->config.variant, s->config.tm, s->config.tn, s->config.nt,
+      s->config.bs, s->config.kt, s->config.flush, s->config.split, s->config.bs_short, s->config.split_short);
     ++nrows;
   }
   pthread_mutex_unlock(&lxs_smm_lock);
@@ -601,7 +603,7 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
   lxs_smm_args_t args;
   void* token;
-  int result = lxs_ensure_device();
+  int split, result = lxs_ensure_device();
   if (EXIT_SUCCESS != result) return result;
   memset(&args, 0, sizeof(args));
   args.c = c;
@@ -619,17 +621,22 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   args.nt = config->nt;
   args.bs = config->bs;
   args.kt = config->kt;
+  split = config->split;
+  if (0 < config->bs_short && 0 < config->s && stack_size <= config->s) { /* the short-stack pair */
+    args.bs = config->bs_short;
+    split = config->split_short;
+  }
   /* a stack shorter than the one bs was chosen for spreads over more work-groups (smm_kernel.c:463-473) */
-  if (1 < config->bs && 1 < config->s && stack_size < config->s) {
-    const long cbs = (config->bs > LXS_SMM_BS_MIN ? config->bs : LXS_SMM_BS_MIN);
+  if (1 < args.bs && 1 < config->s && stack_size < config->s) {
+    const long bs0 = args.bs, cbs = (bs0 > LXS_SMM_BS_MIN ? bs0 : LXS_SMM_BS_MIN);
     const long bs = ((long)stack_size * cbs + config->s - 2) / (config->s - 1);
-    args.bs = (int)(bs < config->bs ? (1 > bs ? 1 : bs) : config->bs);
+    args.bs = (int)(bs < bs0 ? (1 > bs ? 1 : bs) : bs0);
   }
   /* flush: 1 = read-modify-write (caller guarantees each C block is one run), else atomic add,
      which is exact-once per element when the stack is grouped by C and correct otherwise */
   args.atomic = (1 == config->flush ? 0 : 1);
   args.pfmt = param_format;
-  args.tsplit = lxs_smm_tsplit(args.bs, config->split);
+  args.tsplit = lxs_smm_tsplit(args.bs, split);
   token = lxs_profile_begin(hs);
   result = lxs_smm_launch(&args, (void*)hs);
   lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,

@@ -341,7 +341,7 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
       tune_work_t w2;
       void *da2 = NULL, *db2 = NULL, *dc2 = NULL, *ds2 = NULL;
       const int s2 = (16 * w.stack_size < 500000 ? 16 * w.stack_size : 500000);
-      int ok = (s2 > w.stack_size && EXIT_SUCCESS == work_init(&w2, dtype, m, n, k, s2, 0x5eed2507ull ^ (uint64_t)m));
+      int short_pair = 0, ok = (s2 > w.stack_size && EXIT_SUCCESS == work_init(&w2, dtype, m, n, k, s2, 0x5eed2507ull ^ (uint64_t)m));
       if (0 != ok) {
         ok = (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&da2, (size_t)w2.ts * w2.na * m * k)) &&
              (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&db2, (size_t)w2.ts * w2.nb * k * n)) &&
@@ -357,9 +357,10 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
             const double t_plain = tune_time(&best, &w2, ds2, da2, db2, dc2, m, n, k, o->reps, stream);
             const double t_cand = tune_time(&cand_best, &w2, ds2, da2, db2, dc2, m, n, k, o->reps, stream);
             ok = (0 < t_plain && 0 < t_cand && t_cand <= 1.01 * t_plain);
-            if (0 == ok) {
-              printf("%ix%ix%i: bs=%i split=%i is faster at S=%i but not at S=%i (%.4f vs %.4f ms): not taken\n", m, n,
-                k, cand_best.bs, cand_best.split, w.stack_size, s2, t_cand, t_plain);
+            if (0 == ok) { /* kept for stacks of at most S entries only */
+              printf("%ix%ix%i: bs=%i split=%i is faster at S=%i but not at S=%i (%.4f vs %.4f ms): short stacks only\n",
+                m, n, k, cand_best.bs, cand_best.split, w.stack_size, s2, t_cand, t_plain);
+              short_pair = 1;
             }
           }
         }
@@ -369,9 +370,16 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
         c_dbcsr_acc_dev_mem_deallocate(ds2);
         work_free(&w2);
       }
-      if (0 != ok) {
+      if (0 != ok || 0 != short_pair) {
         const double err = tune_error(&cand_best, &w, ds, da, db, dc, m, n, k, stream);
-        if (err <= (8 == w.ts ? 1e-13 : 1e-5)) best = cand_best;
+        if (err <= (8 == w.ts ? 1e-13 : 1e-5)) {
+          if (0 != ok) best = cand_best;
+          else {
+            best.bs_short = cand_best.bs;
+            best.split_short = cand_best.split;
+            best.gflops = cand_best.gflops; /* the row's GFLOPS are measured at S */
+          }
+        }
       }
     }
     libxstream_profile_enable(0);
@@ -379,16 +387,17 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   if (0 < best.nt) {
     best.s = w.stack_size; /* bs was chosen for this stack size */
     libsmm_acc_config_set((libsmm_acc_data_t)dtype, m, n, k, &best);
-    printf("%ix%ix%i %s: %.1f GFLOPS/s (variant=%i tm=%i tn=%i nt=%i bs=%i kt=%i flush=%i split=%i, %i configurations)\n",
-      m, n, k, 8 == w.ts ? "f64" : "f32", best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt,
-      best.flush, best.split, nmeasured);
+    printf("%ix%ix%i %s: %.1f GFLOPS/s (variant=%i tm=%i tn=%i nt=%i bs=%i kt=%i flush=%i split=%i bss=%i splits=%i, "
+      "%i configurations)\n", m, n, k, 8 == w.ts ? "f64" : "f32", best.gflops, best.variant, best.tm, best.tn, best.nt,
+      best.bs, best.kt, best.flush, best.split, best.bs_short, best.split_short, nmeasured);
     if (NULL != o->csv) {
       FILE* const csv = fopen(o->csv, "a+");
       if (NULL != csv) {
         fseek(csv, 0, SEEK_END);
-        if (0 == ftell(csv)) fprintf(csv, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT\n");
-        fprintf(csv, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i;%i\n", o->device, dtype, m, n, k, w.stack_size,
-          best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt, best.flush, best.split);
+        if (0 == ftell(csv)) fprintf(csv, "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT;BSS;SPLITS\n");
+        fprintf(csv, "%s;%i;%i;%i;%i;%i;%.1f;%i;%i;%i;%i;%i;%i;%i;%i;%i;%i\n", o->device, dtype, m, n, k,
+          w.stack_size, best.gflops, best.variant, best.tm, best.tn, best.nt, best.bs, best.kt, best.flush, best.split,
+          best.bs_short, best.split_short);
         fclose(csv);
       }
     }

@@ -83,7 +83,9 @@ def cut_limit(tid, m, n, k, stack_size):
     """Longest C-run that libsmm_acc_process keeps whole for this shape and stack size: split x bs of the
     launch, bs scaled down for stacks shorter than the configuration's S (smm_kernel.c:463-473)."""
     cfg, _ = lx.config_get(tid, m, n, k)
-    bs = cfg.bs if cfg.bs > 0 else 32
+    bs, split = (cfg.bs if cfg.bs > 0 else 32), cfg.split
+    if cfg.bs_short > 0 and cfg.s > 0 and stack_size <= cfg.s:  # the configuration's short-stack pair
+        bs, split = cfg.bs_short, cfg.split_short
     if bs > 1 and cfg.s > 1 and stack_size < cfg.s:
         bs = min(bs, max(1, -(-stack_size * max(bs, 32) // (cfg.s - 1))))
-    return (cfg.split if cfg.split > 0 else 2) * bs
+    return (split if split > 0 else 2) * bs

@@ -171,16 +171,17 @@ def test_params_load_reference_and_roundtrip(tmp_path, monkeypatch):
         rows = L.libsmm_acc_params_save(str(out).encode(), b"MI355X")
         assert rows == 5  # distinct shapes of the excerpt: 5^3, 13^3, 23^3, 32^3, 64^3
         text = out.read_text().splitlines()
-        assert text[0] == "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT"
-        mine = lx.SmmConfig(variant=0, tm=3, tn=3, nt=128, bs=40, kt=0, flush=1, s=7777, split=3, gflops=1234.5)
+        assert text[0] == "DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT;BSS;SPLITS"
+        mine = lx.SmmConfig(variant=0, tm=3, tn=3, nt=128, bs=40, kt=0, flush=1, s=7777, split=3, bs_short=8,
+                            split_short=1, gflops=1234.5)
         lx.config_set(lx.REAL_8, 23, 23, 23, mine)
         L.libsmm_acc_params_save(str(out).encode(), b"MI355X")
         lx.config_set(lx.REAL_8, 23, 23, 23, lx.SmmConfig(variant=0, tm=4, tn=4, nt=64, bs=8))
         assert L.libsmm_acc_params_load(str(out).encode()) == 5
         monkeypatch.delenv("OPENCL_LIBSMM_SMM_PARAMS")
         back, src = lx.config_get(lx.REAL_8, 23, 23, 23)
-        assert (back.variant, back.tm, back.tn, back.nt, back.bs, back.flush, back.s, back.split) == (
-            0, 3, 3, 128, 40, 1, 7777, 3)
+        assert (back.variant, back.tm, back.tn, back.nt, back.bs, back.flush, back.s, back.split, back.bs_short,
+                back.split_short) == (0, 3, 3, 128, 40, 1, 7777, 3, 8, 1)
     finally:
         assert L.libsmm_acc_finalize() == 0
 

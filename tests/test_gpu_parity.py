@@ -103,7 +103,7 @@ def test_flush_modes_bit_identical(stream, flush):
     m = n = k = 23
     w = make_stack_workload(m, n, k, 6000, dtype=np.float64, seed=_seed("flush"), c_init="uniform", spread=2)
     cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
-    cfg.bs, cfg.s, cfg.split = 16, 0, 2
+    cfg.bs, cfg.s, cfg.split, cfg.bs_short = 16, 0, 2, 0
     cfg.flush = 1
     ref = run_stack(stream, w, config=cfg)
     cfg.flush = flush
@@ -118,7 +118,7 @@ def test_deterministic(stream):
     m = n = k = 32
     w = make_stack_workload(m, n, k, 5000, dtype=np.float64, seed=_seed("det"), c_init="uniform", spread=2)
     cfg, _ = lx.config_get(lx.REAL_8, m, n, k)
-    cfg.bs, cfg.s, cfg.split = 16, 0, 2
+    cfg.bs, cfg.s, cfg.split, cfg.bs_short = 16, 0, 2, 0
     a = run_stack(stream, w, config=cfg)
     b = run_stack(stream, w, config=cfg)
     short = uncut_blocks(w, limit=32)
