@@ -10,7 +10,11 @@
  *   - one JSON line per measured configuration to the sweep file (resumable: shapes already
  *     marked done there are skipped, tune_multiply.py:656-671), and
  *   - the best configuration per shape to a CSV in the MI355X format
- *     DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH (load with OPENCL_LIBSMM_SMM_PARAMS).
+ *     DEVICE;TYPEID;M;N;K;S;GFLOPS;VARIANT;TM;TN;NT;BS;KT;FLUSH;SPLIT;BSS;SPLITS (load with
+ *     OPENCL_LIBSMM_SMM_PARAMS).
+ * After the sweep, bs and the long-run cut factor are refined around the winner: a refinement that is
+ * faster at the tuned stack size replaces BS/SPLIT if it is not slower at 16 times that size, and is
+ * kept as the short-stack pair BSS/SPLITS (used for stacks of at most S entries) otherwise.
  *
  * usage: tune_multiply [-t f64|f32] [-s stack] [-r reps] [-o best.csv] [-j sweep.jsonl]
  *                      [-d device] [-T seconds] [-l model.txt [-k topk]] [-f triplets.txt | m n k [m n k ...]]
