@@ -775,6 +775,9 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": (round(tr) if tr is not None else None),
             "traffic_detail": tr_detail,
+            # the physical bytes behind L2 per launch (PMC) over the same kernel time, against the same peak
+            "traffic_gbs": (round(tr / (kernel_ms * 1e-3) * 1e-9, 1) if tr is not None else None),
+            "traffic_frac": (round(tr / (kernel_ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if tr is not None else None),
             "kernel": "smm_mfma (libsmm_acc_process)",
             "kernel_ms": round(kernel_ms, 5),
             "bytes_alg_per_launch": bytes_alg,
