@@ -7,7 +7,7 @@ the configuration's throughput relative to the shape's best and log2 of its GFLO
 inference run on the GPU through libsmm_acc_mlp_* (hand-written HIP kernels); this module only
 prepares arrays.
 
-Evaluation follows the exhaustive autotuner as the judge: the network's choice for a held-out shape
+The exhaustive autotuner is the measure of every choice: the network's choice for a held-out shape
 is looked up in that shape's sweep (every configuration was measured there) and compared with the
 best measured one. The learned-guided autotuner (tune_multiply -l model -k K) measures only the
 network's top K; its result is the best of those K measured configurations, looked up the same way. The methods without the network are scored the same way: the built-in default
