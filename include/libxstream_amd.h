@@ -106,6 +106,8 @@ typedef struct libsmm_acc_batch_t libsmm_acc_batch_t;
  *  entries), blocks_per_cu persistent work-groups per CU; 0 selects the default (32 entries of 23x23x23, 8) */
 typedef struct libsmm_acc_batch_params_t {
   int chunk_elements, blocks_per_cu;
+  int per_stack; /* 1: one libsmm_acc_process launch per stack (its own configuration) instead of the
+                    persistent kernel; libsmm_acc_batch_replay captures all of them in one graph */
 } libsmm_acc_batch_params_t;
 /** plan the batch and upload the plan (allocates device memory; not for use under capture) */
 LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,

@@ -71,10 +71,11 @@ class BatchStack(ctypes.Structure):
 class BatchParams(ctypes.Structure):
     """libsmm_acc_batch_params_t (include/libxstream_amd.h): the batch's schedule."""
 
-    _fields_ = [("chunk_elements", ctypes.c_int), ("blocks_per_cu", ctypes.c_int)]
+    _fields_ = [("chunk_elements", ctypes.c_int), ("blocks_per_cu", ctypes.c_int), ("per_stack", ctypes.c_int)]
 
     def as_dict(self):
-        return {"chunk_elements": self.chunk_elements, "blocks_per_cu": self.blocks_per_cu}
+        return {"chunk_elements": self.chunk_elements, "blocks_per_cu": self.blocks_per_cu,
+                "per_stack": self.per_stack}
 
 
 class OzakiContext(ctypes.Structure):

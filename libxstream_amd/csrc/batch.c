@@ -22,6 +22,8 @@ struct libsmm_acc_batch_t {
   hipGraph_t graph;
   hipGraphExec_t exec;
   double flops, bytes;
+  int per_stack;                 /* launch mode, libsmm_acc_batch_params_t */
+  libsmm_acc_batch_stack_t* host; /* the stacks as given (per-stack mode) */
 };
 
 typedef struct lxs_chunk_t {
@@ -136,6 +138,18 @@ LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const 
     b->dtype = (int)datatype;
     b->nstacks = nstacks;
     b->ntasks = ntasks;
+    b->per_stack = (NULL != params && 0 != params->per_stack);
+    b->host = (libsmm_acc_batch_stack_t*)malloc(sizeof(*stacks) * (size_t)nstacks);
+    if (NULL == b->host) result = lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: out of memory");
+    else memcpy(b->host, stacks, sizeof(*stacks) * (size_t)nstacks);
+    for (i = 0; i < nstacks && 0 != b->per_stack && EXIT_SUCCESS == result; ++i) {
+      /* resolve (and cache) every shape's configuration now, so that no lookup or prediction runs
+         while libsmm_acc_batch_replay captures the launches */
+      libsmm_acc_smm_config_t cfg;
+      result = libsmm_acc_config_get(datatype, stacks[i].m, stacks[i].n, stacks[i].k, &cfg, NULL);
+    }
+  }
+  if (EXIT_SUCCESS == result) {
     b->nblocks = (0 < lxs_cfg.ncus ? blocks_per_cu * lxs_cfg.ncus : 256 * blocks_per_cu);
     if (b->nblocks > ntasks) b->nblocks = ntasks;
     result = lxs_hip(hipMalloc((void**)&b->desc, sizeof(*desc) * (size_t)nstacks), "hipMalloc batch");
@@ -159,13 +173,24 @@ LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const 
   else {
     if (NULL != b->desc) (void)hipFree(b->desc);
     if (NULL != b->tasks) (void)hipFree(b->tasks);
+    free(b->host);
     free(b);
   }
   return result;
 }
 
 static int lxs_batch_enqueue(libsmm_acc_batch_t* b, hipStream_t hs) {
-  const int code = lxs_batch_launch(b->dtype, b->desc, b->tasks, b->ntasks, b->nblocks, (void*)hs);
+  int i, code = 0;
+  if (0 == b->per_stack) {
+    code = lxs_batch_launch(b->dtype, b->desc, b->tasks, b->ntasks, b->nblocks, (void*)hs);
+  }
+  else {
+    for (i = 0; i < b->nstacks && 0 == code; ++i) {
+      const libsmm_acc_batch_stack_t* const s = b->host + i;
+      code = lxs_smm_enqueue(b->dtype, s->dev_param_stack, s->stack_size, s->dev_a_data, s->dev_b_data, s->dev_c_data,
+        s->m, s->n, s->k, (void*)hs);
+    }
+  }
   return (0 <= code ? lxs_hip((hipError_t)code, "batch launch") : lxs_error(code, "batch launch"));
 }
 
@@ -227,14 +252,15 @@ LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch) {
     if (NULL != batch->capture) (void)hipStreamDestroy(batch->capture);
     if (NULL != batch->desc) (void)hipFree(batch->desc);
     if (NULL != batch->tasks) (void)hipFree(batch->tasks);
+    free(batch->host);
     free(batch);
   }
   return result;
 }
 
 /*
- * The batch's schedule autotuner: every candidate (chunk size, persistent work-groups per CU) is planned
- * and timed with HIP events on the caller's stacks and pools, with C redirected to scratch buffers (the
+ * The batch's schedule autotuner: every candidate (chunk size, persistent work-groups per CU, and the
+ * per-stack launches as one more) is planned and timed with HIP events on the caller's stacks and pools, with C redirected to scratch buffers (the
  * caller's C is neither read nor written), and the fastest is returned.
  */
 LIBXSTREAM_API int libsmm_acc_batch_tune(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
@@ -283,13 +309,18 @@ LIBXSTREAM_API int libsmm_acc_batch_tune(libsmm_acc_data_t datatype, const libsm
   if (EXIT_SUCCESS == result) result = lxs_hip(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking), "hipStreamCreate");
   if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreate(&e0), "hipEventCreate");
   if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreate(&e1), "hipEventCreate");
-  for (i = 0; i < (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs)) && EXIT_SUCCESS == result; ++i) {
+  for (i = 0; i <= (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs)) && EXIT_SUCCESS == result; ++i) {
+    const int last = (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs));
     libsmm_acc_batch_params_t p;
     libsmm_acc_batch_t* b = NULL;
     float ms = 0;
     int r;
-    p.chunk_elements = scales[i / 3] * 8 * 2 * 23 * 23;
-    p.blocks_per_cu = wgs[i % 3];
+    memset(&p, 0, sizeof(p));
+    if (i < last) {
+      p.chunk_elements = scales[i / 3] * 8 * 2 * 23 * 23;
+      p.blocks_per_cu = wgs[i % 3];
+    }
+    else p.per_stack = 1; /* last candidate: one launch per stack with its own configuration */
     result = libsmm_acc_batch_create_ex(datatype, trial, nstacks, &p, &b);
     if (EXIT_SUCCESS == result) result = lxs_batch_enqueue(b, hs); /* warm-up */
     if (EXIT_SUCCESS == result) result = lxs_hip(hipEventRecord(e0, hs), "hipEventRecord");

@@ -63,5 +63,8 @@ void lxs_smm_finalize(void);
 /* C-runs of a launch with bs entries per work-group are cut past this many entries (0: never);
    split: the configuration's factor (0: the library default) */
 int lxs_smm_tsplit(int bs, int split);
+/* one stack with its resolved configuration on a HIP stream, no profile records (hipError_t or LXS_E*) */
+int lxs_smm_enqueue(int datatype, const int* dev_param_stack, int stack_size, const void* a, const void* b, void* c,
+  int m, int n, int k, void* hip_stream);
 
 #endif /*LXS_INTERNAL_H*/

@@ -598,13 +598,11 @@ int lxs_smm_tsplit(int bs, int split) {
   return (0 < lxs_cfg.split ? (floor < t ? t : floor) : 0);
 }
 
-static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,
-  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream, int param_format) {
-  const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
+/* the launch arguments of one stack under a configuration (bs scaling, short-stack pair, cut length) */
+static void lxs_smm_fill(lxs_smm_args_t* out, const libsmm_acc_smm_config_t* config, const int* dev_param_stack,
+  int stack_size, int datatype, const void* a, const void* b, void* c, int m, int n, int k, int param_format) {
   lxs_smm_args_t args;
-  void* token;
-  int split, result = lxs_ensure_device();
-  if (EXIT_SUCCESS != result) return result;
+  int split;
   memset(&args, 0, sizeof(args));
   args.c = c;
   args.a = a;
@@ -637,11 +635,34 @@ static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_par
   args.atomic = (1 == config->flush ? 0 : 1);
   args.pfmt = param_format;
   args.tsplit = lxs_smm_tsplit(args.bs, split);
+  *out = args;
+}
+
+static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,
+  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream, int param_format) {
+  const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
+  lxs_smm_args_t args;
+  void* token;
+  int result = lxs_ensure_device();
+  if (EXIT_SUCCESS != result) return result;
+  lxs_smm_fill(&args, config, dev_param_stack, stack_size, datatype, a, b, c, m, n, k, param_format);
   token = lxs_profile_begin(hs);
   result = lxs_smm_launch(&args, (void*)hs);
   lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,
     (double)stack_size * lxs_typesize(datatype) * ((double)m * k + (double)k * n + 2.0 * m * n));
   return result;
+}
+
+/* one stack with its resolved configuration on a HIP stream, without profile records (the batch's
+   per-stack mode, also under graph capture); the result is a hipError_t or an LXS_E* code */
+int lxs_smm_enqueue(int datatype, const int* dev_param_stack, int stack_size, const void* a, const void* b, void* c,
+  int m, int n, int k, void* hip_stream) {
+  libsmm_acc_smm_config_t config;
+  lxs_smm_args_t args;
+  if (0 >= stack_size) return 0;
+  lxs_smm_resolve(datatype, m, n, k, &config, NULL);
+  lxs_smm_fill(&args, &config, dev_param_stack, stack_size, datatype, a, b, c, m, n, k, 0);
+  return lxs_smm_launch(&args, hip_stream);
 }
 
 LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* config, const int* dev_param_stack,

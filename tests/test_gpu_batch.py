@@ -142,6 +142,30 @@ def test_batch_long_runs(stream):
         res.free()
 
 
+def test_batch_per_stack_mode(stream):
+    """per_stack = 1: one libsmm_acc_process launch per stack; eager, then three captured-graph replays,
+    each checked with eager work in between."""
+    works = make_mixed_batch(12, seed=17, dtype=np.float64, len_range=(20, 3000), c_init="uniform")
+    res = Resident(works, stream)
+    try:
+        batch = lx.Batch(lx.REAL_8, res.items, lx.BatchParams(per_stack=1))
+        c_ref = [w.c.ravel().copy() for w in works]
+        for step in range(4):
+            if step == 0:
+                batch.process(stream)
+            else:
+                batch.replay(stream)
+            stream.sync()
+            expected = []
+            for i, w in enumerate(works):
+                expected.append(oracle.stack_dd(w.m, w.n, w.k, w.stack, w.a, w.b, c_in=c_ref[i]))
+                c_ref[i] = res.download(i).ravel().copy()
+            check_all(res, expected)
+        batch.close()
+    finally:
+        res.free()
+
+
 def test_batch_rejects_large_blocks(stream):
     w = np.zeros(8)
     d = lx.DeviceBuffer.from_array(w)
