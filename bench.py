@@ -1,17 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark: DBCSR stack multiply (libsmm_acc_process) on one MI355X, FP64 23x23x23.
 
-One step = one libsmm_acc_process call over the whole parameter stack (inputs resident in HBM),
-like one repetition of samples/smm/acc_bench.c:338-342. Prints one JSON line.
+Workload (the configuration the metric is quoted on): m = n = k = 23, FP64, pools of 2000 A and 2000 B
+blocks (uniform[-1,1], seeded) and 1000 C blocks (zero), a parameter stack of 30000 (a, b, c) 1-based
+element offsets sorted by C (runs of 30 +- 4), a and b uniform. One step = one libsmm_acc_process call
+over the whole stack with inputs resident in HBM, like one repetition of samples/smm/acc_bench.c:338-342.
 
-  value         whole-job GFLOP/s = 2*m*n*k*S / wall time per step (host clock, stream-synchronised)
+The LAST stdout line is one compact JSON record (about 2 KB, the shape of acc_bench.c:349-350's
+closing report line):
+  value         whole-job GFLOP/s = 2*m*n*k*S / host wall time per step (stream-synchronised)
   roofline      the stack kernel's algorithmic bytes S*8*(mk+kn+2mn) (samples/smm/smm_kernel.c:504-506)
-                per launch / its average duration from HIP events recorded on the launch stream,
-                against 8 TB/s HBM; `traffic` is the HBM bytes per launch read by rocprofv3 PMC
-                counters (FETCH_SIZE, WRITE_SIZE) in two child runs, each scaled by what the same
-                counter reads for a device-to-device copy of known size in the same run
-  cpu_baseline  the reference host path (OpenMP, whole C-runs per thread) on the host's cores,
-                timed on a bounded sample of the same stack
+                per launch / its average launch duration (one pair of HIP events on the launch stream
+                around the timed launches), against 8 TB/s; `traffic` = bytes per launch behind L2 from
+                rocprofv3 PMC counters (FETCH_SIZE, WRITE_SIZE) in child runs, each scaled by what the
+                same counter reads for a device-to-device copy of known size; `mfma_fp64_tflops` from
+                SQ_INSTS_VALU_MFMA_MOPS_F64 (512 flops per unit) over the same launch duration
+  cpu_baseline  the reference host path (OpenMP, whole C-runs per thread, AVX-512/AVX2 entry kernel) on
+                all of the host's cores, timed on the same stack
+  secondary     the 500000-entry stack, the batched mixed-shape graph, Ozaki-int8 DGEMM against rocBLAS,
+                and the learned predictor, each summarised in one short object
+Everything else (sweeps, per-stack details, transfer, transpose) goes to the --report JSON file.
 """
 import argparse
 import csv
@@ -32,31 +40,48 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 COPY_BYTES = 256 << 20  # calibration copy for the PMC byte counters
+METRIC = "SMM stack GFLOP/s (FP64 23\u00b3) & Ozaki-int8/native-FP64 speedup at equal error"
 
 
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=20)
-    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--steps", type=int, default=50)
+    p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--m", type=int, default=23)
     p.add_argument("--n", type=int, default=None)
     p.add_argument("--k", type=int, default=None)
-    p.add_argument("--stack-size", type=int, default=500000)
+    p.add_argument("--stack-size", type=int, default=30000)
+    p.add_argument("--nc", type=int, default=None, help="C blocks (default 1000 at S=30000, else S/30)")
+    p.add_argument("--na", type=int, default=None, help="A blocks (default 2000 at S=30000, else 2 nc)")
+    p.add_argument("--nb", type=int, default=None, help="B blocks (default 2000 at S=30000, else 2 nc)")
     p.add_argument("--dtype", choices=["f64", "f32"], default="f64")
     p.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline time (0 = skip)")
     p.add_argument("--no-cpu", action="store_true")
     p.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 PMC child runs")
-    p.add_argument("--no-small", action="store_true", help="skip the 30000-entry stack line")
+    p.add_argument("--no-extras", action="store_true", help="only the headline stack (no secondary sections)")
+    p.add_argument("--no-large", action="store_true", help="skip the 500000-entry stack")
     p.add_argument("--no-batch", action="store_true", help="skip the batched mixed-shape measurement")
     p.add_argument("--no-ozaki", action="store_true", help="skip the Ozaki-int8 DGEMM measurement")
     p.add_argument("--ozaki-sizes", default="1024,4096,8192", help="square DGEMM sizes for the Ozaki section")
     p.add_argument("--ozaki-only", action="store_true", help="print only the Ozaki section (no stack measurement)")
     p.add_argument("--no-learned", action="store_true", help="skip the learned-predictor evaluation")
+    p.add_argument("--report", default=os.path.join("gpurun_out", "bench_report.json"),
+                   help="JSON file for the full secondary sections ('' = none)")
     p.add_argument("--sweep", action="store_true",
                    help="print only the plain-stack sweep: 7 shapes x FP64/FP32 x S in {30k, 100k, 500k}")
     p.add_argument("--traffic-probe", action="store_true", help=argparse.SUPPRESS)
     return p.parse_args()
+
+
+def pools(args):
+    """(nc, na, nb): 1000/2000/2000 at S = 30000, else S/30 C blocks and twice as many A and B blocks"""
+    if args.stack_size == 30000:
+        nc, na, nb = 1000, 2000, 2000
+    else:
+        nc = max(1, int(round(args.stack_size / 30.0)))
+        na = nb = 2 * nc
+    return (args.nc or nc, args.na or na, args.nb or nb)
 
 
 def make_workload(args):
@@ -66,7 +91,8 @@ def make_workload(args):
     n = args.n or m
     k = args.k or m
     dtype = np.float64 if args.dtype == "f64" else np.float32
-    return make_stack_workload(m, n, k, args.stack_size, dtype=dtype, seed=25071975)
+    nc, na, nb = pools(args)
+    return make_stack_workload(m, n, k, args.stack_size, nc=nc, na=na, nb=nb, dtype=dtype, seed=25071975)
 
 
 class Resident:
@@ -93,19 +119,23 @@ class Resident:
 
 
 def timed(lx, res, stream, steps, warmup):
+    """(host ms per step, ms per launch between two HIP events on the launch stream around the same
+    timed launches)"""
     for _ in range(warmup):
         res.process(stream)
     stream.sync()
-    lx.profile_enable(True)
-    lx.profile_reset()
+    e0, e1 = lx.Event(), lx.Event()
     t0 = time.perf_counter()
+    e0.record(stream)
     for _ in range(steps):
         res.process(stream)
+    e1.record(stream)
     stream.sync()
     t1 = time.perf_counter()
-    prof = lx.profile_get("smm")
-    lx.profile_enable(False)
-    return (t1 - t0) * 1e3 / steps, prof["ms"] / max(prof["count"], 1)
+    kernel_ms = e0.elapsed_ms(e1) / steps
+    e0.close()
+    e1.close()
+    return (t1 - t0) * 1e3 / steps, kernel_ms
 
 
 def copyin_section(lx, w, stream, reps=5):
@@ -324,15 +354,48 @@ def _rowcol(rng, rows, cols, e, per_row):
     return mant * np.exp2(rng.integers(-e, e + 1, (rows, 1) if per_row else (1, cols))).astype(np.float64)
 
 
-def _sampled_error(A, B, c_colmajor, rng, count=2048):
-    """normwise error over sampled entries against 80-bit extended-precision dot products"""
-    if np.finfo(np.longdouble).nmant < 63:
-        return None
+def _two_prod(x, y):
+    """x*y = p + e exactly, elementwise (Dekker's product with Veltkamp's splitting; numpy does not contract)"""
+    p = x * y
+    t = 134217729.0 * x  # 2^27 + 1
+    xh = t - (t - x)
+    xl = x - xh
+    t = 134217729.0 * y
+    yh = t - (t - y)
+    yl = y - yh
+    return p, ((xh * yh - p) + xh * yl + xl * yh) + xl * yl
+
+
+def _dd_rows(p, e):
+    """double-double sums (hi, lo) of the rows of p + e: hi the correctly rounded sum of the exact terms
+    (math.fsum), lo the correctly rounded remainder"""
+    import math
+
+    hi, lo = np.empty(p.shape[0]), np.empty(p.shape[0])
+    for r in range(p.shape[0]):
+        terms = p[r].tolist() + e[r].tolist()
+        hi[r] = math.fsum(terms)
+        terms.append(-hi[r])
+        lo[r] = math.fsum(terms)
+    return hi, lo
+
+
+def _gemm_reference(A, B, rng, count=1024):
+    """(flat column-major positions, hi, lo) of `count` sampled entries of A B in double-double"""
     m, n = A.shape[0], B.shape[1]
     i, j = rng.integers(0, m, count), rng.integers(0, n, count)
-    ref = (A[i].astype(np.longdouble) * B[:, j].T.astype(np.longdouble)).sum(axis=1)
-    got = c_colmajor[i + j * m].astype(np.longdouble)
-    return float(np.sqrt(((got - ref) ** 2).sum() / (ref ** 2).sum()))
+    hi, lo = np.empty(count), np.empty(count)
+    for s0 in range(0, count, 128):
+        p, e = _two_prod(A[i[s0:s0 + 128]], np.ascontiguousarray(B[:, j[s0:s0 + 128]].T))
+        hi[s0:s0 + 128], lo[s0:s0 + 128] = _dd_rows(p, e)
+    return i + j * m, hi, lo
+
+
+def _ref_error(ref, c_flat):
+    """normwise error ||C - C_dd|| / ||C_dd|| over the sampled entries of a double-double reference"""
+    idx, hi, lo = ref
+    d = (c_flat[idx] - hi) - lo
+    return float(np.sqrt((d * d).sum() / (hi * hi).sum()))
 
 
 class _RocBlas:
@@ -406,19 +469,20 @@ def ozaki_section(lx, stream, sizes, reps=3):
             da, db = lx.DeviceBuffer.from_array(a), lx.DeviceBuffer.from_array(b)
             dc = lx.DeviceBuffer(8 * n * n)
             out = np.empty(n * n)
+            ref = _gemm_reference(A, B, rng)
             if e == 10:  # timing (independent of the exponent range)
                 ms = _event_ms(lx, stream, lambda: lx.dgemm_native("N", "N", n, n, n, 1.0, da, n, db, n, 0.0, dc, n,
                                                                    stream), reps)
                 dc.download(out)
                 entry["native_ms"] = round(ms, 4)
                 entry["native_tflops"] = round(flops / ms * 1e-9, 2)
-                entry["native_error"] = _sampled_error(A, B, out, rng)
+                entry["native_error"] = _ref_error(ref, out)
                 if roc.lib is not None:
                     ms = _event_ms(lx, stream, lambda: roc.dgemm(n, da, db, dc), reps)
                     dc.download(out)
                     entry["rocblas_ms"] = round(ms, 4)
                     entry["rocblas_tflops"] = round(flops / ms * 1e-9, 2)
-                    entry["rocblas_error"] = _sampled_error(A, B, out, rng)
+                    entry["rocblas_error"] = _ref_error(ref, out)
                 entry["configs"] = []
             for cfg_i, (kind, sl, trim) in enumerate(OZAKI_CONFIGS):
                 oz = lx.Ozaki(nslices=sl, trim=trim) if kind == 1 else lx.Ozaki(kind=kind, nprimes=sl, trim=trim)
@@ -452,7 +516,7 @@ def ozaki_section(lx, stream, sizes, reps=3):
                 stream.sync()
                 dc.download(out)
                 oz.close()
-                entry["configs"][cfg_i]["errors"]["e=%d" % e] = _sampled_error(A, B, out, rng)
+                entry["configs"][cfg_i]["errors"]["e=%d" % e] = _ref_error(ref, out)
             for x in (da, db, dc):
                 x.free()
             del A, B, a, b
@@ -461,22 +525,23 @@ def ozaki_section(lx, stream, sizes, reps=3):
     return {
         "workload": "square DGEMM NN; A(i,k) = m 2^x_i, B(k,j) = m 2^y_j, m uniform +-[1,2), x, y uniform integers "
         "in [-e, e]; timing at e=10",
-        "error": "normwise over 2048 sampled entries against 80-bit extended-precision dot products",
+        "error": "normwise over 1024 sampled entries against double-double dot products (exact products, "
+        "correctly rounded sums)",
         "native": "own v_mfma_f64_16x16x4 kernel (ozaki_dgemm_native) and rocBLAS dgemm",
         "results": results,
     }
 
 
-def _stack_sampled_reference(stack, a, bl, m, n, k, rng, count=2048):
-    """flat C positions of `count` sampled elements of the C blocks the (sorted) stack addresses, and their
-    values C = sum over the block's entries of A B as 80-bit extended-precision dot products (C zero)"""
+def _stack_sampled_reference(stack, a, bl, m, n, k, rng, count=1024):
+    """(flat C positions, hi, lo) of `count` sampled elements of the C blocks the (sorted) stack addresses:
+    C = sum over the block's entries of A B in double-double (C zero)"""
     cofs = stack[:, 2] - 1
     blocks = np.unique(cofs)
     pick = blocks[rng.integers(0, len(blocks), count)]
     i, j = rng.integers(0, m, count), rng.integers(0, n, count)
-    ref = np.zeros(count, dtype=np.longdouble)
-    af = a.reshape(-1).astype(np.longdouble)
-    bf = bl.reshape(-1).astype(np.longdouble)
+    hi, lo = np.zeros(count), np.zeros(count)
+    af = a.reshape(-1)
+    bf = bl.reshape(-1)
     kk = np.arange(k)
     for c0 in blocks:
         ents = stack[cofs == c0]
@@ -485,8 +550,9 @@ def _stack_sampled_reference(stack, a, bl, m, n, k, rng, count=2048):
             continue
         arow = af[(ents[:, 0] - 1)[:, None, None] + i[s][None, :, None] + kk[None, None, :] * m]  # A(i, kk)
         bcol = bf[(ents[:, 1] - 1)[:, None, None] + kk[None, None, :] * n + j[s][None, :, None]]  # B(kk, j)
-        ref[s] = (arow * bcol).sum(axis=(0, 2))
-    return pick + i + j * m, ref
+        p, e = _two_prod(arow.transpose(1, 0, 2).reshape(len(s), -1), bcol.transpose(1, 0, 2).reshape(len(s), -1))
+        hi[s], lo[s] = _dd_rows(p, e)
+    return pick + i + j * m, hi, lo
 
 
 OZAKI_STACK_CONFIGS = tuple((s, s - 1) for s in range(3, 9)) + ((8, 0),)  # (slices, trim): triangles, all pairs
@@ -516,8 +582,8 @@ def ozaki_stack_section(lx, stream, stack_size=100000, reps=3):
         da, db = lx.DeviceBuffer.from_array(a), lx.DeviceBuffer.from_array(bl)
         dc = lx.DeviceBuffer.from_array(zeros)
         out = np.empty(w.c.size)
-        sel, ref = _stack_sampled_reference(w.stack[:head], a, bl, m, n, k, rng)
-        err = lambda: float(np.sqrt(((out[sel].astype(np.longdouble) - ref) ** 2).sum() / (ref ** 2).sum()))
+        ref = _stack_sampled_reference(w.stack[:head], a, bl, m, n, k, rng)
+        err = lambda: _ref_error(ref, out)
         if e == 10:
             ms = _event_ms(lx, stream, lambda: lx.process(st, stack_size, lx.REAL_8, da, db, dc, m, n, k, stream), reps)
             entry["native_ms"] = round(ms, 4)
@@ -547,8 +613,8 @@ def ozaki_stack_section(lx, stream, stack_size=100000, reps=3):
     st.free()
     entry["workload"] = ("64x64x64 stack of %d entries sorted by C (pools 2000/2000/1000 blocks); A blocks m 2^x per row, "
                          "B blocks m 2^y per column, m uniform +-[1,2), x, y uniform in [-e, e]; timing at e=10" % stack_size)
-    entry["error"] = ("normwise over 2048 sampled elements of the first 40 C blocks against 80-bit extended-precision "
-                      "dot products")
+    entry["error"] = ("normwise over 1024 sampled elements of the first 40 C blocks against double-double sums "
+                      "(exact products, correctly rounded sums)")
     return entry
 
 
@@ -603,73 +669,97 @@ def traffic_probe(args):
     lx.finalize()
 
 
-def pmc_run(args, counter, tmp):
+def pmc_run(args, counters, tmp):
+    """{counter: (median value per stack launch, largest value of the calibration copy)} from one
+    rocprofv3 --pmc child run of traffic_probe, or None"""
     prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
-    out = os.path.join(tmp, counter.lower())
-    cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", out, "-o", "pmc", "--",
-           sys.executable, os.path.abspath(__file__), "--traffic-probe", "--m", str(args.m),
-           "--stack-size", str(args.stack_size), "--dtype", args.dtype]
-    if args.n:
-        cmd += ["--n", str(args.n)]
-    if args.k:
-        cmd += ["--k", str(args.k)]
-    subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=180, check=False)
+    out = os.path.join(tmp, "_".join(c.lower() for c in counters))
+    cmd = [prof, "--pmc"] + list(counters) + ["--output-format", "csv", "-d", out, "-o", "pmc", "--",
+                                              sys.executable, os.path.abspath(__file__), "--traffic-probe",
+                                              "--m", str(args.m), "--stack-size", str(args.stack_size),
+                                              "--dtype", args.dtype]
+    for flag in ("n", "k", "nc", "na", "nb"):
+        if getattr(args, flag):
+            cmd += ["--" + flag, str(getattr(args, flag))]
+    subprocess.run(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL, timeout=120, check=False)
     rows = []
     for path in glob.glob(os.path.join(out, "**", "*counter_collection*.csv"), recursive=True):
         with open(path) as f:
             rows += list(csv.DictReader(f))
-    copy_vals, smm_vals = [], []
-    for r in rows:
-        if r.get("Counter_Name") != counter:
-            continue
-        name = r.get("Kernel_Name", "")
-        val = float(r.get("Counter_Value", 0))
-        if "smm_" in name:
-            smm_vals.append(val)
-        elif "opy" in name:  # the runtime's device-to-device copy kernel
-            copy_vals.append(val)
-    if not copy_vals or not smm_vals:
-        return None
-    # counter unit per byte, from the copy of known size (reads and writes COPY_BYTES each)
-    factor = max(copy_vals) / COPY_BYTES
-    return {"per_launch_bytes": float(np.median(smm_vals)) / factor, "copy_counter": max(copy_vals),
-            "factor_per_byte": factor}
+    result = {}
+    for counter in counters:
+        copy_vals, smm_vals = [], []
+        for r in rows:
+            if r.get("Counter_Name") != counter:
+                continue
+            name = r.get("Kernel_Name", "")
+            val = float(r.get("Counter_Value", 0))
+            if "smm_" in name:
+                smm_vals.append(val)
+            elif "opy" in name:  # the runtime's device-to-device copy kernel
+                copy_vals.append(val)
+        if smm_vals:
+            result[counter] = (float(np.median(smm_vals)), max(copy_vals) if copy_vals else None)
+    return result or None
 
 
 def traffic(args):
+    """PMC figures of the stack kernel per launch: HBM-side bytes (FETCH_SIZE, WRITE_SIZE, each calibrated
+    by a device copy of COPY_BYTES that reads and writes that many bytes) and FP64 matrix-core work
+    (SQ_INSTS_VALU_MFMA_MOPS_F64 in units of 512 flops, SQ_VALU_MFMA_BUSY_CYCLES, GRBM_GUI_ACTIVE)"""
+    out = {}
     try:
         with tempfile.TemporaryDirectory(prefix="lxs_pmc_") as tmp:
-            fetch = pmc_run(args, "FETCH_SIZE", tmp)
-            write = pmc_run(args, "WRITE_SIZE", tmp)
+            for counters in (("FETCH_SIZE",), ("WRITE_SIZE",),
+                             ("SQ_INSTS_VALU_MFMA_MOPS_F64", "SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE")):
+                got = pmc_run(args, counters, tmp)
+                if got:
+                    out.update(got)
     except (OSError, subprocess.SubprocessError):
-        return None, None
-    if fetch is None or write is None:
-        return None, None
-    return fetch["per_launch_bytes"] + write["per_launch_bytes"], {
-        "fetch_bytes": round(fetch["per_launch_bytes"]), "write_bytes": round(write["per_launch_bytes"]),
-        "fetch_counter_per_byte": fetch["factor_per_byte"], "write_counter_per_byte": write["factor_per_byte"],
-        "calibration": "device-to-device copy of %d bytes in the same run" % COPY_BYTES}
+        pass
+    return out
 
 
 def cpu_baseline(w, seconds):
-    """Reference host path timed on this host (oracle/cpu_baseline.c), bounded to ~`seconds`."""
+    """Reference host path timed on this host (oracle/cpu_baseline.c) on the same stack, on every core
+    this process may run on (and on 16 threads when that is fewer, keeping the faster), bounded to
+    about `seconds` of CPU work in total."""
     import oracle
 
-    nthreads = max(1, min(16, os.cpu_count() or 1))
-    sample = np.ascontiguousarray(w.stack)  # the same stack; repeated until ~`seconds` of CPU work
-    c = np.zeros_like(w.c)
-    t1 = oracle.cpu_stack(w.m, w.n, w.k, sample, w.a, w.b, c, nrepeat=1, warmup=1, nthreads=nthreads)
-    nrep = int(max(1, min(1000, seconds / max(t1, 1e-6))))
-    t = oracle.cpu_stack(w.m, w.n, w.k, sample, w.a, w.b, c, nrepeat=nrep, warmup=0, nthreads=nthreads)
-    flops = 2.0 * w.m * w.n * w.k * len(sample) * nrep
+    avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    best = None
+    for nthreads in sorted({avail, min(16, avail)}, reverse=True):
+        c = np.zeros_like(w.c)
+        t1 = oracle.cpu_stack(w.m, w.n, w.k, w.stack, w.a, w.b, c, nrepeat=1, warmup=1, nthreads=nthreads)
+        nrep = int(max(1, min(10000, 0.5 * seconds / max(t1, 1e-6))))
+        t = oracle.cpu_stack(w.m, w.n, w.k, w.stack, w.a, w.b, c, nrepeat=nrep, warmup=0, nthreads=nthreads)
+        gflops = 2.0 * w.m * w.n * w.k * w.stack_size * nrep / t * 1e-9
+        if best is None or gflops > best[0]:
+            best = (gflops, nthreads, nrep, t)
+    gflops, nthreads, nrep, t = best
     return {
-        "value": round(flops / t * 1e-9, 2),
+        "value": round(gflops, 2),
         "unit": "GFLOP/s",
         "cores": nthreads,
         "kind": "port",
-        "sample": "the full %d-entry stack x %d passes (%.1f s), B column-major as the reference host path"
-        % (len(sample), nrep, t),
+        "sample": "the same %d-entry stack x %d passes (%.1f s), B column-major as the reference host path; "
+        "%s entry kernel; %d cores available" % (w.stack_size, nrep, t, oracle.cpu_isa(), avail),
     }
+
+
+def _compact_ozaki(oz):
+    """best Ozaki-int8 configuration at <= 1e-14 per DGEMM size, against rocBLAS and our FP64 kernel"""
+    out = []
+    for entry in oz["results"]:
+        ok = [c for c in entry["configs"] if all(v is not None and v <= 1e-14 for v in c["errors"].values())]
+        if not ok:
+            continue
+        best = min(ok, key=lambda c: c["ms"])
+        out.append({"n": entry["n"], "scheme": best["scheme"], "moduli_or_slices": best.get("max_primes",
+                    best.get("slices")), "trim": best["trim"], "ms": best["ms"], "rocblas_ms": entry.get("rocblas_ms"),
+                    "vs_rocblas": round(entry["rocblas_ms"] / best["ms"], 3) if entry.get("rocblas_ms") else None,
+                    "vs_own_fp64": best["speedup_vs_native"], "max_error": max(best["errors"].values())})
+    return out
 
 
 def main():
@@ -703,7 +793,8 @@ def main():
         print(json.dumps({"ozaki": out}))
         return
     # PMC child runs first, before this process initialises the GPU
-    tr, tr_detail = (None, None) if args.no_traffic else traffic(args)
+    pmc = {} if args.no_traffic else traffic(args)
+    print("bench: PMC counters %s" % sorted(pmc), file=sys.stderr, flush=True)
 
     import libxstream_amd as lx
     from libxstream_amd.workload import make_stack_workload
@@ -715,24 +806,51 @@ def main():
     res = Resident(lx, w, stream)
     cfg, src = lx.config_get(res.tid, m, n, k)
     ms_per_step, kernel_ms = timed(lx, res, stream, args.steps, args.warmup)
-    trans = transpose_section(lx, res, stream)
+    print("bench: %dx%dx%d S=%d %.4f ms/step, %.4f ms/launch" % (m, n, k, w.stack_size, ms_per_step, kernel_ms),
+          file=sys.stderr, flush=True)
+    report = {}
+    secondary = {}
+    extras = not args.no_extras
+    if extras:
+        report["transpose_b_pool"] = transpose_section(lx, res, stream)
     res.free()
-    copyin = copyin_section(lx, w, stream)
-    small = None
-    if not args.no_small:  # the 30000-entry stack over 2000/2000/1000 blocks (the reference's stack size)
-        ws = make_stack_workload(m, n, k, 30000, nc=1000, na=2000, nb=2000, dtype=w.dtype, seed=25071975)
-        rs = Resident(lx, ws, stream)
-        s_step, s_kernel = timed(lx, rs, stream, max(args.steps, 20), args.warmup)
-        rs.free()
-        small = {"stack_size": 30000, "blocks_a_b_c": [2000, 2000, 1000], "ms_per_step": round(s_step, 5),
-                 "gflops": round(ws.flops() / (s_step * 1e-3) * 1e-9, 1), "kernel_ms": round(s_kernel, 5),
-                 "kernel_gflops": round(ws.flops() / (s_kernel * 1e-3) * 1e-9, 1)}
-    transfer = transfer_section(lx, m, n, k, w.dtype, stream)
-    batch = None if args.no_batch else batched(lx, stream, args.steps, args.warmup)
-    ozaki = None if args.no_ozaki else ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
-    if ozaki is not None:
-        ozaki["stack_64"] = ozaki_stack_section(lx, stream)
-    learned_rep = None if args.no_learned else learned_section(lx)
+    if extras:
+        report["copy_in_pinned"] = copyin_section(lx, w, stream)
+        report["stack_transfer"] = transfer_section(lx, m, n, k, w.dtype, stream)
+    if extras and not args.no_large:  # the 500000-entry stack (pools of 16667/33334/33334 blocks: beyond the MALL)
+        wl = make_stack_workload(m, n, k, 500000, dtype=w.dtype, seed=25071975)
+        rl = Resident(lx, wl, stream)
+        l_step, l_kernel = timed(lx, rl, stream, max(5, args.steps // 5), 2)
+        rl.free()
+        secondary["stack_500000"] = {
+            "pools_a_b_c": [wl.na, wl.nb, wl.nc], "ms_per_step": round(l_step, 4), "kernel_ms": round(l_kernel, 4),
+            "gflops": round(wl.flops() / (l_step * 1e-3) * 1e-9, 1),
+            "alg_frac": round(wl.bytes_alg() / (l_kernel * 1e-3) * 1e-9 / HBM_PEAK_GBS, 3)}
+        print("bench: S=500000 %.4f ms/launch" % l_kernel, file=sys.stderr, flush=True)
+        del wl, rl
+    if extras and not args.no_batch:
+        report["batched_mixed"] = b = batched(lx, stream, 20, 3)
+        secondary["batched_mixed"] = {"stacks": 24, "entries": b["entries"], "graph_ms": b["graph_ms_per_step"],
+                                      "graph_gflops": b["graph_gflops"], "schedule": b["schedule"].get("per_stack")
+                                      and "per-stack launches" or "persistent kernel",
+                                      "separate_launches_ms": b["separate_launches_ms_per_step"]}
+        print("bench: batched %.4f ms" % b["graph_ms_per_step"], file=sys.stderr, flush=True)
+    if extras and not args.no_ozaki:
+        report["ozaki"] = oz = ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])
+        oz["stack_64"] = ozs = ozaki_stack_section(lx, stream)
+        ok = [c for c in ozs["configs"] if all(v <= 1e-14 for v in c["errors"].values())]
+        secondary["ozaki_dgemm"] = _compact_ozaki(oz)
+        if ok:
+            best = min(ok, key=lambda c: c["ms"])
+            secondary["ozaki_stack_64"] = {"slices": best["slices"], "ms": best["ms"], "native_ms": ozs["native_ms"],
+                                           "speedup": best["speedup_vs_native"],
+                                           "max_error": max(best["errors"].values())}
+        print("bench: ozaki done", file=sys.stderr, flush=True)
+    if extras and not args.no_learned:
+        report["learned"] = rep = learned_section(lx)
+        if rep is not None:
+            secondary["learned_heldout"] = {f: {"mean": rep[f]["mean"], "within_10pct": rep[f]["within_10pct"]}
+                                            for f in ("learned", "learned_top4", "default", "nearest")}
     ncus, arch = lx.device_info()
     stream.close()
     lx.finalize()
@@ -744,58 +862,59 @@ def main():
     cpu = None
     if not args.no_cpu and args.cpu_seconds > 0:
         cpu = cpu_baseline(w, args.cpu_seconds)
+        print("bench: cpu baseline %s" % cpu, file=sys.stderr, flush=True)
 
+    traffic_bytes = mfma = None
+    if "FETCH_SIZE" in pmc and "WRITE_SIZE" in pmc and pmc["FETCH_SIZE"][1] and pmc["WRITE_SIZE"][1]:
+        fetch = pmc["FETCH_SIZE"][0] / (pmc["FETCH_SIZE"][1] / COPY_BYTES)
+        write = pmc["WRITE_SIZE"][0] / (pmc["WRITE_SIZE"][1] / COPY_BYTES)
+        traffic_bytes = fetch + write
+    if "SQ_INSTS_VALU_MFMA_MOPS_F64" in pmc:
+        mfma = 512.0 * pmc["SQ_INSTS_VALU_MFMA_MOPS_F64"][0]
+    roof = {
+        "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": round(traffic_bytes) if traffic_bytes is not None else None,
+        "traffic_frac": round(traffic_bytes / (kernel_ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if traffic_bytes else None,
+        "kernel": "smm_mfma (libsmm_acc_process)", "kernel_ms": round(kernel_ms, 5),
+        "bytes_alg_per_launch": bytes_alg,
+        "fp64_tflops": round(flops / (kernel_ms * 1e-3) * 1e-12, 3),
+        "fp64_frac": round(flops / (kernel_ms * 1e-3) * 1e-12 / FP64_PEAK_TFLOPS, 4),
+        "mfma_fp64_tflops": round(mfma / (kernel_ms * 1e-3) * 1e-12, 3) if mfma else None,
+        "mfma_busy": (round(pmc["SQ_VALU_MFMA_BUSY_CYCLES"][0] / (4.0 * ncus * pmc["GRBM_GUI_ACTIVE"][0]), 4)
+                      if "SQ_VALU_MFMA_BUSY_CYCLES" in pmc and pmc.get("GRBM_GUI_ACTIVE", (0,))[0] else None),
+    }
+    report["pmc"] = {c: {"per_launch": v[0], "copy_of_%d_bytes" % COPY_BYTES: v[1]} for c, v in pmc.items()}
     line = {
-        "metric": "SMM stack GFLOP/s (FP64 23^3)" if (m, n, k, args.dtype) == (23, 23, 23, "f64")
-        else "SMM stack GFLOP/s (%s %dx%dx%d)" % (args.dtype.upper(), m, n, k),
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "GFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
+        "ms_per_step": round(ms_per_step, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": args.dtype,
-        "data": "synthetic (seeded uniform[-1,1] pools, sorted-by-C stack, runs 30+-4)",
+        "data": "synthetic: seeded uniform[-1,1] pools, zero C, sorted-by-C stack with runs of 30+-4",
         "config": {
-            "workload": "libsmm_acc_process %dx%dx%d %s stack S=%d (C blocks %d, A/B blocks %d/%d), "
-            "1-based offsets sorted by C" % (m, n, k, args.dtype, w.stack_size, w.nc, w.na, w.nb),
-            "m": m, "n": n, "k": k, "stack_size": w.stack_size,
-            "kernel_config": {kk: v for kk, v in cfg.as_dict().items() if kk != "gflops"},
+            "workload": "libsmm_acc_process %dx%dx%d %s, S=%d, pools A/B/C %d/%d/%d blocks, 1-based offsets "
+            "sorted by C" % (m, n, k, args.dtype, w.stack_size, w.na, w.nb, w.nc),
+            "kernel_config": {kk: v for kk, v in cfg.as_dict().items() if kk not in ("gflops",)},
             "config_source": ["default", "tuned", "env", "learned"][src],
             "device": "%s (%d CUs)" % (arch, ncus),
         },
-        "roofline": {
-            "bound": "hbm",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": (round(tr) if tr is not None else None),
-            "traffic_detail": tr_detail,
-            # the physical bytes behind L2 per launch (PMC) over the same kernel time, against the same peak
-            "traffic_gbs": (round(tr / (kernel_ms * 1e-3) * 1e-9, 1) if tr is not None else None),
-            "traffic_frac": (round(tr / (kernel_ms * 1e-3) * 1e-9 / HBM_PEAK_GBS, 4) if tr is not None else None),
-            "kernel": "smm_mfma (libsmm_acc_process)",
-            "kernel_ms": round(kernel_ms, 5),
-            "bytes_alg_per_launch": bytes_alg,
-            "flops_per_launch": flops,
-            "fp64_tflops": round(flops / (kernel_ms * 1e-3) * 1e-12, 3),
-            "fp64_frac": round(flops / (kernel_ms * 1e-3) * 1e-12 / FP64_PEAK_TFLOPS, 4),
-        },
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "stack_30000": small,
-        "transpose_b_pool": trans,
-        "copy_in_pinned": copyin,
-        "stack_transfer": transfer,
-        "batched_mixed": batch,
-        "ozaki": ozaki,
-        "learned": learned_rep,
-        "reference_published_gflops_23cubed_fp64_s30000": {"V100": 2459.7, "P100": 1657.8},
+        "secondary": secondary,
+        "report": args.report or None,
     }
-    print(json.dumps(line))
+    if args.report:
+        os.makedirs(os.path.dirname(os.path.abspath(args.report)), exist_ok=True)
+        with open(args.report, "w") as f:
+            json.dump(dict(report, headline=line), f)
+    print(json.dumps(line, separators=(",", ":")))
 
 
 if __name__ == "__main__":

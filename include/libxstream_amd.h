@@ -28,7 +28,11 @@
  *            falls in its range and finishes them, so each C block is written by one work-group
  *   kt:      variant 0: k-slice staged in LDS at once (0 = all of k when it fits);
  *            variant 1: k per register stage (a multiple of 4 fixed by the kernel; 0 = any)
- *   flush:   0 = automatic, 1 = plain read-modify-write (stack grouped by C), 2 = atomic add
+ *   flush:   0 = automatic, 1 = plain read-modify-write (stack grouped by C), 2 = atomic add.
+ *            Configurations reached through libsmm_acc_process (tuned CSV, learned predictor, built-in
+ *            default, environment) and the batch always flush with atomic adds, which is correct for
+ *            any stack order, unless LIBXSTREAM_SMM_GROUPED=1 declares that every stack holds each C
+ *            block in one contiguous run; libsmm_acc_process_config takes flush 1 as that declaration.
  */
 typedef struct libsmm_acc_smm_config_t {
   int variant, tm, tn, nt, bs, kt, flush;
@@ -160,9 +164,14 @@ LIBXSTREAM_API int libsmm_acc_mlp_train(libsmm_acc_mlp_t* model, const float* x,
   const float* out_weights, int batch, int epochs, double lr, double weight_decay, double* loss);
 /** inference on the GPU: ensemble mean out[batch][2] */
 LIBXSTREAM_API int libsmm_acc_mlp_forward(libsmm_acc_mlp_t* model, const float* x, int batch, float* out);
+/** the same inference on the host, with the same per-output operation order (equal results); prediction
+ *  (predict_config, predict_topk and the predictor slot of libsmm_acc_process) uses it, so choosing a
+ *  configuration allocates no device memory and never waits for the GPU */
+LIBXSTREAM_API int libsmm_acc_mlp_forward_host(const libsmm_acc_mlp_t* model, const float* x, int batch, float* out);
 LIBXSTREAM_API int libsmm_acc_mlp_save(const libsmm_acc_mlp_t* model, const char* filename);
 LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* filename);
-/** best valid configuration for the shape by the ensemble; confidence = its predicted relative
+/** best valid configuration for the shape by the ensemble (configurations with flush 1 take part only
+ *  when LIBXSTREAM_SMM_GROUPED is set, since they launch with the atomic flush otherwise); confidence = its predicted relative
  *  throughput (0..1); config->gflops = predicted GFLOP/s */
 LIBXSTREAM_API int libsmm_acc_mlp_predict_config(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
   int k, libsmm_acc_smm_config_t* config, double* confidence);

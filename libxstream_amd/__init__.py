@@ -165,6 +165,7 @@ def _declare(L):
         "libsmm_acc_mlp_train": (i, [vp, vp, vp, vp, i, i, ctypes.c_double, ctypes.c_double,
                                      ctypes.POINTER(ctypes.c_double)]),
         "libsmm_acc_mlp_forward": (i, [vp, vp, i, vp]),
+        "libsmm_acc_mlp_forward_host": (i, [vp, vp, i, vp]),
         "libsmm_acc_mlp_save": (i, [vp, ctypes.c_char_p]),
         "libsmm_acc_mlp_load": (i, [pvp, ctypes.c_char_p]),
         "libsmm_acc_mlp_predict_config": (i, [vp, i, i, i, i, ctypes.POINTER(SmmConfig),
@@ -626,6 +627,14 @@ class Mlp:
         out = np.zeros((x.shape[0], 2), dtype=np.float32)
         check(lib().libsmm_acc_mlp_forward(self.handle, x.ctypes.data, x.shape[0], out.ctypes.data),
               "libsmm_acc_mlp_forward")
+        return out
+
+    def forward_host(self, x):
+        """the same inference on the host (what prediction uses)"""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        out = np.zeros((x.shape[0], 2), dtype=np.float32)
+        check(lib().libsmm_acc_mlp_forward_host(self.handle, x.ctypes.data, x.shape[0], out.ctypes.data),
+              "libsmm_acc_mlp_forward_host")
         return out
 
     def predict_config(self, tid, m, n, k):

@@ -30,6 +30,8 @@ typedef struct lxs_config_t {
   int profile;   /* LIBXSTREAM_PROFILE */
   int split;     /* LIBXSTREAM_SMM_SPLIT: C-runs are cut past max(split*bs, split_min) entries (0: never) */
   int split_min; /* LIBXSTREAM_SMM_SPLIT_MIN: runs of up to this many entries are never cut (default 1) */
+  int grouped;   /* LIBXSTREAM_SMM_GROUPED: the caller guarantees that every stack holds each C block in one
+                    contiguous run, which lets configurations with flush 1 write C without atomics (default 0) */
   int ncus;      /* compute units of the active device */
   char arch[256]; /* gcnArchName of the active device */
   pthread_mutex_t lock;

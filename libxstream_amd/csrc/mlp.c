@@ -470,6 +470,79 @@ LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* fil
 }
 
 /*
+ * Ensemble mean of the network outputs on the host (out: batch x outputs), from model->params. It runs
+ * the same arithmetic as the device forward pass (mlp.hip, mlp_forward): per output, the bias, then one
+ * fused multiply-add per input in input order, ReLU on every layer but the last, and the members'
+ * outputs added in member order after division by the member count, so both give the same values.
+ * The predictor slot of libsmm_acc_process uses it: no device memory, copy or synchronisation on the
+ * launch path. Weights are walked input-major (a transposed copy), so that the compiler vectorises
+ * across outputs without changing any output's summation order.
+ */
+__attribute__((target_clones("fma", "default"))) static int mlp_forward_host(const libsmm_acc_mlp_t* model,
+  const float* x, int batch, float* out) {
+  int wmax = 0, l, net, b, i, o;
+  float *wt, *act0, *act1;
+  for (l = 0; l <= model->nlayers; ++l) wmax = (wmax < model->width[l] ? model->width[l] : wmax);
+  wt = (float*)malloc(sizeof(float) * (size_t)model->nparams);
+  act0 = (float*)malloc(sizeof(float) * (size_t)wmax);
+  act1 = (float*)malloc(sizeof(float) * (size_t)wmax);
+  if (NULL == wt || NULL == act0 || NULL == act1) {
+    free(wt);
+    free(act0);
+    free(act1);
+    return EXIT_FAILURE;
+  }
+  for (i = 0; i < batch * model->width[model->nlayers]; ++i) out[i] = 0.f;
+  for (net = 0; net < model->nnets; ++net) {
+    const float* const p = model->params + (size_t)net * model->nparams;
+    size_t off = 0;
+    for (l = 0; l < model->nlayers; ++l) { /* [out][in] weights, then the bias: to [in][out], then the bias */
+      const int nin = model->width[l], nout = model->width[l + 1];
+      for (o = 0; o < nout; ++o) {
+        for (i = 0; i < nin; ++i) wt[off + (size_t)i * nout + o] = p[off + (size_t)o * nin + i];
+      }
+      for (o = 0; o < nout; ++o) wt[off + (size_t)nin * nout + o] = p[off + (size_t)nout * nin + o];
+      off += (size_t)(nin + 1) * nout;
+    }
+    for (b = 0; b < batch; ++b) {
+      const float* in = x + (size_t)b * model->width[0];
+      off = 0;
+      for (l = 0; l < model->nlayers; ++l) {
+        const int nin = model->width[l], nout = model->width[l + 1], relu = (l + 1 < model->nlayers);
+        float* const y = (0 == (l & 1) ? act0 : act1);
+        const float* const w = wt + off;
+        for (o = 0; o < nout; ++o) y[o] = w[(size_t)nin * nout + o];
+        for (i = 0; i < nin; ++i) {
+          const float xi = in[i];
+          const float* const wi = w + (size_t)i * nout;
+          for (o = 0; o < nout; ++o) y[o] = fmaf(xi, wi[o], y[o]);
+        }
+        if (relu) {
+          for (o = 0; o < nout; ++o) y[o] = (y[o] < 0.f ? 0.f : y[o]);
+        }
+        off += (size_t)(nin + 1) * nout;
+        in = y;
+      }
+      {
+        const int nout = model->width[model->nlayers];
+        for (o = 0; o < nout; ++o) out[(size_t)b * nout + o] += in[o] / (float)model->nnets;
+      }
+    }
+  }
+  free(wt);
+  free(act0);
+  free(act1);
+  return EXIT_SUCCESS;
+}
+
+LIBXSTREAM_API int libsmm_acc_mlp_forward_host(const libsmm_acc_mlp_t* model, const float* x, int batch, float* out) {
+  if (NULL == model || NULL == x || NULL == out || 0 >= batch) {
+    return lxs_error(EXIT_FAILURE, "libsmm_acc_mlp_forward_host: invalid argument");
+  }
+  return mlp_forward_host(model, x, batch, out);
+}
+
+/*
  * The ensemble's ranking of every valid configuration of the shape (the set the exhaustive autotuner
  * enumerates): the topk best by predicted relative throughput, best first, with their predicted
  * relative throughput (scores) and predicted GFLOP/s (configs[i].gflops); *count of them.
@@ -493,7 +566,8 @@ static int mlp_rank(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, 
   if (NULL != cfgs && NULL != x && NULL != out && NULL != idx) {
     for (i = 0; i < ncand; ++i) { /* every valid configuration, as the exhaustive autotuner enumerates them */
       for (b = 0; b < 4; ++b) {
-        for (fl = 1; fl <= 2; ++fl) {
+        /* flush 1 launches as the atomic flush unless stacks are declared grouped by C */
+        for (fl = (0 != lxs_cfg.grouped ? 1 : 2); fl <= 2; ++fl) {
           libsmm_acc_smm_config_t cfg;
           memset(&cfg, 0, sizeof(cfg));
           cfg.variant = cand[5 * i];
@@ -515,7 +589,7 @@ static int mlp_rank(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, 
         }
       }
     }
-    if (0 < count && EXIT_SUCCESS == libsmm_acc_mlp_forward(model, x, count, out)) {
+    if (0 < count && EXIT_SUCCESS == mlp_forward_host(model, x, count, out)) {
       const int nsel = (topk < count ? topk : count);
       for (i = 0; i < count; ++i) idx[i] = i;
       for (i = 0; i < nsel; ++i) { /* selection of the nsel best, ties to the earlier candidate */

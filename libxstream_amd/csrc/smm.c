@@ -600,7 +600,8 @@ int lxs_smm_tsplit(int bs, int split) {
 
 /* the launch arguments of one stack under a configuration (bs scaling, short-stack pair, cut length) */
 static void lxs_smm_fill(lxs_smm_args_t* out, const libsmm_acc_smm_config_t* config, const int* dev_param_stack,
-  int stack_size, int datatype, const void* a, const void* b, void* c, int m, int n, int k, int param_format) {
+  int stack_size, int datatype, const void* a, const void* b, void* c, int m, int n, int k, int param_format,
+  int grouped) {
   lxs_smm_args_t args;
   int split;
   memset(&args, 0, sizeof(args));
@@ -630,22 +631,25 @@ static void lxs_smm_fill(lxs_smm_args_t* out, const libsmm_acc_smm_config_t* con
     const long bs = ((long)stack_size * cbs + config->s - 2) / (config->s - 1);
     args.bs = (int)(bs < bs0 ? (1 > bs ? 1 : bs) : bs0);
   }
-  /* flush: 1 = read-modify-write (caller guarantees each C block is one run), else atomic add,
-     which is exact-once per element when the stack is grouped by C and correct otherwise */
-  args.atomic = (1 == config->flush ? 0 : 1);
+  /* flush: atomic add, which is correct for any stack order (multiply.cl:657,687) and exact-once per
+     element for a C-run that is not cut; flush 1 (plain read-modify-write) only when the caller has
+     guaranteed (grouped) that each C block of the stack is one contiguous run, because two work-groups
+     owning separate runs of one C block would otherwise lose updates */
+  args.atomic = (1 == config->flush && 0 != grouped) ? 0 : 1;
   args.pfmt = param_format;
   args.tsplit = lxs_smm_tsplit(args.bs, split);
   *out = args;
 }
 
 static int lxs_smm_run(const libsmm_acc_smm_config_t* config, const int* dev_param_stack, int stack_size,
-  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream, int param_format) {
+  int datatype, const void* a, const void* b, void* c, int m, int n, int k, void* stream, int param_format,
+  int grouped) {
   const hipStream_t hs = lxs_stream_hip((const libxstream_stream_t*)stream);
   lxs_smm_args_t args;
   void* token;
   int result = lxs_ensure_device();
   if (EXIT_SUCCESS != result) return result;
-  lxs_smm_fill(&args, config, dev_param_stack, stack_size, datatype, a, b, c, m, n, k, param_format);
+  lxs_smm_fill(&args, config, dev_param_stack, stack_size, datatype, a, b, c, m, n, k, param_format, grouped);
   token = lxs_profile_begin(hs);
   result = lxs_smm_launch(&args, (void*)hs);
   lxs_profile_end(token, hs, "smm", 2.0 * stack_size * m * n * k,
@@ -661,7 +665,7 @@ int lxs_smm_enqueue(int datatype, const int* dev_param_stack, int stack_size, co
   lxs_smm_args_t args;
   if (0 >= stack_size) return 0;
   lxs_smm_resolve(datatype, m, n, k, &config, NULL);
-  lxs_smm_fill(&args, &config, dev_param_stack, stack_size, datatype, a, b, c, m, n, k, 0);
+  lxs_smm_fill(&args, &config, dev_param_stack, stack_size, datatype, a, b, c, m, n, k, 0, lxs_cfg.grouped);
   return lxs_smm_launch(&args, hip_stream);
 }
 
@@ -676,8 +680,9 @@ LIBXSTREAM_API int libsmm_acc_process_config(const libsmm_acc_smm_config_t* conf
   if (EXIT_SUCCESS != libsmm_acc_config_valid(datatype, m, n, k, config)) {
     return lxs_error(EXIT_FAILURE, "libsmm_acc_process_config: configuration not valid for this shape");
   }
+  /* an explicit configuration with flush 1 is the caller's statement that the stack is grouped by C */
   result = lxs_smm_run(config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data, m, n, k,
-    stream, 0);
+    stream, 0, 1);
   return (0 <= result ? lxs_hip((hipError_t)result, "stack kernel launch") : lxs_error(result, "stack kernel launch"));
 }
 
@@ -703,11 +708,11 @@ static int lxs_smm_process(const int* dev_param_stack, int stack_size, libsmm_ac
   lxs_smm_resolve((int)datatype, m_max, n_max, k_max, &config, NULL);
   if (0 >= config.nt) return -1; /* no compiled kernel covers this shape: host fallback */
   result = lxs_smm_run(&config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data, m_max,
-    n_max, k_max, stack_stream, param_format);
+    n_max, k_max, stack_stream, param_format, lxs_cfg.grouped);
   if (LXS_ENOVARIANT == result || LXS_ELIMIT == result) {
     if (EXIT_SUCCESS != lxs_smm_default((int)datatype, m_max, n_max, k_max, &config)) return -1;
     result = lxs_smm_run(&config, dev_param_stack, stack_size, (int)datatype, dev_a_data, dev_b_data, dev_c_data,
-      m_max, n_max, k_max, stack_stream, param_format);
+      m_max, n_max, k_max, stack_stream, param_format, lxs_cfg.grouped);
     if (0 > result) return -1;
   }
   return lxs_hip((hipError_t)result, "stack kernel launch");

@@ -187,6 +187,8 @@ static double tune_error(const libsmm_acc_smm_config_t* cfg, const tune_work_t* 
 
 static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   const int dtype = o->dtype;
+  const char* const env_grouped = getenv("LIBXSTREAM_SMM_GROUPED");
+  const int grouped = (NULL != env_grouped && 0 != atoi(env_grouped));
   tune_work_t w;
   void *da = NULL, *db = NULL, *dc = NULL, *ds = NULL, *stream = NULL;
   int cand[5 * 256], ncand, i, j, b, f, result = EXIT_SUCCESS, nmeasured = 0, ntodo = 0;
@@ -298,10 +300,15 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
     }
   }
   memset(&best, 0, sizeof(best));
-  for (i = 0; i < nmeasured && i < 8 && EXIT_SUCCESS == result; ++i) {
+  for (i = j = 0; i < nmeasured && j < 8 && EXIT_SUCCESS == result; ++i) {
     const int nblocks = (w.nc < 40 ? w.nc : 40);
     double err;
-    void* host = malloc((size_t)w.ts * nblocks * m * n);
+    void* host;
+    /* flush 1 (no atomics) launches as the atomic flush unless LIBXSTREAM_SMM_GROUPED is set: it is
+       measured for the sweep file but cannot win */
+    if (1 == order[i].flush && 0 == grouped) continue;
+    ++j;
+    host = malloc((size_t)w.ts * nblocks * m * n);
     if (NULL == host) break;
     c_dbcsr_acc_memset_zero(dc, 0, (size_t)w.ts * w.nc * m * n, stream);
     libsmm_acc_process_config(order + i, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,

@@ -15,6 +15,7 @@ heuristic, and the reference's nearest tuned triplet (samples/smm/smm_kernel.c:1
 """
 import gzip
 import json
+import os
 from collections import defaultdict
 
 import numpy as np
@@ -83,6 +84,17 @@ def to_config(row):
     return cfg
 
 
+def grouped():
+    """LIBXSTREAM_SMM_GROUPED: stacks are declared grouped by C, so configurations with flush 1 (no atomics)
+    launch as measured; otherwise they launch with the atomic flush and are not candidates"""
+    return os.environ.get("LIBXSTREAM_SMM_GROUPED", "0") not in ("", "0")
+
+
+def runnable(rows):
+    """the measured configurations that libsmm_acc_process can launch as measured"""
+    return rows if grouped() else [r for r in rows if param_value(r, "flush") != 1]
+
+
 def best_row(rows):
     return max(rows, key=lambda r: r["gflops"])
 
@@ -132,14 +144,14 @@ def evaluate(model, sweep, keys, tuned_keys):
     nearest tuned triplet's configuration, and the best measured one (the exhaustive autotuner's)"""
     out = []
     for key in keys:
-        rows = sweep[key]
+        rows = runnable(sweep[key])
         best = best_row(rows)["gflops"]
         cfg, conf = model.predict_config(*key)
         default = lookup(rows, lx.default_config(*key))
         near = nearest_tuned(key, tuned_keys)
         nearest = None
         if near is not None:
-            ncfg = to_config(best_row(sweep[near]))
+            ncfg = to_config(best_row(runnable(sweep[near])))
             if lx.config_valid(key[0], key[1], key[2], key[3], ncfg):
                 nearest = lookup(rows, ncfg)
         if nearest is None:  # not valid for this shape: the reference falls back to its default

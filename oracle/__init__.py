@@ -44,6 +44,7 @@ def lib():
         L.oracle_cpu_stack.argtypes = [i, i, i, i, i, vp, vp, vp, vp, i, i, i]
         L.oracle_cpu_stack.restype = d
         L.oracle_max_threads.restype = i
+        L.oracle_cpu_isa.restype = i
         L.oracle_gemm_dd.argtypes = [ctypes.c_char, ctypes.c_char, i, i, i, d, vp, i, vp, i, d, vp, i, vp, vp, i, vp, vp]
         L.oracle_gemm_dd.restype = i
         L.oracle_oz_digits.argtypes = [i, vp, i, vp, vp]
@@ -134,6 +135,11 @@ def cpu_stack(m, n, k, stack, a, b, c, nrepeat=1, warmup=0, nthreads=0):
 
 def max_threads():
     return lib().oracle_max_threads()
+
+
+def cpu_isa():
+    """instruction set of the CPU baseline's entry kernel on this host"""
+    return ("x86-64", "avx2+fma", "avx512+fma")[lib().oracle_cpu_isa()]
 
 
 def gemm_dd(transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, rows=None, cols=None):

@@ -166,6 +166,29 @@ def test_batch_per_stack_mode(stream):
         res.free()
 
 
+@pytest.mark.parametrize("per_stack", [0, 1])
+def test_batch_unsorted_stacks(stream, per_stack):
+    """Stacks whose C-runs are interleaved (legal input): both batch modes flush with atomic adds, so
+    every stack still matches the oracle, whatever configuration the per-stack mode resolves."""
+    works = make_mixed_batch(10, seed=29, dtype=np.float64, len_range=(200, 3000), c_init="uniform")
+    rng = np.random.default_rng(31)
+    for w in works:
+        w.stack = np.ascontiguousarray(w.stack[rng.permutation(w.stack_size)])
+    works.append(make_mixed_batch(1, seed=37, dtype=np.float64, dim_range=(45, 45), len_range=(3000, 3000),
+                                  c_init="uniform")[0])  # 45x45x45: a wide block, runs interleaved below
+    works[-1].stack = np.ascontiguousarray(works[-1].stack[rng.permutation(works[-1].stack_size)])
+    res = Resident(works, stream)
+    try:
+        expected = [oracle.stack_dd(w.m, w.n, w.k, w.stack, w.a, w.b, c_in=w.c.ravel()) for w in works]
+        batch = lx.Batch(lx.REAL_8, res.items, lx.BatchParams(per_stack=per_stack))
+        batch.process(stream)
+        stream.sync()
+        check_all(res, expected)
+        batch.close()
+    finally:
+        res.free()
+
+
 def test_batch_rejects_large_blocks(stream):
     w = np.zeros(8)
     d = lx.DeviceBuffer.from_array(w)
@@ -183,7 +206,9 @@ def test_batch_tune_schedule(stream):
     try:
         before = [res.download(i).copy() for i in range(len(works))]
         best, ms = lx.Batch.tune(lx.REAL_8, res.items)
-        assert ms > 0 and best.chunk_elements > 0 and best.blocks_per_cu > 0
+        # either schedule may win: per-stack launches, or the persistent kernel with its chunk size and
+        # work-groups per CU
+        assert ms > 0 and (best.per_stack == 1 or (best.chunk_elements > 0 and best.blocks_per_cu > 0))
         for i in range(len(works)):
             assert np.array_equal(res.download(i), before[i]), "tuning wrote the caller's C (stack %d)" % i
         expected = [oracle.stack_dd(w.m, w.n, w.k, w.stack, w.a, w.b, c_in=w.c.ravel()) for w in works]

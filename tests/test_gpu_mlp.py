@@ -67,6 +67,22 @@ def test_gradient_matches_oracle(dev):
     assert np.corrcoef(g, ref_g)[0, 1] > 0.99999
 
 
+def test_host_inference_matches_device(dev):
+    """Prediction runs the ensemble on the host (no device work on the launch path); it performs the same
+    operations in the same order as the device forward pass, so the outputs agree to the last bit or
+    within one rounding of FP32 where the compiler chose a different but equivalent instruction."""
+    rng = np.random.default_rng(9)
+    x = _inputs(rng, 300)
+    model = lx.Mlp(3, 64, 3, seed=21)
+    try:
+        dev_out = model.forward(x)
+        host_out = model.forward_host(x)
+    finally:
+        model.close()
+    assert np.allclose(host_out, dev_out, rtol=2e-6, atol=1e-6)
+    assert np.mean(host_out == dev_out) > 0.95
+
+
 def test_training_reduces_loss(dev):
     rng = np.random.default_rng(2)
     x = _inputs(rng, 512)

@@ -137,6 +137,58 @@ def test_unsorted_stack(stream, dtype):
     assert oracle_error(w, out, stack=stack)[0] <= tol(dtype)
 
 
+UNSORTED_SHAPES = [(45, 45, 13), (23, 23, 23), (64, 64, 64), (5, 5, 5), (4, 9, 22), (13, 13, 13), (32, 32, 32)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", UNSORTED_SHAPES, ids=["%dx%dx%d" % s for s in UNSORTED_SHAPES])
+def test_unsorted_default_path(stream, shape, dtype):
+    """libsmm_acc_process with the shipped configuration of every workload shape on stacks whose C-runs
+    are interleaved: several work-groups own runs of one C block, and the default flush (atomic adds,
+    samples/smm/kernels/multiply.cl:657,687) keeps every update."""
+    m, n, k = shape
+    w = make_stack_workload(m, n, k, 3000, dtype=dtype, seed=_seed("unsorted-default", shape), c_init="uniform")
+    # shuffle whole runs, then split every run in two halves placed apart: each C block has two runs
+    rng = np.random.default_rng(_seed("perm", shape))
+    half = [w.stack[i::2] for i in range(2)]
+    stack = np.ascontiguousarray(np.concatenate([h[rng.permutation(len(h))] for h in half]))
+    out = run_stack(stream, w, stack=stack)
+    assert oracle_error(w, out, stack=stack)[0] <= tol(dtype)
+
+
+# shapes without a tuned row whose exhaustive sweep winner was a read-modify-write (flush 1) configuration:
+# the learned predictor serves them; unsorted stacks must still be correct
+UNTUNED_RMW_SHAPES = [(lx.REAL_8, 30, 60, 67), (lx.REAL_8, 14, 10, 32), (lx.REAL_4, 79, 24, 36), (lx.REAL_4, 47, 57, 8)]
+
+
+@pytest.mark.parametrize("key", UNTUNED_RMW_SHAPES, ids=["%d-%dx%dx%d" % s for s in UNTUNED_RMW_SHAPES])
+def test_unsorted_learned_shapes(stream, key):
+    tid, m, n, k = key
+    cfg, src = lx.config_get(tid, m, n, k)
+    assert src != 1, "tuned row for an untuned test shape"
+    assert cfg.flush != 1, "a predicted configuration without atomics: %s" % cfg
+    dtype = lx.nptype(tid)
+    w = make_stack_workload(m, n, k, 2500, dtype=dtype, seed=_seed("unsorted-learned", key), c_init="uniform")
+    stack = np.ascontiguousarray(w.stack[np.random.default_rng(5).permutation(w.stack_size)])
+    out = run_stack(stream, w, stack=stack, device_transpose=False)
+    assert oracle_error(w, out, stack=stack)[0] <= tol(dtype)
+
+
+SHAPES_30K = [s for s in SHAPES if s != (23, 23, 23)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", SHAPES_30K, ids=["%dx%dx%d" % s for s in SHAPES_30K])
+def test_full_size_shapes_30k_oracle(stream, shape, dtype):
+    """The other workload shapes at the size the stack metric is quoted on: 30000 entries over 2000 A,
+    2000 B and 1000 C blocks (the S >= 30000 launch geometry, no bs scaling), against the oracle."""
+    m, n, k = shape
+    w = make_stack_workload(m, n, k, 30000, nc=1000, na=2000, nb=2000, dtype=dtype, seed=_seed("30k", shape))
+    out = run_stack(stream, w, device_transpose=False)
+    err = oracle_error(w, out)
+    assert err[0] <= tol(dtype), err
+
+
 @pytest.mark.parametrize("case", ["one_run", "all_distinct", "single", "tail_run"])
 def test_run_structure(stream, case):
     m, n, k = 13, 13, 13
