@@ -138,6 +138,7 @@ def _declare(L):
         "libsmm_acc_config_set": (i, [i, i, i, i, ctypes.POINTER(SmmConfig)]),
         "libsmm_acc_config_valid": (i, [i, i, i, i, ctypes.POINTER(SmmConfig)]),
         "libsmm_acc_kernel_variants": (i, [i, pi, i]),
+        "libsmm_acc_kernel_candidates": (i, [i, i, i, i, pi, i]),
         "libsmm_acc_params_load": (i, [ctypes.c_char_p]),
         "libsmm_acc_params_save": (i, [ctypes.c_char_p, ctypes.c_char_p]),
         "libsmm_acc_read_ref_params": (i, [ctypes.c_char_p, i, i, ctypes.c_char_p, sz, pi, pi, ctypes.POINTER(ctypes.c_double)]),
@@ -434,6 +435,15 @@ def kernel_variants(tid):
     cap = 256
     buf = (ctypes.c_int * (5 * cap))()
     count = lib().libsmm_acc_kernel_variants(tid, buf, cap)
+    return [tuple(buf[5 * i + j] for j in range(5)) for i in range(min(count, cap))]
+
+
+def kernel_candidates(tid, m, n, k):
+    """Every compiled stack kernel usable for (m, n, k), runtime-shape or compiled for exactly this shape:
+    (variant, tm, tn, nt, kt) tuples."""
+    cap = 256
+    buf = (ctypes.c_int * (5 * cap))()
+    count = lib().libsmm_acc_kernel_candidates(tid, m, n, k, buf, cap)
     return [tuple(buf[5 * i + j] for j in range(5)) for i in range(min(count, cap))]
 
 

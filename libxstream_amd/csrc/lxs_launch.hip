@@ -16,6 +16,7 @@ struct smm_entry_t {
 
 struct mfma_entry_t {
   int ti, tj, nwi, kb, m, n, k;
+  int ep; /* 1: one wavefront per run (variant 1); > 1: ep wavefronts share each run (variant 2) */
   const void* fn;
 };
 
@@ -83,14 +84,25 @@ static const mfma_entry_t* mfma_table(int dtype, int* count) {
   return (const mfma_entry_t*)t;
 }
 
-/* MFMA kernel for (dtype, tiles tm x tn, nt threads, kt = k per stage or 0): exact shape first */
-static const mfma_entry_t* mfma_find(int dtype, int m, int n, int k, int tm, int tn, int nt, int kt) {
+/* threads of an MFMA kernel: nwi wavefronts splitting the tiles, or ep wavefronts sharing each run */
+static int mfma_threads(const mfma_entry_t* e) { return 64 * (1 < e->ep ? e->ep : e->nwi); }
+static int mfma_variant(const mfma_entry_t* e) { return (1 < e->ep ? 2 : 1); }
+/* LDS bytes of an MFMA kernel: the partial blocks of the ep wavefronts (16 x 16 tiles) */
+static size_t mfma_lds(const mfma_entry_t* e, int dtype) {
+  return (1 < e->ep ? (size_t)e->ep * e->ti * e->tj * 256 * (3 == dtype ? 8 : 4) : 0);
+}
+
+/* MFMA kernel for (dtype, variant 1 or 2, tiles tm x tn, nt threads, kt = k per stage or 0): exact shape first */
+static const mfma_entry_t* mfma_find(int dtype, int variant, int m, int n, int k, int tm, int tn, int nt, int kt) {
   int count = 0, i;
   const mfma_entry_t* const table = mfma_table(dtype, &count);
   const mfma_entry_t* generic = NULL;
   for (i = 0; i < count; ++i) {
     const mfma_entry_t* const e = table + i;
-    if (e->ti != tm || e->tj != tn || 64 * e->nwi != nt || (0 != kt && 4 * e->kb != kt)) continue;
+    if (mfma_variant(e) != variant || e->ti != tm || e->tj != tn || mfma_threads(e) != nt ||
+        (0 != kt && 4 * e->kb != kt)) {
+      continue;
+    }
     if (e->m == m && e->n == n && e->k == k) return e;
     if (0 == e->m && NULL == generic && m <= 16 * e->ti && n <= 16 * e->tj) generic = e;
   }
@@ -138,6 +150,11 @@ extern "C" int lxs_smm_launchable(const lxs_smm_args_t* args) {
   size_t lds;
   int kt;
   if (1 == args->variant) return 1;
+  if (2 == args->variant) {
+    const mfma_entry_t* const me = mfma_find(args->dtype, 2, args->m, args->n, args->k, args->tm, args->tn, args->nt,
+      args->kt);
+    return (NULL != me && mfma_lds(me, args->dtype) <= lds_limit) ? 1 : 0;
+  }
   e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
   if (NULL == e) return 0;
   if (0 > e->kt) return 1;
@@ -150,6 +167,11 @@ extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
   const smm_entry_t* e;
   int kt;
   if (1 == args->variant) return 0;
+  if (2 == args->variant) {
+    const mfma_entry_t* const me = mfma_find(args->dtype, 2, args->m, args->n, args->k, args->tm, args->tn, args->nt,
+      args->kt);
+    return (NULL != me ? mfma_lds(me, args->dtype) : 0);
+  }
   e = smm_find(args->dtype, args->m, args->n, args->k, args->tm, args->tn, args->nt);
   if (NULL == e || 0 > e->kt) return 0; /* no LDS for the wavefront-per-run kernels */
   kt = (0 < e->kt ? e->kt : (0 < args->kt ? (args->kt < args->k ? args->kt : args->k) : args->k));
@@ -157,8 +179,8 @@ extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
 }
 
 extern "C" int lxs_smm_available(int dtype, int variant, int m, int n, int k, int tm, int tn, int nt, int kt) {
-  if (1 == variant) {
-    const mfma_entry_t* const e = mfma_find(dtype, m, n, k, tm, tn, nt, kt);
+  if (1 == variant || 2 == variant) {
+    const mfma_entry_t* const e = mfma_find(dtype, variant, m, n, k, tm, tn, nt, kt);
     return (NULL == e ? 0 : (0 != e->m ? 2 : 1));
   }
   if (0 == variant) {
@@ -187,10 +209,10 @@ extern "C" int lxs_smm_variants(int dtype, int* q, int cap) {
   for (i = 0; i < count; ++i) {
     if (0 != mtable[i].m) continue;
     if (n < cap && NULL != q) {
-      q[5 * n + 0] = 1;
+      q[5 * n + 0] = mfma_variant(mtable + i);
       q[5 * n + 1] = mtable[i].ti;
       q[5 * n + 2] = mtable[i].tj;
-      q[5 * n + 3] = 64 * mtable[i].nwi;
+      q[5 * n + 3] = mfma_threads(mtable + i);
       q[5 * n + 4] = 4 * mtable[i].kb;
     }
     ++n;
@@ -228,13 +250,17 @@ extern "C" int lxs_smm_candidates(int dtype, int m, int n, int k, int* q, int ca
     const mfma_entry_t* const e = mtable + i;
     int v[5];
     if (0 != e->m ? (e->m != m || e->n != n || e->k != k) : (m > 16 * e->ti || n > 16 * e->tj)) continue;
-    v[0] = 1;
+    if (mfma_lds(e, dtype) > lds_limit) continue;
+    v[0] = mfma_variant(e);
     v[1] = e->ti;
     v[2] = e->tj;
-    v[3] = 64 * e->nwi;
+    v[3] = mfma_threads(e);
     v[4] = 4 * e->kb;
     for (j = 0; j < nq && j < cap; ++j) {
-      if (q[5 * j] == 1 && q[5 * j + 1] == v[1] && q[5 * j + 2] == v[2] && q[5 * j + 3] == v[3] && q[5 * j + 4] == v[4]) break;
+      if (q[5 * j] == v[0] && q[5 * j + 1] == v[1] && q[5 * j + 2] == v[2] && q[5 * j + 3] == v[3] &&
+          q[5 * j + 4] == v[4]) {
+        break;
+      }
     }
     if (j == nq) {
       if (nq < cap) {
@@ -248,14 +274,14 @@ extern "C" int lxs_smm_candidates(int dtype, int m, int n, int k, int* q, int ca
 
 extern "C" int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* q) {
   int count = 0, i;
-  if (1 == variant) {
+  if (1 == variant || 2 == variant) {
     const mfma_entry_t* const table = mfma_table(dtype, &count);
     for (i = 0; i < count; ++i) {
-      if (table[i].m == m && table[i].n == n && table[i].k == k) {
-        q[0] = 1;
+      if (table[i].m == m && table[i].n == n && table[i].k == k && mfma_variant(table + i) == variant) {
+        q[0] = variant;
         q[1] = table[i].ti;
         q[2] = table[i].tj;
-        q[3] = 64 * table[i].nwi;
+        q[3] = mfma_threads(table + i);
         q[4] = 4 * table[i].kb;
         return i;
       }
@@ -286,12 +312,13 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
   const void *pa, *pb;
   const int* stack;
   void* args[14];
-  if (1 == a->variant) {
-    const mfma_entry_t* const e = mfma_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt, a->kt);
+  if (1 == a->variant || 2 == a->variant) {
+    const mfma_entry_t* const e = mfma_find(a->dtype, a->variant, a->m, a->n, a->k, a->tm, a->tn, a->nt, a->kt);
     if (NULL == e) return LXS_ENOVARIANT;
     fn = e->fn;
     kt = 4 * e->kb;
-    lds = 0;
+    lds = mfma_lds(e, a->dtype);
+    if (lds > lds_limit) return LXS_ELIMIT;
   }
   else {
     const smm_entry_t* const e = smm_find(a->dtype, a->m, a->n, a->k, a->tm, a->tn, a->nt);

@@ -41,6 +41,7 @@ namespace {
 typedef LXS_REAL real_t;
 struct mfma_entry_t {
   int ti, tj, nwi, kb, m, n, k; /* m = n = k = 0: runtime shape with m <= 16*ti, n <= 16*tj */
+  int ep;                       /* 1: smm_mfma; > 1: smm_mfma_ep with ep wavefronts per C-run */
   const void* fn;
 };
 
@@ -247,8 +248,207 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
 #undef LXS_LOAD_KSTEP
 }
 
-#define LXS_MG(TI, TJ, NWI, KB) {TI, TJ, NWI, KB, 0, 0, 0, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, 0, 0, 0>},
-#define LXS_MS(M, N, K, TI, TJ, NWI, KB) {TI, TJ, NWI, KB, M, N, K, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, M, N, K>},
+/*
+ * Entry-parallel form for short runs: the EP wavefronts of a work-group share each C-run. Wavefront q
+ * takes the run's entries h+q, h+q+EP, ... (each wavefront covers the whole C block with TI x TJ tiles, the
+ * same rolling fragment loads as smm_mfma), then the EP partial blocks meet in LDS and are added in a
+ * fixed order (wavefront 0 first), the tiles of the block shared out among the wavefronts for that sum
+ * and the flush. So a run that is not cut gives the same bits on every call whatever the timing, and a
+ * run of L entries costs each wavefront about L/EP entries instead of L (runs of ~30 entries and a few
+ * thousand of them leave one wavefront per run too little memory parallelism).
+ * Run ownership and cuts as in smm_mfma: the work-group owns the runs whose first entry lies in
+ * [w*bs, (w+1)*bs); a run continues past that range until its C block changes or a segment boundary
+ * past it cuts the run (lxs_cut). The end of a run is found by a 64-wide ballot over the C indices
+ * (every wavefront computes it, so all take the same path through the barriers).
+ * Barriers wait for LDS only (a fence restricted to the local address space), so fragment loads of the
+ * next run stay in flight across them.
+ */
+template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
+__global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, const T* __restrict__ adata,
+  const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
+  int kt_unused, int atomic, int ps, int pz, int tsplit) {
+  typedef typename mfma<T>::acc_t acc_t;
+  constexpr int SEG_LEN = 16; /* entries per matrix-core accumulation before the fold into the run total */
+  constexpr int NTILE = TI * TJ;
+  const int M = (0 < MC ? MC : m_rt), N = (0 < NC ? NC : n_rt), K = (0 < KC ? KC : k_rt);
+  const int KS = (K + 3) >> 2;       /* k-steps of 4 */
+  const int NS = (KS + KB - 1) / KB; /* k-stages per entry */
+  const int lane = threadIdx.x & 63;
+  const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int li = lane & 15, lk = lane >> 4;
+  extern __shared__ double lxs_smem[];
+  T* const red = reinterpret_cast<T*>(lxs_smem); /* [EP][NTILE][4][64] partial blocks */
+  (void)kt_unused;
+
+  const int begin = blockIdx.x * bs;
+  const int end = min(begin + bs, stack_size);
+  int h = end; /* first piece head in [begin, end) */
+  for (int base = begin; base < end; base += 64) {
+    const int i = base + lane;
+    bool head = false;
+    if (i < end) {
+      head = (0 == i) || (stack[ps * i + 2] != stack[ps * (i - 1) + 2]) ||
+             (i == begin && lxs_cut(stack, ps, i, stack[ps * i + 2], tsplit));
+    }
+    const unsigned long long mask = __ballot(head);
+    if (0 != mask) {
+      h = base + __ffsll((long long)mask) - 1;
+      break;
+    }
+  }
+  if (h >= end) return; /* the same for every wavefront of the work-group */
+
+  int aoff[TI], boff[TJ];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti) aoff[ti] = min(16 * ti + li, M - 1);
+#pragma unroll
+  for (int tj = 0; tj < TJ; ++tj) boff[tj] = min(16 * tj + li, N - 1);
+  const T* const zeros = lxs::zeros<T>();
+  acc_t acc[TI][TJ], prod[TI][TJ];
+  T fa[KB][TI], fb[KB][TJ];
+  const acc_t zero = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = prod[ti][tj] = zero;
+  }
+
+#define LXS_LOAD_KSTEP(KBI, A0, B0, KSTEP) \
+  do { \
+    const int k_ = 4 * (KSTEP) + lk; \
+    const bool kv_ = (k_ < K); \
+    _Pragma("unroll") for (int ti = 0; ti < TI; ++ti) { \
+      fa[KBI][ti] = gload(kv_ ? (adata + ((A0) + k_ * M + aoff[ti])) : zeros); \
+    } \
+    _Pragma("unroll") for (int tj = 0; tj < TJ; ++tj) { \
+      fb[KBI][tj] = gload(kv_ ? (bdata + ((B0) + k_ * N + boff[tj])) : zeros); \
+    } \
+  } while (0)
+
+  int e = -1, a0 = 0, b0 = 0; /* entry whose first k-stage is in flight in the fragment registers */
+  for (;;) {                  /* one run (or piece of a cut run) [h, r) per iteration */
+    const int c0 = stack[ps * h + 2];
+    int r = stack_size;
+    for (int base = h + 1; base < stack_size; base += 64) {
+      const int i = base + lane;
+      bool stop = (i >= stack_size);
+      if (!stop) {
+        stop = (stack[ps * i + 2] != c0) || (end <= i && 0 == (i - end) % bs && lxs_cut(stack, ps, i, c0, tsplit));
+      }
+      const unsigned long long mask = __ballot(stop);
+      if (0 != mask) {
+        r = base + __ffsll((long long)mask) - 1;
+        break;
+      }
+    }
+    /* a piece that starts or ends inside a run shares its C block with another work-group */
+    const bool flush_atomic = atomic || (0 < h && stack[ps * (h - 1) + 2] == c0) ||
+                              (r < stack_size && stack[ps * r + 2] == c0);
+    const bool own_next = (r < end);
+    int cur = h + q;
+    if (cur < r) {
+      if (cur != e) { /* not prefetched: load the first k-stage now */
+        a0 = stack[ps * cur + 0] - pz;
+        b0 = stack[ps * cur + 1] - pz;
+#pragma unroll
+        for (int kb = 0; kb < KB; ++kb) {
+          if (kb < KS) LXS_LOAD_KSTEP(kb, a0, b0, kb);
+        }
+      }
+      int nent = 0;
+      for (;;) { /* one entry of this wavefront per iteration */
+        const int nx = cur + EP;
+        const bool in_run = (nx < r);
+        /* after the run: this wavefront's first entry of the next run, if the work-group owns it */
+        const int cand = r + q;
+        const bool pre = !in_run && own_next && cand < stack_size;
+        const int nxt = (in_run ? nx : (pre ? cand : -1));
+        const int na = (0 <= nxt ? stack[ps * nxt + 0] - pz : a0), nb = (0 <= nxt ? stack[ps * nxt + 1] - pz : b0);
+        for (int s = 0; s < NS; ++s) {
+          const bool last = (s + 1 == NS);
+          const int la = (last ? na : a0), lb = (last ? nb : b0), ns = (last ? 0 : s + 1);
+#pragma unroll
+          for (int kb = 0; kb < KB; ++kb) {
+            const int ks = s * KB + kb;
+            if (ks < KS) {
+#pragma unroll
+              for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+                for (int tj = 0; tj < TJ; ++tj) prod[ti][tj] = mfma<T>::mma(fa[kb][ti], fb[kb][tj], prod[ti][tj]);
+              }
+            }
+            /* refill the slot with the next unit while the matrix core works */
+            if (ns * KB + kb < KS) LXS_LOAD_KSTEP(kb, la, lb, ns * KB + kb);
+          }
+        }
+        if (SEG_LEN == ++nent) { /* fold the segment into the run total */
+#pragma unroll
+          for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+            for (int tj = 0; tj < TJ; ++tj) {
+              acc[ti][tj] += prod[ti][tj];
+              prod[ti][tj] = zero;
+            }
+          }
+          nent = 0;
+        }
+        a0 = na;
+        b0 = nb;
+        if (!in_run) {
+          e = nxt;
+          break;
+        }
+        cur = nx;
+      }
+    }
+    /* this wavefront's partial block into LDS, then the fixed-order sum and the flush, tiles shared out */
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj) {
+        const acc_t sum = acc[ti][tj] + prod[ti][tj];
+        acc[ti][tj] = prod[ti][tj] = zero;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) red[((q * NTILE + ti * TJ + tj) * 4 + x) * 64 + lane] = sum[x];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    {
+      T* const cblk = cdata + (c0 - pz);
+      for (int t = q; t < NTILE; t += EP) {
+        const int ti = t / TJ, tj = t - ti * TJ, j = 16 * tj + li;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          T v = red[(t * 4 + x) * 64 + lane];
+#pragma unroll
+          for (int p = 1; p < EP; ++p) v += red[((p * NTILE + t) * 4 + x) * 64 + lane];
+          const int i = 16 * ti + mfma<T>::row(lane, x);
+          if (i < M && j < N) {
+            T* const ptr = cblk + j * M + i;
+            if (flush_atomic) {
+              unsafeAtomicAdd(ptr, v);
+            }
+            else {
+              *ptr += v;
+            }
+          }
+        }
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier(); /* every sum is read before the next run's partials overwrite LDS */
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+    if (!own_next) break;
+    h = r;
+  }
+#undef LXS_LOAD_KSTEP
+}
+
+#define LXS_MG(TI, TJ, NWI, KB) {TI, TJ, NWI, KB, 0, 0, 0, 1, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, 0, 0, 0>},
+#define LXS_MS(M, N, K, TI, TJ, NWI, KB) {TI, TJ, NWI, KB, M, N, K, 1, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, M, N, K>},
+#define LXS_ME(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, KB, M, N, K, EP, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, M, N, K>},
 
 const mfma_entry_t mfma_table[] = {
   /* runtime shape (m <= 16*ti, n <= 16*tj), 4 k-steps (16 of k) per stage: every tile grid up to
@@ -266,6 +466,11 @@ const mfma_entry_t mfma_table[] = {
   LXS_MS(32, 32, 32, 2, 2, 1, 8) LXS_MS(32, 32, 32, 2, 2, 2, 8) LXS_MS(32, 32, 32, 2, 2, 1, 4)
   LXS_MS(64, 64, 64, 4, 4, 4, 4) LXS_MS(64, 64, 64, 4, 4, 2, 4) LXS_MS(64, 64, 64, 4, 4, 4, 8)
   LXS_MS(4, 9, 22, 1, 1, 1, 6) LXS_MS(45, 45, 13, 3, 3, 1, 4) LXS_MS(45, 45, 13, 3, 3, 3, 4)
+  /* entry-parallel (deterministic whole runs) for the stack workloads */
+  LXS_ME(23, 23, 23, 2, 2, 2, 6) LXS_ME(23, 23, 23, 2, 2, 4, 6) LXS_ME(23, 23, 23, 2, 2, 8, 6)
+  LXS_ME(13, 13, 13, 1, 1, 4, 4) LXS_ME(13, 13, 13, 1, 1, 8, 4) LXS_ME(32, 32, 32, 2, 2, 2, 8)
+  LXS_ME(32, 32, 32, 2, 2, 4, 8) LXS_ME(64, 64, 64, 4, 4, 2, 4) LXS_ME(45, 45, 13, 3, 3, 2, 4)
+  LXS_ME(5, 5, 5, 1, 1, 4, 2) LXS_ME(4, 9, 22, 1, 1, 4, 6)
 };
 
 } /* namespace */

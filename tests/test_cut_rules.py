@@ -121,3 +121,66 @@ def test_long_tail_is_spread():
     for T, bound in ((0, 1890), (32, 34 + 16)):
         most = max(sum(len(p[1]) for p in owner_walk(c, w, 16, T)) for w in range(30000 // 16))
         assert most <= bound
+
+
+def ep_walk(c, w, bs, T, EP):
+    """smm_mfma_ep: the pieces of work-group w, each with the entries every wavefront sums. Follows the
+    kernel's per-wavefront state: the entry whose first k-stage is in flight (e), prefetched at the end of a
+    run as r + q when the work-group owns the next run; an entry is summed from the fragments in flight only
+    when they belong to it (checked), and every stack index the kernel reads is in bounds (checked)."""
+    S = len(c)
+
+    def rd(i):
+        assert 0 <= i < S, "stack read out of bounds: %d" % i
+        return c[i]
+
+    begin, end = w * bs, min(w * bs + bs, S)
+    h = next((i for i in range(begin, end) if i == 0 or rd(i) != rd(i - 1) or (i == begin and cut(c, i, c[i], T))), end)
+    if h >= end:
+        return []
+    inflight = [None] * EP  # entry whose fragments each wavefront holds
+    pieces = []
+    while True:
+        c0 = rd(h)
+        r = h + 1
+        while r < S and not (rd(r) != c0 or (end <= r and (r - end) % bs == 0 and cut(c, r, c0, T))):
+            r += 1
+        atomic = (h > 0 and rd(h - 1) == c0) or (r < S and rd(r) == c0)
+        own_next = r < end
+        per_wave = []
+        for q in range(EP):
+            mine = []
+            cur = h + q
+            if cur < r:
+                if inflight[q] != cur:
+                    inflight[q] = cur  # fresh load of the first k-stage
+                while True:
+                    assert inflight[q] == cur, "fragments of entry %s used for entry %d" % (inflight[q], cur)
+                    rd(cur)
+                    mine.append(cur)
+                    nx = cur + EP
+                    in_run = nx < r
+                    cand = r + q
+                    pre = (not in_run) and own_next and cand < S
+                    nxt = nx if in_run else (cand if pre else -1)
+                    if nxt >= 0:
+                        rd(nxt)
+                    inflight[q] = nxt if nxt >= 0 else None
+                    if not in_run:
+                        break
+                    cur = nx
+            per_wave.append(mine)
+        entries = sorted(x for mine in per_wave for x in mine)
+        assert entries == list(range(h, r))
+        pieces.append((c0, entries, atomic))
+        if not own_next:
+            return pieces
+        h = r
+
+
+def test_ep_walk_exactly_once():
+    rng = np.random.default_rng(3)
+    for c, srt in stacks(rng, 1500):
+        bs, T, EP = int(rng.integers(1, 70)), int(rng.integers(0, 80)), int(rng.choice([2, 4, 8]))
+        pieces = [p for w in range((len(c) + bs - 1) // bs) for p in ep_walk(c, w, bs, T, EP)]
+        check(c, pieces, srt)

@@ -1,0 +1,104 @@
+"""GPU parity of the entry-parallel matrix-core stack kernel (variant 2: several wavefronts share each
+C-run, their partial blocks added in a fixed order through LDS) through the C ABI, against the oracle.
+
+Covered: every compiled variant-2 kernel on its shape, FP64 and FP32, sorted stacks with ragged runs,
+long runs cut into pieces (atomic flush of the pieces), stacks whose runs are interleaved (atomic flush),
+the read-modify-write flush on sorted stacks, and bit-reproducibility: runs that are not cut give the same
+bits on every call, for every bs, and under both flush modes.
+"""
+import numpy as np
+import pytest
+
+import libxstream_amd as lx
+import oracle
+from libxstream_amd.workload import make_stack_workload
+from tests.helpers import oracle_error, run_stack, tol
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(5, 5, 5), (13, 13, 13), (23, 23, 23), (32, 32, 32), (64, 64, 64), (4, 9, 22), (45, 45, 13)]
+DTYPES = [np.float64, np.float32]
+
+
+@pytest.fixture(scope="module")
+def stream():
+    lx.init()
+    assert lx.device_count() > 0, "no HIP device visible"
+    s = lx.Stream("ep")
+    yield s
+    s.close()
+    lx.finalize()
+
+
+def _ep_configs(tid, m, n, k):
+    out = []
+    for variant, tm, tn, nt, kt in lx.kernel_candidates(tid, m, n, k):
+        if variant == 2:
+            out.append(lx.SmmConfig(variant=2, tm=tm, tn=tn, nt=nt, kt=kt, bs=32, flush=2))
+    return out
+
+
+def _with_runs(w, lengths):
+    stack = w.stack.copy()
+    stack[:, 2] = np.repeat(np.arange(len(lengths), dtype=np.int32), lengths) * (w.m * w.n) + 1
+    return stack
+
+
+def test_every_workload_shape_has_ep_kernels():
+    lx.init()
+    try:
+        for tid in (lx.REAL_8, lx.REAL_4):
+            for shape in SHAPES:
+                cfgs = _ep_configs(tid, *shape)
+                assert cfgs, shape
+                assert all(lx.config_valid(tid, *shape, c) for c in cfgs)
+    finally:
+        lx.finalize()
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", SHAPES, ids=["%dx%dx%d" % s for s in SHAPES])
+def test_ep_parity(stream, shape, dtype):
+    m, n, k = shape
+    tid = lx.typeid(dtype)
+    w = make_stack_workload(m, n, k, 3000, dtype=dtype, seed=sum(shape) * 7 + 1, c_init="uniform", spread=12)
+    long_runs = [700, 3, 7, 40, 1, 64, 65, 33, 300] + [17] * 40 + [1107]
+    wl = make_stack_workload(m, n, k, sum(long_runs), nc=len(long_runs), dtype=dtype, seed=sum(shape) * 7 + 2,
+                             c_init="uniform")
+    stack_long = _with_runs(wl, long_runs)
+    rng = np.random.default_rng(sum(shape))
+    stack_unsorted = np.ascontiguousarray(w.stack[rng.permutation(w.stack_size)])
+    ref_sorted = oracle.stack_dd(m, n, k, w.stack, w.a, w.b, c_in=w.c.ravel())
+    ref_long = oracle.stack_dd(m, n, k, stack_long, wl.a, wl.b, c_in=wl.c.ravel())
+    ref_unsorted = oracle.stack_dd(m, n, k, stack_unsorted, w.a, w.b, c_in=w.c.ravel())
+    for cfg in _ep_configs(tid, m, n, k):
+        for bs in (1, 8, 32, 100):
+            cfg.bs, cfg.s = bs, 0
+            for flush in (1, 2):
+                cfg.flush = flush
+                out = run_stack(stream, w, config=cfg, device_transpose=False)
+                assert oracle.error(out.ravel(), *ref_sorted)[0] <= tol(dtype), (cfg, "sorted")
+                out = run_stack(stream, wl, config=cfg, device_transpose=False, stack=stack_long)
+                assert oracle.error(out.ravel(), *ref_long)[0] <= tol(dtype), (cfg, "long runs")
+            cfg.flush = 2
+            out = run_stack(stream, w, config=cfg, device_transpose=False, stack=stack_unsorted)
+            assert oracle.error(out.ravel(), *ref_unsorted)[0] <= tol(dtype), (cfg, "unsorted")
+
+
+@pytest.mark.parametrize("shape", [(23, 23, 23), (13, 13, 13), (64, 64, 64)], ids=lambda s: "%dx%dx%d" % s)
+def test_ep_bit_reproducible(stream, shape):
+    """runs of at most 60 entries, never cut (split x bs >= 64): one flush per C block, the partial blocks
+    summed in wavefront order, so every call, every bs >= 32 and both flush modes give the same bits"""
+    m, n, k = shape
+    tid = lx.REAL_8
+    w = make_stack_workload(m, n, k, 6000, dtype=np.float64, seed=101, c_init="uniform", spread=25)
+    for cfg in _ep_configs(tid, m, n, k):
+        cfg.s, cfg.split = 0, 2
+        ref = None
+        for bs, flush in ((32, 2), (32, 2), (64, 2), (32, 1), (100, 1)):
+            cfg.bs, cfg.flush = bs, flush
+            out = run_stack(stream, w, config=cfg, device_transpose=False)
+            if ref is None:
+                ref = out
+                assert oracle_error(w, out)[0] <= 1e-13
+            assert np.array_equal(ref.view(np.uint64), out.view(np.uint64)), (cfg, bs, flush)
