@@ -190,7 +190,16 @@ LIBXSTREAM_API int libxstream_profile_reset(void);
 /** totals for one kernel name ("smm", "trans", ...); waits for outstanding records */
 LIBXSTREAM_API int libxstream_profile_get(const char* kernel, long* count, double* total_ms, double* total_flops,
   double* total_bytes);
+/** per-name summary of the kept samples: shortest, median and longest duration, the median sample's own
+ *  rate (GFLOP/s for kernels, GB/s for transfers, from that sample's work and duration), and the number of
+ *  samples discarded as shorter than 10 ticks of the event clock (src/libxstream.c:2640-2660) */
+LIBXSTREAM_API int libxstream_profile_stats(const char* name, double* min_ms, double* median_ms, double* max_ms,
+  double* median_rate, long* discarded);
+/** per-name histograms (LIBXSTREAM_PROFILE=N: N + 1 buckets) as the reference prints them
+ *  (src/libxstream.c:853-1015): kernels by duration, transfers (H2D, D2H, D2D, ZERO) by size */
 LIBXSTREAM_API int libxstream_profile_print(FILE* stream);
+/** copy and fill records (LIBXSTREAM_PROFILE_MEM, src/libxstream_mem.c:805-893) */
+LIBXSTREAM_API int libxstream_profile_mem_enable(int enable);
 
 /** last error on the calling thread: returns its code (0 = none) and clears it */
 LIBXSTREAM_API int libxstream_error_consume(char* message, size_t size);

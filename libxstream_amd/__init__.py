@@ -124,6 +124,10 @@ def _declare(L):
         "libxstream_profile_enable": (i, [i]),
         "libxstream_profile_reset": (i, []),
         "libxstream_profile_get": (i, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_long)] + [ctypes.POINTER(ctypes.c_double)] * 3),
+        "libxstream_profile_stats": (i, [ctypes.c_char_p] + [ctypes.POINTER(ctypes.c_double)] * 4 +
+                                     [ctypes.POINTER(ctypes.c_long)]),
+        "libxstream_profile_print": (i, [vp]),
+        "libxstream_profile_mem_enable": (i, [i]),
         "libsmm_acc_init": (i, []),
         "libsmm_acc_finalize": (i, []),
         "libsmm_acc_is_thread_safe": (i, []),
@@ -449,6 +453,27 @@ def kernel_candidates(tid, m, n, k):
 
 def profile_enable(on=True):
     lib().libxstream_profile_enable(1 if on else 0)
+
+
+def profile_mem_enable(on=True):
+    check(lib().libxstream_profile_mem_enable(1 if on else 0), "libxstream_profile_mem_enable")
+
+
+def profile_stats(name):
+    """{min_ms, median_ms, max_ms, median_rate, discarded} of one profiled kernel or transfer kind"""
+    vals = [ctypes.c_double(0) for _ in range(4)]
+    disc = ctypes.c_long(0)
+    rc = lib().libxstream_profile_stats(os.fsencode(name), *[ctypes.byref(v) for v in vals], ctypes.byref(disc))
+    if rc != 0:
+        lib().libxstream_error_consume(None, 0)
+        return None
+    return {"min_ms": vals[0].value, "median_ms": vals[1].value, "max_ms": vals[2].value,
+            "median_rate": vals[3].value, "discarded": disc.value}
+
+
+def profile_print():
+    """histograms of every profiled kernel and transfer kind, printed to stderr"""
+    check(lib().libxstream_profile_print(None), "libxstream_profile_print")
 
 
 def profile_reset():

@@ -27,7 +27,8 @@ typedef struct lxs_config_t {
   int ndevices;  /* 0 when no HIP device is visible */
   int device;    /* active device, -1 if none */
   int verbosity;
-  int profile;   /* LIBXSTREAM_PROFILE */
+  int profile;   /* LIBXSTREAM_PROFILE: launch records (|value| + 1 histogram buckets; < 0 also prints each) */
+  int profile_mem; /* LIBXSTREAM_PROFILE_MEM: copy and fill records (H2D, D2H, D2D, ZERO) */
   int split;     /* LIBXSTREAM_SMM_SPLIT: C-runs are cut past max(split*bs, split_min) entries (0: never) */
   int split_min; /* LIBXSTREAM_SMM_SPLIT_MIN: runs of up to this many entries are never cut (default 1) */
   int grouped;   /* LIBXSTREAM_SMM_GROUPED: the caller guarantees that every stack holds each C block in one
@@ -50,6 +51,7 @@ hipStream_t lxs_stream_hip(const libxstream_stream_t* stream);
 
 /** profiling around one launch: token is NULL when profiling is off */
 void* lxs_profile_begin(hipStream_t stream);
+void* lxs_profile_begin_mem(hipStream_t stream);
 void lxs_profile_end(void* token, hipStream_t stream, const char* kernel, double flops, double bytes);
 void lxs_profile_finalize(void);
 

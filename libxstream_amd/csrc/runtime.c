@@ -4,6 +4,7 @@
  * src/libxstream_event.c and src/libxstream_mem.c of the reference with HIP runtime calls.
  */
 #include "lxs_internal.h"
+#include <math.h>
 #include <string.h>
 
 lxs_config_t lxs_cfg = {.device = -1, .split_min = 1, .lock = PTHREAD_MUTEX_INITIALIZER};
@@ -98,6 +99,7 @@ LIBXSTREAM_API int libxstream_init_config(const libxstream_init_config_t* cfg) {
     lxs_cfg.verbosity = (NULL != cfg && 0 <= cfg->verbosity) ? cfg->verbosity
                                                              : lxs_getenv_int("LIBXSTREAM_VERBOSE", 0);
     lxs_cfg.profile = lxs_getenv_int("LIBXSTREAM_PROFILE", 0);
+    lxs_cfg.profile_mem = lxs_getenv_int("LIBXSTREAM_PROFILE_MEM", 0);
     lxs_cfg.split = lxs_getenv_int("LIBXSTREAM_SMM_SPLIT", 2);
     lxs_cfg.split_min = lxs_getenv_int("LIBXSTREAM_SMM_SPLIT_MIN", 1);
     lxs_cfg.grouped = lxs_getenv_int("LIBXSTREAM_SMM_GROUPED", 0);
@@ -365,7 +367,12 @@ static int lxs_copy(void* dst, const void* src, size_t nbytes, hipMemcpyKind kin
   if (0 == nbytes) return EXIT_SUCCESS;
   if (NULL == dst || NULL == src) return lxs_error(EXIT_FAILURE, what);
   result = lxs_ensure_device();
-  if (EXIT_SUCCESS == result) result = lxs_hip(hipMemcpyAsync(dst, src, nbytes, kind, hs), what);
+  if (EXIT_SUCCESS == result) {
+    void* const token = lxs_profile_begin_mem(hs);
+    result = lxs_hip(hipMemcpyAsync(dst, src, nbytes, kind, hs), what);
+    lxs_profile_end(token, hs,
+      hipMemcpyHostToDevice == kind ? "H2D" : (hipMemcpyDeviceToHost == kind ? "D2H" : "D2D"), 0, (double)nbytes);
+  }
   /* a NULL stream makes the copy blocking (src/libxstream_mem.c:1105) */
   if (EXIT_SUCCESS == result && NULL == stream) result = lxs_hip(hipStreamSynchronize(hs), what);
   return result;
@@ -393,25 +400,41 @@ LIBXSTREAM_API int libxstream_mem_zero(void* dev_mem, size_t offset, size_t nbyt
   if (NULL == dev_mem) return lxs_error(EXIT_FAILURE, "libxstream_mem_zero: NULL pointer");
   result = lxs_ensure_device();
   if (EXIT_SUCCESS == result) {
+    void* const token = lxs_profile_begin_mem(hs);
     const int code = lxs_zero_launch((char*)dev_mem + offset, nbytes, (void*)hs);
+    lxs_profile_end(token, hs, "ZERO", 0, (double)nbytes);
     result = (0 == code ? EXIT_SUCCESS : lxs_error(code, "zero fill launch"));
   }
   if (EXIT_SUCCESS == result && NULL == stream) result = lxs_hip(hipStreamSynchronize(hs), "zero fill");
   return result;
 }
 
-/* launch profiling (LIBXSTREAM_PROFILE, src/libxstream.c:2629-2781) ------------------------------ */
+/* launch profiling (LIBXSTREAM_PROFILE, src/libxstream.c:853-1015, :2629-2781) and transfer profiling
+   (LIBXSTREAM_PROFILE_MEM, src/libxstream_mem.c:805-893) ------------------------------------------------
+   Every profiled launch or copy is bracketed by two HIP events on its stream. A completed record becomes a
+   sample {duration, flops, bytes} of its name; durations shorter than LXS_PROFILE_TICKS ticks of the device's
+   wall clock are counted as discarded (quantisation, not a measurement). Each name keeps its totals and the
+   latest LXS_PROFILE_SAMPLES samples, from which the median and a histogram are formed at print time; rates
+   are always taken from a sample's own work and duration. */
 
-#define LXS_PROFILE_NAMES 16
+#define LXS_PROFILE_NAMES 24
+#define LXS_PROFILE_SAMPLES 65536
+#define LXS_PROFILE_TICKS 10
 typedef struct lxs_record_t {
   hipEvent_t begin, end;
   int name;
   double flops, bytes;
 } lxs_record_t;
+typedef struct lxs_sample_t {
+  float ms;
+  double flops, bytes;
+} lxs_sample_t;
 typedef struct lxs_stat_t {
   char name[32];
-  long count;
-  double ms, flops, bytes;
+  long count, discarded;
+  double ms, flops, bytes, min_ms, max_ms;
+  lxs_sample_t* samples; /* ring of the latest LXS_PROFILE_SAMPLES samples */
+  int transfer;          /* a copy or fill: samples are keyed by size */
 } lxs_stat_t;
 
 static pthread_mutex_t lxs_prof_lock = PTHREAD_MUTEX_INITIALIZER;
@@ -421,6 +444,7 @@ static hipEvent_t* lxs_prof_pool = NULL;
 static int lxs_prof_npool = 0, lxs_prof_poolcap = 0;
 static lxs_stat_t lxs_prof_stats[LXS_PROFILE_NAMES];
 static int lxs_prof_nstats = 0;
+static double lxs_prof_timer_ns = 0; /* tick of the event clock (0: unknown) */
 
 static hipEvent_t lxs_prof_event(void) { /* lock held */
   hipEvent_t event = NULL;
@@ -443,15 +467,59 @@ static void lxs_prof_release(hipEvent_t event) { /* lock held */
   lxs_prof_pool[lxs_prof_npool++] = event;
 }
 
+static int lxs_prof_is_transfer(const char* name) {
+  return 0 == strcmp(name, "H2D") || 0 == strcmp(name, "D2H") || 0 == strcmp(name, "D2D") || 0 == strcmp(name, "ZERO");
+}
+
 static int lxs_prof_name(const char* kernel) { /* lock held */
   int i;
   for (i = 0; i < lxs_prof_nstats; ++i) {
     if (0 == strcmp(lxs_prof_stats[i].name, kernel)) return i;
   }
   if (LXS_PROFILE_NAMES <= lxs_prof_nstats) return LXS_PROFILE_NAMES - 1;
+  free(lxs_prof_stats[i].samples);
   memset(lxs_prof_stats + i, 0, sizeof(*lxs_prof_stats));
   snprintf(lxs_prof_stats[i].name, sizeof(lxs_prof_stats[i].name), "%s", kernel);
+  lxs_prof_stats[i].transfer = lxs_prof_is_transfer(kernel);
   return lxs_prof_nstats++;
+}
+
+static double lxs_prof_floor_ms(void) {
+  if (0 == lxs_prof_timer_ns) {
+    int khz = 0;
+    /* the events' clock: the device's constant-rate wall clock */
+    if (0 <= lxs_cfg.device && hipSuccess == hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, lxs_cfg.device) &&
+        0 < khz) {
+      lxs_prof_timer_ns = 1e6 / khz;
+    }
+    else {
+      (void)hipGetLastError();
+      lxs_prof_timer_ns = 10.0;
+    }
+  }
+  return 1e-6 * LXS_PROFILE_TICKS * lxs_prof_timer_ns;
+}
+
+static void lxs_prof_push(lxs_stat_t* st, double ms, double flops, double bytes) { /* lock held */
+  if (ms < lxs_prof_floor_ms()) {
+    ++st->discarded;
+    if (0 > lxs_cfg.profile) fprintf(stderr, "PROF ACC/HIP: %s ms=%.4f (below the floor, discarded)\n", st->name, ms);
+    return;
+  }
+  if (NULL == st->samples) st->samples = (lxs_sample_t*)malloc(sizeof(lxs_sample_t) * LXS_PROFILE_SAMPLES);
+  if (NULL != st->samples) {
+    lxs_sample_t* const x = st->samples + (st->count % LXS_PROFILE_SAMPLES);
+    x->ms = (float)ms;
+    x->flops = flops;
+    x->bytes = bytes;
+  }
+  st->min_ms = (0 == st->count || ms < st->min_ms ? ms : st->min_ms);
+  st->max_ms = (0 == st->count || ms > st->max_ms ? ms : st->max_ms);
+  ++st->count;
+  st->ms += ms;
+  st->flops += flops;
+  st->bytes += bytes;
+  if (0 > lxs_cfg.profile) fprintf(stderr, "PROF ACC/HIP: %s ms=%.4f\n", st->name, ms);
 }
 
 /* fold completed records into the statistics; blocking waits for all of them (lock held) */
@@ -463,11 +531,7 @@ static void lxs_prof_drain(int blocking) {
     if (hipSuccess == state) {
       float ms = 0;
       if (hipSuccess == hipEventElapsedTime(&ms, r->begin, r->end)) {
-        lxs_stat_t* const s = lxs_prof_stats + r->name;
-        ++s->count;
-        s->ms += ms;
-        s->flops += r->flops;
-        s->bytes += r->bytes;
+        lxs_prof_push(lxs_prof_stats + r->name, ms, r->flops, r->bytes);
       }
       lxs_prof_release(r->begin);
       lxs_prof_release(r->end);
@@ -480,9 +544,30 @@ static void lxs_prof_drain(int blocking) {
   lxs_prof_npending = j;
 }
 
-void* lxs_profile_begin(hipStream_t stream) {
+static int lxs_prof_cmp_ms(const void* a, const void* b) {
+  const float x = ((const lxs_sample_t*)a)->ms, y = ((const lxs_sample_t*)b)->ms;
+  return (x < y ? -1 : (x > y ? 1 : 0));
+}
+
+static int lxs_prof_cmp_bytes(const void* a, const void* b) {
+  const double x = ((const lxs_sample_t*)a)->bytes, y = ((const lxs_sample_t*)b)->bytes;
+  return (x < y ? -1 : (x > y ? 1 : 0));
+}
+
+/* the kept samples of a statistic, sorted by duration (kernels) or by size (transfers); caller frees */
+static lxs_sample_t* lxs_prof_sorted(const lxs_stat_t* st, long* n) { /* lock held */
+  const long count = (st->count < LXS_PROFILE_SAMPLES ? st->count : LXS_PROFILE_SAMPLES);
+  lxs_sample_t* const x = (0 < count && NULL != st->samples ? (lxs_sample_t*)malloc(sizeof(lxs_sample_t) * count) : NULL);
+  *n = 0;
+  if (NULL == x) return NULL;
+  memcpy(x, st->samples, sizeof(lxs_sample_t) * (size_t)count);
+  qsort(x, (size_t)count, sizeof(lxs_sample_t), 0 != st->transfer ? lxs_prof_cmp_bytes : lxs_prof_cmp_ms);
+  *n = count;
+  return x;
+}
+
+static void* lxs_profile_begin_any(hipStream_t stream) {
   hipEvent_t event;
-  if (0 == lxs_cfg.profile) return NULL;
   pthread_mutex_lock(&lxs_prof_lock);
   event = lxs_prof_event();
   pthread_mutex_unlock(&lxs_prof_lock);
@@ -493,6 +578,16 @@ void* lxs_profile_begin(hipStream_t stream) {
     event = NULL;
   }
   return (void*)event;
+}
+
+void* lxs_profile_begin(hipStream_t stream) {
+  if (0 == lxs_cfg.profile) return NULL;
+  return lxs_profile_begin_any(stream);
+}
+
+void* lxs_profile_begin_mem(hipStream_t stream) {
+  if (0 == lxs_cfg.profile_mem) return NULL;
+  return lxs_profile_begin_any(stream);
 }
 
 void lxs_profile_end(void* token, hipStream_t stream, const char* kernel, double flops, double bytes) {
@@ -531,9 +626,19 @@ LIBXSTREAM_API int libxstream_profile_enable(int enable) {
   return EXIT_SUCCESS;
 }
 
+LIBXSTREAM_API int libxstream_profile_mem_enable(int enable) {
+  lxs_cfg.profile_mem = enable;
+  return EXIT_SUCCESS;
+}
+
 LIBXSTREAM_API int libxstream_profile_reset(void) {
+  int i;
   pthread_mutex_lock(&lxs_prof_lock);
   lxs_prof_drain(1);
+  for (i = 0; i < lxs_prof_nstats; ++i) {
+    free(lxs_prof_stats[i].samples);
+    lxs_prof_stats[i].samples = NULL;
+  }
   lxs_prof_nstats = 0;
   pthread_mutex_unlock(&lxs_prof_lock);
   return EXIT_SUCCESS;
@@ -564,17 +669,132 @@ LIBXSTREAM_API int libxstream_profile_get(const char* kernel, long* count, doubl
   return EXIT_SUCCESS;
 }
 
+LIBXSTREAM_API int libxstream_profile_stats(const char* name, double* min_ms, double* median_ms, double* max_ms,
+  double* median_rate, long* discarded) {
+  int i, found = -1;
+  if (NULL == name) return lxs_error(EXIT_FAILURE, "libxstream_profile_stats: NULL name");
+  pthread_mutex_lock(&lxs_prof_lock);
+  lxs_prof_drain(1);
+  for (i = 0; i < lxs_prof_nstats && 0 > found; ++i) {
+    if (0 == strcmp(lxs_prof_stats[i].name, name)) found = i;
+  }
+  if (NULL != min_ms) *min_ms = 0;
+  if (NULL != median_ms) *median_ms = 0;
+  if (NULL != max_ms) *max_ms = 0;
+  if (NULL != median_rate) *median_rate = 0;
+  if (NULL != discarded) *discarded = 0;
+  if (0 <= found) {
+    const lxs_stat_t* const st = lxs_prof_stats + found;
+    long n = 0;
+    lxs_sample_t* const x = lxs_prof_sorted(st, &n);
+    if (NULL != min_ms) *min_ms = st->min_ms;
+    if (NULL != max_ms) *max_ms = st->max_ms;
+    if (NULL != discarded) *discarded = st->discarded;
+    if (NULL != x && 0 < n) {
+      const lxs_sample_t* const mid = x + n / 2; /* the median sample: by duration, or by size for transfers */
+      if (NULL != median_ms) *median_ms = mid->ms;
+      if (NULL != median_rate && 0 < mid->ms) { /* GB/s of a transfer, GFLOP/s of a kernel */
+        *median_rate = 1e-6 * (0 != st->transfer ? mid->bytes : mid->flops) / mid->ms;
+      }
+    }
+    free(x);
+  }
+  pthread_mutex_unlock(&lxs_prof_lock);
+  return (0 <= found ? EXIT_SUCCESS : EXIT_FAILURE);
+}
+
+/* one statistic as the reference prints it: the median sample's duration and rates (kernels) or the rate of
+   the most populated size bucket (transfers), then nbuckets rows of counts and per-bucket averages */
+static void lxs_prof_print_stat(FILE* stream, const lxs_stat_t* st, int nbuckets) { /* lock held */
+  long n = 0, b, i0 = 0;
+  lxs_sample_t* const x = lxs_prof_sorted(st, &n);
+  if (NULL == x || 0 == n) {
+    free(x);
+    return;
+  }
+  if (0 != st->transfer) { /* buckets of sizes on a log2 axis; the mode bucket gives the headline rate */
+    const double lo = log2(x[0].bytes > 1 ? x[0].bytes : 1), hi = log2(x[n - 1].bytes > 1 ? x[n - 1].bytes : 1);
+    double best_rate = 0;
+    long best = -1;
+    for (b = 0; b < nbuckets; ++b) {
+      double sb = 0, sms = 0;
+      long cnt = 0;
+      const double edge = (b + 1 == nbuckets ? hi + 1 : lo + (hi - lo) * (b + 1) / nbuckets);
+      while (i0 < n && log2(x[i0].bytes > 1 ? x[i0].bytes : 1) < edge) {
+        sb += x[i0].bytes;
+        sms += x[i0].ms;
+        ++cnt;
+        ++i0;
+      }
+      if (cnt > best && 0 < sms) {
+        best = cnt;
+        best_rate = 1e-6 * sb / sms;
+      }
+    }
+    fprintf(stream, "PROF ACC/HIP: %s=%.1f GB/s n=%li\n", st->name, best_rate, st->count);
+    for (b = 0, i0 = 0; b < nbuckets; ++b) {
+      double sb = 0, sms = 0;
+      long cnt = 0;
+      const double edge = (b + 1 == nbuckets ? hi + 1 : lo + (hi - lo) * (b + 1) / nbuckets);
+      while (i0 < n && log2(x[i0].bytes > 1 ? x[i0].bytes : 1) < edge) {
+        sb += x[i0].bytes;
+        sms += x[i0].ms;
+        ++cnt;
+        ++i0;
+      }
+      if (0 < cnt) {
+        fprintf(stream, "  %.1f MB x%li: %.1f us %.1f GB/s\n", 1e-6 * sb / cnt, cnt, 1e3 * sms / cnt,
+          0 < sms ? 1e-6 * sb / sms : 0.0);
+      }
+    }
+  }
+  else {
+    const lxs_sample_t* const mid = x + n / 2;
+    const double lo = x[0].ms, hi = x[n - 1].ms;
+    fprintf(stream, "PROF ACC/HIP: %s=%.4f ms", st->name, (double)mid->ms);
+    if (0 < mid->flops) fprintf(stream, " %.1f GFLOPS/s", 1e-6 * mid->flops / mid->ms);
+    if (0 < mid->bytes) fprintf(stream, " %.1f GB/s", 1e-6 * mid->bytes / mid->ms);
+    fprintf(stream, " n=%li min=%.4f max=%.4f ms\n", st->count, st->min_ms, st->max_ms);
+    for (b = 0; b < nbuckets && lo < hi; ++b) {
+      double sf = 0, sb = 0, sms = 0;
+      long cnt = 0;
+      const double edge = (b + 1 == nbuckets ? hi * 2 : lo + (hi - lo) * (b + 1) / nbuckets);
+      while (i0 < n && x[i0].ms < edge) {
+        sf += x[i0].flops;
+        sb += x[i0].bytes;
+        sms += x[i0].ms;
+        ++cnt;
+        ++i0;
+      }
+      if (0 < cnt) {
+        fprintf(stream, "  <%.4f ms x%li: %.4f ms", (b + 1 == nbuckets ? hi : edge), cnt, sms / cnt);
+        if (0 < sf) fprintf(stream, " %.1f GFLOPS/s", 1e-6 * sf / sms);
+        if (0 < sb) fprintf(stream, " %.1f GB/s", 1e-6 * sb / sms);
+        fprintf(stream, "\n");
+      }
+    }
+  }
+  free(x);
+}
+
 LIBXSTREAM_API int libxstream_profile_print(FILE* stream) {
+  const int setting = (0 <= lxs_cfg.profile ? lxs_cfg.profile : -lxs_cfg.profile);
+  const int nbuckets = (2 < setting ? setting : 2) + 1;
+  long discarded = 0;
   int i;
   if (NULL == stream) stream = stderr;
   pthread_mutex_lock(&lxs_prof_lock);
   lxs_prof_drain(1);
   for (i = 0; i < lxs_prof_nstats; ++i) {
-    const lxs_stat_t* const s = lxs_prof_stats + i;
-    if (0 < s->count && 0 < s->ms) {
-      fprintf(stream, "PROF ACC/HIP: %s n=%li avg=%.4f ms %.1f GFLOPS/s %.1f GB/s\n", s->name, s->count,
-        s->ms / (double)s->count, 1E-6 * s->flops / s->ms, 1E-6 * s->bytes / s->ms);
-    }
+    lxs_prof_print_stat(stream, lxs_prof_stats + i, nbuckets);
+    discarded += lxs_prof_stats[i].discarded;
+  }
+  if (0 < discarded) {
+    fprintf(stream, "PROF ACC/HIP: discarded=%li timer=%.0fns floor=%.0fns\n", discarded, lxs_prof_timer_ns,
+      LXS_PROFILE_TICKS * lxs_prof_timer_ns);
+  }
+  else if (0 == lxs_prof_nstats) {
+    fprintf(stream, "PROF ACC/HIP: no samples recorded\n");
   }
   pthread_mutex_unlock(&lxs_prof_lock);
   return EXIT_SUCCESS;
@@ -582,9 +802,15 @@ LIBXSTREAM_API int libxstream_profile_print(FILE* stream) {
 
 void lxs_profile_finalize(void) {
   int i;
-  if (0 != lxs_cfg.profile && 0 < lxs_prof_nstats) libxstream_profile_print(stderr);
+  if ((0 != lxs_cfg.profile || 0 != lxs_cfg.profile_mem) && 0 < lxs_prof_nstats) libxstream_profile_print(stderr);
   pthread_mutex_lock(&lxs_prof_lock);
+  lxs_prof_drain(1);
   for (i = 0; i < lxs_prof_npool; ++i) (void)hipEventDestroy(lxs_prof_pool[i]);
   lxs_prof_npool = 0;
+  for (i = 0; i < lxs_prof_nstats; ++i) {
+    free(lxs_prof_stats[i].samples);
+    lxs_prof_stats[i].samples = NULL;
+  }
+  lxs_prof_nstats = 0;
   pthread_mutex_unlock(&lxs_prof_lock);
 }
