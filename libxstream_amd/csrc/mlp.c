@@ -478,7 +478,12 @@ LIBXSTREAM_API int libsmm_acc_mlp_load(libsmm_acc_mlp_t** model, const char* fil
  * launch path. Weights are walked input-major (a transposed copy), so that the compiler vectorises
  * across outputs without changing any output's summation order.
  */
-__attribute__((target_clones("fma", "default"))) static int mlp_forward_host(const libsmm_acc_mlp_t* model,
+#if defined(__SANITIZE_THREAD__) /* a clone's resolver runs before the ThreadSanitizer runtime is up */
+#define LXS_MLP_CLONES
+#else
+#define LXS_MLP_CLONES __attribute__((target_clones("fma", "default")))
+#endif
+LXS_MLP_CLONES static int mlp_forward_host(const libsmm_acc_mlp_t* model,
   const float* x, int batch, float* out) {
   int wmax = 0, l, net, b, i, o;
   float *wt, *act0, *act1;
