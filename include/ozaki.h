@@ -82,6 +82,19 @@ LIBXSTREAM_API int ozaki_npanel(const ozaki_context_t* ctx, int M, int N, int tm
 LIBXSTREAM_API int ozaki_gemm(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M, int N,
   int K, double alpha, const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc, int dev);
 
+/**
+ * No counterpart in the reference: ozaki_gemm for callers that cannot allocate device memory (code under
+ * HIP graph capture, serving loops, harnesses that forbid hipMalloc). Same arguments, scheme, kernels and
+ * results as ozaki_gemm with dev = 1 (a, b, c device pointers); its scratch (digit or residue planes,
+ * exponents, partial products) lives in a static device array of 64 MiB per device that the library
+ * holds until the process ends, so the call allocates nothing. A call whose scratch would exceed 64 MiB
+ * (FP64 Scheme 2 with 16 moduli: up to about M = N = K = 1024) fails with EXIT_FAILURE and a message.
+ * Every call on one device shares that device's array: calls that overlap on one device (several streams
+ * or threads) race and must be ordered by the caller.
+ */
+LIBXSTREAM_API int ozaki_gemm_noalloc(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb,
+  int M, int N, int K, double alpha, const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc);
+
 /** samples/ozaki/ozaki_opencl.h:430 (3M method); alpha/beta point to (re, im); device pointers */
 LIBXSTREAM_API int ozaki_gemm_complex(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M,
   int N, int K, const double* alpha, const void* a, int lda, const void* b, int ldb, const double* beta, void* c, int ldc);

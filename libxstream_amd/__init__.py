@@ -185,6 +185,8 @@ def _declare(L):
         "ozaki_scratch_get": (vp, [ctypes.POINTER(OzakiContext), psz]),
         "ozaki_gemm": (i, [ctypes.POINTER(OzakiContext), vp, ctypes.c_char, ctypes.c_char, i, i, i, ctypes.c_double,
                            vp, i, vp, i, ctypes.c_double, vp, i, i]),
+        "ozaki_gemm_noalloc": (i, [ctypes.POINTER(OzakiContext), vp, ctypes.c_char, ctypes.c_char, i, i, i,
+                                   ctypes.c_double, vp, i, vp, i, ctypes.c_double, vp, i]),
         "ozaki_gemm_complex": (i, [ctypes.POINTER(OzakiContext), vp, ctypes.c_char, ctypes.c_char, i, i, i,
                                    ctypes.POINTER(ctypes.c_double), vp, i, vp, i, ctypes.POINTER(ctypes.c_double), vp,
                                    i]),
@@ -553,6 +555,13 @@ class Ozaki:
         check(lib().ozaki_gemm(ctypes.byref(self.ctx), stream.handle if stream else None, transa.encode(),
                                transb.encode(), m, n, k, alpha, ptr(a), lda, ptr(b), ldb, beta, ptr(c), ldc,
                                1 if dev else 0), "ozaki_gemm")
+
+    def gemm_noalloc(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
+        """gemm on DeviceBuffers with the library's static device scratch (no allocation; include/ozaki.h:
+        ozaki_gemm_noalloc)."""
+        check(lib().ozaki_gemm_noalloc(ctypes.byref(self.ctx), stream.handle if stream else None, transa.encode(),
+                                       transb.encode(), m, n, k, alpha, a.ptr, lda, b.ptr, ldb, beta, c.ptr, ldc),
+              "ozaki_gemm_noalloc")
 
     def gemm_complex(self, transa, transb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc, stream=None):
         """Complex C = alpha op(A) op(B) + beta C on DeviceBuffers of interleaved (re, im) values."""

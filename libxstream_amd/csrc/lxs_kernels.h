@@ -91,6 +91,11 @@ int lxs_oz_level_gemm(int crt, const signed char* pa, const signed char* pb, con
 int lxs_oz_recombine(int dtype, const int* c32, const int* zb, int cutoff, int Mp, int Np, const int* ea,
   const int* eb, int M, int N, double alpha, double beta, void* c, int ldc, void* hip_stream);
 int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* hip_stream);
+/* bytes of the per-device static scratch of ozaki_gemm_noalloc (ozaki.hip) */
+#define LXS_OZ_STATIC_BYTES ((size_t)64 << 20)
+/* device address of that scratch on the current device (looked up once per device): 0, a hipError_t or
+   LXS_ELIMIT */
+int lxs_oz_static_scratch(void** ptr);
 /* Ozaki scheme 2 (CRT): moduli in use order, pairwise coprime and at most 256, so that symmetric residues
    fit int8; the first np of them serve a call. Residue planes: np x Rp x Kp int8 of the integers
    trunc(|v| 2^(mb - trim - E)) (E: the row's exponent, mb = 53 or 24) with the sign of v. */

@@ -21,6 +21,7 @@
 #define OZ_FLUSH_CRT 2047 /* units per Scheme 2 work item: 2047 * 64 * 128^2 < 2^31 */
 #define OZ_WG_TARGET 1024 /* work-groups the item plan aims for */
 #define OZ_KPASS_MAX ((1 << 22) - OZ_TILE_K) /* K per pass (32-bit operand offsets of the int8 GEMM) */
+#define OZ_NOALLOC_ITEMS ((int)(LXS_OZ_STATIC_BYTES / ((size_t)OZ_TILE_MN * OZ_TILE_MN)))
 #define OZ_XOVER 16.0     /* adaptive choice: reconstruction cost per prime^2, in int8 GEMMs of K = 1 */
 
 typedef unsigned __int128 oz_u128_t;
@@ -115,6 +116,22 @@ LIBXSTREAM_API int ozaki_init(ozaki_context_t* ctx, int tm, int tn, int use_doub
   if (2 != kind && oztrim > 2 * (nslices - 1)) oztrim = 2 * (nslices - 1);
   impl = (oz_impl_t*)calloc(1, sizeof(oz_impl_t));
   if (NULL == impl) return EXIT_FAILURE;
+  /* work items for any call that fits the static scratch of ozaki_gemm_noalloc (an item's output is at
+     least one byte per element of a 256 x 256 tile), so that entry point never grows the array */
+  impl->items = (int*)malloc(sizeof(int) * 4 * OZ_NOALLOC_ITEMS);
+  if (NULL == impl->items) {
+    free(impl);
+    return EXIT_FAILURE;
+  }
+  impl->items_cap = OZ_NOALLOC_ITEMS;
+  if (0 <= lxs_cfg.device && 0 == lxs_ensure_device()) { /* the static scratch's address on the active device */
+    void* scratch = NULL;
+    if (0 != lxs_oz_static_scratch(&scratch)) {
+      free(impl->items);
+      free(impl);
+      return lxs_error(EXIT_FAILURE, "ozaki_init: device address of the static scratch");
+    }
+  }
   ctx->use_double = (0 != use_double ? 1 : 0);
   ctx->kind = kind;
   ctx->nslices = nslices;
@@ -359,8 +376,11 @@ static char* oz_scratch(oz_impl_t* impl, size_t nbytes, hipStream_t hs) {
   return (char*)impl->arena;
 }
 
+/* fixed != NULL: the call's scratch is fixed[0 .. fixed_size) (ozaki_gemm_noalloc): nothing is allocated,
+   device operands only, no profile records; a call that needs more fails */
 static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, int M, int N, int K, double alpha,
-  const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc, int dev) {
+  const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc, int dev, char* fixed,
+  size_t fixed_size) {
   oz_impl_t* const impl = (oz_impl_t*)ctx->impl;
   const int dtype = (0 != ctx->use_double ? 3 : 1);
   const size_t ts = (0 != ctx->use_double ? 8 : 4);
@@ -379,8 +399,18 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
   if ((0 != K && (NULL == a || NULL == b)) || NULL == c) return lxs_error(EXIT_FAILURE, "ozaki_gemm: NULL operand");
   oz_layout_call(ctx, ta, tb, M, N, K, lda, ldb, ldc, dev, &l);
   if (0 == l.scheme) return lxs_error(EXIT_FAILURE, "ozaki_gemm: too few CRT primes for an exact product");
-  base = oz_scratch(impl, l.total, hs);
-  if (NULL == base) return EXIT_FAILURE;
+  if (NULL != fixed) {
+    if (0 == dev) return lxs_error(EXIT_FAILURE, "ozaki_gemm_noalloc: operands must be device pointers");
+    if (l.total > fixed_size || l.nitems > impl->items_cap) {
+      return lxs_error(EXIT_FAILURE, "ozaki_gemm_noalloc: the call needs more than the static scratch "
+                                     "(LXS_OZ_STATIC_BYTES)");
+    }
+    base = fixed;
+  }
+  else {
+    base = oz_scratch(impl, l.total, hs);
+    if (NULL == base) return EXIT_FAILURE;
+  }
   if (0 == dev) { /* host operands: copy in */
     da = base + l.a;
     db = base + l.b;
@@ -425,7 +455,7 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
       const int nitems = oz_plan(ctx, l.crt, np, Kp / OZ_UNIT_K, tiles, impl->items, zb);
       const double nprod = (0 != l.crt ? np : ozaki_count_pairs(s, cutoff, 0));
       const int nplanes = (0 != l.crt ? np : s);
-      void* token = lxs_profile_begin(hs);
+      void* token = (NULL == fixed ? lxs_profile_begin(hs) : NULL);
       if (0 != l.crt) {
         result = lxs_oz_crt_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, np, trim, l.Mp, Kp,
           (int*)(base + l.ea), (signed char*)(base + l.planes_a), hs);
@@ -444,7 +474,7 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
       }
       lxs_profile_end(token, hs, "ozaki_pre", 0.0, (double)ts * ((double)M + N) * kk + (double)nplanes * (l.Mp + l.Np) * Kp);
       if (0 == result) { /* int8 work of every pair up to the cutoff (pairs of all-zero planes are skipped), or every prime */
-        token = lxs_profile_begin(hs);
+        token = (NULL == fixed ? lxs_profile_begin(hs) : NULL);
         result = lxs_oz_level_gemm(l.crt, (const signed char*)(base + l.planes_a), (const signed char*)(base + l.planes_b),
           occ, occ + 8, impl->items, nitems, nplanes, l.Mp, l.Np, Kp, base + l.out, hs);
         lxs_profile_end(token, hs, "ozaki_i8", 2.0 * nprod * l.Mp * l.Np * Kp,
@@ -452,7 +482,7 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
       }
       if (0 == result) {
         const double bc = (double)ts * M * N * (0 == k0 && 0 == beta ? 1 : 2);
-        token = lxs_profile_begin(hs);
+        token = (NULL == fixed ? lxs_profile_begin(hs) : NULL);
         if (0 != l.crt) {
           result = lxs_oz_crt_recombine(dtype, (const unsigned char*)(base + l.out), zb, np, l.Mp, l.Np,
             (const int*)(base + l.ea), (const int*)(base + l.eb), M, N, 2 * trim - 2 * mb, alpha, (0 == k0 ? beta : 1.0), dc,
@@ -485,9 +515,23 @@ LIBXSTREAM_API int ozaki_gemm(ozaki_context_t* ctx, libxstream_stream_t* stream,
   if (0 != lxs_ensure_device()) return EXIT_FAILURE;
   hs = lxs_stream_hip(stream);
   token = lxs_profile_begin(hs);
-  result = oz_gemm_real(ctx, hs, ta, tb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, dev);
+  result = oz_gemm_real(ctx, hs, ta, tb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, dev, NULL, 0);
   lxs_profile_end(token, hs, "ozaki", 2.0 * M * N * K, (double)(0 != ctx->use_double ? 8 : 4) * ((double)M * K + (double)K * N + 2.0 * M * N));
   return result;
+}
+
+LIBXSTREAM_API int ozaki_gemm_noalloc(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb,
+  int M, int N, int K, double alpha, const void* a, int lda, const void* b, int ldb, double beta, void* c, int ldc) {
+  const int ta = oz_trans(transa), tb = oz_trans(transb);
+  void* scratch = NULL;
+  if (NULL == ctx || NULL == ctx->impl) return lxs_error(EXIT_FAILURE, "ozaki_gemm_noalloc: context not initialised");
+  if (0 > ta || 0 > tb || 0 > M || 0 > N || 0 > K) return lxs_error(EXIT_FAILURE, "ozaki_gemm_noalloc: invalid argument");
+  if (0 != lxs_ensure_device()) return EXIT_FAILURE;
+  if (0 != lxs_oz_static_scratch(&scratch)) {
+    return lxs_error(EXIT_FAILURE, "ozaki_gemm_noalloc: device address of the static scratch");
+  }
+  return oz_gemm_real(ctx, lxs_stream_hip(stream), ta, tb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc, 1,
+    (char*)scratch, LXS_OZ_STATIC_BYTES);
 }
 
 LIBXSTREAM_API int ozaki_gemm_complex(ozaki_context_t* ctx, libxstream_stream_t* stream, char transa, char transb, int M,

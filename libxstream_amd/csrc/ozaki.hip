@@ -31,6 +31,7 @@
 #include <stdint.h>
 #include <string.h>
 #include <stdlib.h>
+#include <pthread.h>
 #include "lxs_kernels.h"
 
 namespace {
@@ -938,6 +939,35 @@ static const int8_t* oz_zero_ptr() {
     ptr[dev] = (const int8_t*)p;
   }
   return ptr[dev];
+}
+
+/*
+ * Device scratch of ozaki_gemm_noalloc: one array of LXS_OZ_STATIC_BYTES per device, held until the
+ * process ends; every call on a device shares it, so calls that overlap on one device race. On the host
+ * the array's name only stands in for the symbol: its device address comes from hipGetSymbolAddress,
+ * looked up once per device with that device current, and only that address reaches kernels.
+ */
+static __device__ __align__(256) unsigned char scratch[LXS_OZ_STATIC_BYTES];
+
+extern "C" int lxs_oz_static_scratch(void** ptr) {
+  static pthread_mutex_t lock = PTHREAD_MUTEX_INITIALIZER;
+  static void* addr[64];
+  int dev = 0, result = 0;
+  if (NULL == ptr) return LXS_ELIMIT;
+  *ptr = NULL;
+  result = (int)hipGetDevice(&dev);
+  if (0 != result) return result;
+  if (0 > dev || 64 <= dev) return LXS_ELIMIT;
+  pthread_mutex_lock(&lock);
+  if (NULL == addr[dev]) {
+    void* p = NULL;
+    result = (int)hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (0 == result && NULL == p) result = LXS_ELIMIT;
+    if (0 == result) addr[dev] = p;
+  }
+  *ptr = addr[dev];
+  pthread_mutex_unlock(&lock);
+  return result;
 }
 
 /* items: host array of nitems x {level, u0, u1, pad}; item z writes at out + z * Mp * Np (int32 partials,
