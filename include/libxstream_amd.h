@@ -19,15 +19,18 @@
  * not meaningful for these kernels.
  *   variant: 0 = vector-FMA kernel, operands staged through LDS
  *            1 = matrix-core (MFMA 16x16x4) kernel, operands loaded straight into registers
+ *            2 = matrix-core kernel whose wavefronts share each C-run's entries (entry-parallel), their
+ *                partial blocks summed in LDS in a fixed order before the one flush of the run
  *   tm, tn:  variant 0: C microtile (rows x columns) owned by one thread;
- *            variant 1: number of 16x16 output tiles along m and n (padded cover of the C block)
+ *            variants 1, 2: number of 16x16 output tiles along m and n (padded cover of the C block)
  *   nt:      threads per work-group; variant 0: 64, 128, 256 or 512, the threads left after
  *            covering one C block with microtiles form further groups that share the run's
- *            stack entries; variant 1: 64 per wavefront, the wavefronts split the tile rows
+ *            stack entries; variant 1: 64 per wavefront, the wavefronts split the tile rows;
+ *            variant 2: 64 per wavefront, the wavefronts split the run's entries
  *   bs:      nominal stack entries per work-group; a work-group owns the C-runs whose first entry
  *            falls in its range and finishes them, so each C block is written by one work-group
  *   kt:      variant 0: k-slice staged in LDS at once (0 = all of k when it fits);
- *            variant 1: k per register stage (a multiple of 4 fixed by the kernel; 0 = any)
+ *            variants 1, 2: k per register stage (a multiple of 4 fixed by the kernel; 0 = any)
  *   flush:   0 = automatic, 1 = plain read-modify-write (stack grouped by C), 2 = atomic add.
  *            Configurations reached through libsmm_acc_process (tuned CSV, learned predictor, built-in
  *            default, environment) and the batch always flush with atomic adds, which is correct for
@@ -39,7 +42,9 @@ typedef struct libsmm_acc_smm_config_t {
   int s;         /* stack size bs was chosen for (CSV column S); shorter stacks launch with bs scaled down
                     (samples/smm/smm_kernel.c:463-473); 0: bs as given */
   int split;     /* C-runs longer than split x bs entries are cut into pieces added atomically (CSV column
-                    SPLIT); 0: LIBXSTREAM_SMM_SPLIT (default 2) */
+                    SPLIT); 0: LIBXSTREAM_SMM_SPLIT (default 2). In stacks of at least 16384 entries runs of
+                    up to 64 entries are never cut (bitwise-reproducible C); LIBXSTREAM_SMM_SPLIT_MIN=n
+                    sets that floor to n entries for every stack */
   int bs_short, split_short; /* used instead of bs and split for stacks of at most s entries (CSV columns
                     BSS, SPLITS); 0: none. Tuned where they win at s but lose on longer stacks. */
   double gflops; /* measured (autotuner), predicted (learned model) or 0 */

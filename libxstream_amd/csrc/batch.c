@@ -88,7 +88,7 @@ LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const 
     d->bs = chunk_elements / (s->m * s->k + s->k * s->n);
     if (d->bs < 8) d->bs = 8;
     if (d->bs > 512) d->bs = 512;
-    d->tsplit = lxs_smm_tsplit(d->bs, 0);
+    d->tsplit = lxs_smm_tsplit(d->bs, 0, s->stack_size);
     if (4 < d->ti * d->tj) nbig += (s->stack_size + d->bs - 1) / d->bs;
     else nsmall += (s->stack_size + d->bs - 1) / d->bs;
     b->flops += 2.0 * s->m * s->n * s->k * (double)s->stack_size;

@@ -30,7 +30,8 @@ typedef struct lxs_config_t {
   int profile;   /* LIBXSTREAM_PROFILE: launch records (|value| + 1 histogram buckets; < 0 also prints each) */
   int profile_mem; /* LIBXSTREAM_PROFILE_MEM: copy and fill records (H2D, D2H, D2D, ZERO) */
   int split;     /* LIBXSTREAM_SMM_SPLIT: C-runs are cut past max(split*bs, split_min) entries (0: never) */
-  int split_min; /* LIBXSTREAM_SMM_SPLIT_MIN: runs of up to this many entries are never cut (default 1) */
+  int split_min; /* LIBXSTREAM_SMM_SPLIT_MIN: runs of up to this many entries are never cut (0: 64 in stacks of
+                    at least LXS_SMM_SPLIT_FILL entries, none in shorter ones) */
   int grouped;   /* LIBXSTREAM_SMM_GROUPED: the caller guarantees that every stack holds each C block in one
                     contiguous run, which lets configurations with flush 1 write C without atomics (default 0) */
   int ncus;      /* compute units of the active device */
@@ -65,8 +66,10 @@ void lxs_learned_load_default(const char* path);
 int lxs_smm_init(void);
 void lxs_smm_finalize(void);
 /* C-runs of a launch with bs entries per work-group are cut past this many entries (0: never);
-   split: the configuration's factor (0: the library default) */
-int lxs_smm_tsplit(int bs, int split);
+   split: the configuration's factor (0: the library default); stacks of at least LXS_SMM_SPLIT_FILL
+   entries keep runs of up to 64 entries whole (unless LIBXSTREAM_SMM_SPLIT_MIN says otherwise) */
+#define LXS_SMM_SPLIT_FILL 16384
+int lxs_smm_tsplit(int bs, int split, int stack_size);
 /* one stack with its resolved configuration on a HIP stream, no profile records (hipError_t or LXS_E*) */
 int lxs_smm_enqueue(int datatype, const int* dev_param_stack, int stack_size, const void* a, const void* b, void* c,
   int m, int n, int k, void* hip_stream);

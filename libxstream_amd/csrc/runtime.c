@@ -7,7 +7,7 @@
 #include <math.h>
 #include <string.h>
 
-lxs_config_t lxs_cfg = {.device = -1, .split_min = 1, .lock = PTHREAD_MUTEX_INITIALIZER};
+lxs_config_t lxs_cfg = {.device = -1, .lock = PTHREAD_MUTEX_INITIALIZER};
 
 static __thread int lxs_tls_code = 0;
 static __thread char lxs_tls_msg[256];
@@ -101,7 +101,7 @@ LIBXSTREAM_API int libxstream_init_config(const libxstream_init_config_t* cfg) {
     lxs_cfg.profile = lxs_getenv_int("LIBXSTREAM_PROFILE", 0);
     lxs_cfg.profile_mem = lxs_getenv_int("LIBXSTREAM_PROFILE_MEM", 0);
     lxs_cfg.split = lxs_getenv_int("LIBXSTREAM_SMM_SPLIT", 2);
-    lxs_cfg.split_min = lxs_getenv_int("LIBXSTREAM_SMM_SPLIT_MIN", 1);
+    lxs_cfg.split_min = lxs_getenv_int("LIBXSTREAM_SMM_SPLIT_MIN", 0);
     lxs_cfg.grouped = lxs_getenv_int("LIBXSTREAM_SMM_GROUPED", 0);
     /* no visible device is not an error here: libxstream_device_count then reports zero */
     if (hipSuccess != hipGetDeviceCount(&ndevices)) {

@@ -587,14 +587,17 @@ LIBXSTREAM_API c_dbcsr_acc_bool_t libsmm_acc_process_suitable(c_dbcsr_acc_bool_t
 
 /* A C-run is summed by the work-group where it starts, unless it is longer than tsplit = split x bs
    entries (the configuration's split, else LIBXSTREAM_SMM_SPLIT, default 2; LIBXSTREAM_SMM_SPLIT=0 never
-   cuts): then it is cut at work-group boundaries and its pieces are added with
-   atomics, so one long run (many k-blocks into one C block) does not leave a single work-group working
-   alone, and a short stack (bs scaled down) spreads its runs over many work-groups. Runs of up to
-   LIBXSTREAM_SMM_SPLIT_MIN entries are never cut (default 1; 64 keeps such runs bit-identical for
-   every bs, at the price of short stacks). */
-int lxs_smm_tsplit(int bs, int split) {
+   cuts): then it is cut at work-group boundaries and its pieces are added with atomics, so one long run
+   (many k-blocks into one C block) does not leave a single work-group working alone. Floor: in a stack of
+   at least LXS_SMM_SPLIT_FILL entries (enough C-runs to occupy the device), runs of up to 64 entries are
+   never cut: each is summed in one work-group in a fixed order and written once, so its C block gets the
+   same bits on every call and for every bs. Shorter stacks may cut any run longer than tsplit, which
+   spreads their few runs over more work-groups. LIBXSTREAM_SMM_SPLIT_MIN=n sets the floor to n entries
+   for every stack (1: no floor). */
+int lxs_smm_tsplit(int bs, int split, int stack_size) {
   const int factor = (0 < split ? split : lxs_cfg.split);
-  const int t = factor * (1 < bs ? bs : 1), floor = (0 < lxs_cfg.split_min ? lxs_cfg.split_min : 1);
+  const int t = factor * (1 < bs ? bs : 1);
+  const int floor = (0 < lxs_cfg.split_min ? lxs_cfg.split_min : (LXS_SMM_SPLIT_FILL <= stack_size ? 64 : 1));
   return (0 < lxs_cfg.split ? (floor < t ? t : floor) : 0);
 }
 
@@ -637,7 +640,7 @@ static void lxs_smm_fill(lxs_smm_args_t* out, const libsmm_acc_smm_config_t* con
      owning separate runs of one C block would otherwise lose updates */
   args.atomic = (1 == config->flush && 0 != grouped) ? 0 : 1;
   args.pfmt = param_format;
-  args.tsplit = lxs_smm_tsplit(args.bs, split);
+  args.tsplit = lxs_smm_tsplit(args.bs, split, stack_size);
   *out = args;
 }
 

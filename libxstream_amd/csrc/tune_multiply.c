@@ -147,25 +147,37 @@ static int shape_done(const char* jsonl, int dtype, int m, int n, int k, int sta
   return done;
 }
 
-/* average launch time of a configuration in ms (profile records), or a negative value if it does not run */
+/* ms per launch of cfg: reps launches back to back between two events on the stream, after one untimed
+   launch (acc_bench.c:330-344 times its repetitions the same way); -1 if the configuration does not run */
 static double tune_time(const libsmm_acc_smm_config_t* cfg, const tune_work_t* w, const void* ds, const void* da,
   const void* db, void* dc, int m, int n, int k, int reps, void* stream) {
-  long cnt = 0;
-  double ms = 0;
-  int r;
-  if (EXIT_SUCCESS != libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, (libsmm_acc_data_t)(8 == w->ts ? 3 : 1), da, db,
-                        dc, m, n, k, stream)) {
+  const libsmm_acc_data_t dtype = (libsmm_acc_data_t)(8 == w->ts ? 3 : 1);
+  void *e0 = NULL, *e1 = NULL;
+  double ms = -1;
+  int r, ok;
+  if (EXIT_SUCCESS != libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, dtype, da, db, dc, m, n, k,
+                        stream)) {
     (void)libxstream_error_consume(NULL, 0);
     return -1;
   }
-  libxstream_profile_reset();
-  for (r = 0; r < reps; ++r) {
-    libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, (libsmm_acc_data_t)(8 == w->ts ? 3 : 1), da, db, dc, m, n, k,
-      stream);
+  ok = (EXIT_SUCCESS == c_dbcsr_acc_event_create(&e0) && EXIT_SUCCESS == c_dbcsr_acc_event_create(&e1) &&
+        EXIT_SUCCESS == c_dbcsr_acc_event_record(e0, stream));
+  for (r = 0; r < reps && 0 != ok; ++r) {
+    ok = (EXIT_SUCCESS == libsmm_acc_process_config(cfg, (const int*)ds, w->stack_size, dtype, da, db, dc, m, n, k,
+                            stream));
   }
-  c_dbcsr_acc_stream_sync(stream);
-  libxstream_profile_get("smm", &cnt, &ms, NULL, NULL);
-  return (0 < cnt && 0 < ms) ? ms / (double)cnt : -1;
+  if (0 != ok && EXIT_SUCCESS == c_dbcsr_acc_event_record(e1, stream) &&
+      EXIT_SUCCESS == c_dbcsr_acc_event_synchronize(e1) &&
+      EXIT_SUCCESS == libxstream_event_elapsed((libxstream_event_t*)e0, (libxstream_event_t*)e1, &ms) && 0 < ms) {
+    ms /= (double)reps;
+  }
+  else {
+    ms = -1;
+    (void)libxstream_error_consume(NULL, 0);
+  }
+  if (NULL != e0) c_dbcsr_acc_event_destroy(e0);
+  if (NULL != e1) c_dbcsr_acc_event_destroy(e1);
+  return ms;
 }
 
 /* max normalised error of one launch of cfg from zeroed C, over the first C blocks */
@@ -185,6 +197,14 @@ static double tune_error(const libsmm_acc_smm_config_t* cfg, const tune_work_t* 
   return err;
 }
 
+#define TUNE_RERANK 6 /* fastest configurations of the sweep timed again, interleaved */
+#define TUNE_ROUNDS 3 /* timing rounds of the re-ranking (median) */
+
+static double median3(const double* t) {
+  const double a = t[0], b = t[1], c = t[2];
+  return (a < b) ? ((b < c) ? b : (a < c ? c : a)) : ((a < c) ? a : (b < c ? c : b));
+}
+
 static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   const int dtype = o->dtype;
   const char* const env_grouped = getenv("LIBXSTREAM_SMM_GROUPED");
@@ -194,8 +214,12 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
   int cand[5 * 256], ncand, i, j, b, f, result = EXIT_SUCCESS, nmeasured = 0, ntodo = 0;
   static const int bss[] = {16, 32, 64, 128};
   static libsmm_acc_smm_config_t todo[2048];
-  libsmm_acc_smm_config_t best, order[2048];
-  double times[2048], flops;
+  libsmm_acc_smm_config_t best, order[2048], top[TUNE_RERANK];
+  double times[2048], flops, ts_s[TUNE_RERANK][TUNE_ROUNDS], ts_l[TUNE_RERANK][TUNE_ROUNDS];
+  double med_s[TUNE_RERANK], med_l[TUNE_RERANK], score[TUNE_RERANK];
+  tune_work_t wl;
+  void *la = NULL, *lb = NULL, *lc = NULL, *ls = NULL;
+  int have_long = 0, ntop = 0, r;
   FILE* sweep = NULL;
   if (shape_done(o->jsonl, dtype, m, n, k, o->stack_size)) {
     fprintf(stderr, "tune_multiply: %ix%ix%i already tuned (skipped)\n", m, n, k);
@@ -252,27 +276,10 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
     }
   }
   if (NULL != o->jsonl) sweep = fopen(o->jsonl, "a");
-  libxstream_profile_enable(1);
   for (i = 0; i < ntodo && EXIT_SUCCESS == result; ++i) {
     libsmm_acc_smm_config_t cfg = todo[i];
-    long cnt = 0;
-    double ms = 0;
-    int r;
-    cfg.gflops = 0;
-    if (EXIT_SUCCESS != libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da,
-                          db, dc, m, n, k, stream)) {
-      (void)libxstream_error_consume(NULL, 0);
-      continue;
-    }
-    libxstream_profile_reset();
-    for (r = 0; r < o->reps; ++r) {
-      libsmm_acc_process_config(&cfg, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,
-        stream);
-    }
-    c_dbcsr_acc_stream_sync(stream);
-    libxstream_profile_get("smm", &cnt, &ms, NULL, NULL);
-    if (0 >= cnt || 0 >= ms) continue;
-    ms /= (double)cnt;
+    const double ms = tune_time(&cfg, &w, ds, da, db, dc, m, n, k, o->reps, stream);
+    if (0 >= ms) continue;
     cfg.gflops = flops / (ms * 1e6);
     if (nmeasured < 2048) {
       order[nmeasured] = cfg;
@@ -285,8 +292,7 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
         dtype, m, n, k, w.stack_size, cfg.variant, cfg.tm, cfg.tn, cfg.nt, cfg.bs, cfg.kt, cfg.flush, ms, cfg.gflops);
     }
   }
-  libxstream_profile_enable(0);
-  /* fastest first; the first one that passes the host check wins */
+  /* fastest first */
   for (i = 0; i < nmeasured; ++i) {
     for (j = i + 1; j < nmeasured; ++j) {
       if (times[j] < times[i]) {
@@ -299,26 +305,70 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
       }
     }
   }
-  memset(&best, 0, sizeof(best));
-  for (i = j = 0; i < nmeasured && j < 8 && EXIT_SUCCESS == result; ++i) {
-    const int nblocks = (w.nc < 40 ? w.nc : 40);
-    double err;
-    void* host;
-    /* flush 1 (no atomics) launches as the atomic flush unless LIBXSTREAM_SMM_GROUPED is set: it is
-       measured for the sweep file but cannot win */
+  /* the long stack (16 x S entries, at most 500000): a row serves every longer stack as tuned */
+  memset(&wl, 0, sizeof(wl));
+  {
+    const int s2 = (16 * w.stack_size < 500000 ? 16 * w.stack_size : 500000);
+    have_long = (s2 > w.stack_size &&
+                 EXIT_SUCCESS == work_init(&wl, dtype, m, n, k, s2, 0x5eed2507ull ^ (uint64_t)m) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&la, (size_t)wl.ts * wl.na * m * k) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&lb, (size_t)wl.ts * wl.nb * k * n) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&lc, (size_t)wl.ts * wl.nc * m * n) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&ls, sizeof(int) * 3 * (size_t)wl.stack_size) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_memcpy_h2d(wl.a, la, (size_t)wl.ts * wl.na * m * k, stream) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_memcpy_h2d(wl.b, lb, (size_t)wl.ts * wl.nb * k * n, stream) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_memcpy_h2d(wl.stack, ls, sizeof(int) * 3 * (size_t)wl.stack_size, stream) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_memset_zero(lc, 0, (size_t)wl.ts * wl.nc * m * n, stream) &&
+                 EXIT_SUCCESS == c_dbcsr_acc_stream_sync(stream));
+  }
+  /* re-rank the TUNE_RERANK fastest eligible configurations: TUNE_ROUNDS rounds that time every candidate
+     once at S and once on the long stack, interleaved (drift in clock or temperature hits them alike);
+     score = geometric mean of the two median times, each relative to the best candidate at its size.
+     Flush 1 (no atomics) launches as the atomic flush unless LIBXSTREAM_SMM_GROUPED is set: it is measured
+     for the sweep file but cannot win. */
+  for (i = 0; i < nmeasured && ntop < TUNE_RERANK; ++i) {
     if (1 == order[i].flush && 0 == grouped) continue;
-    ++j;
-    host = malloc((size_t)w.ts * nblocks * m * n);
-    if (NULL == host) break;
-    c_dbcsr_acc_memset_zero(dc, 0, (size_t)w.ts * w.nc * m * n, stream);
-    libsmm_acc_process_config(order + i, (const int*)ds, w.stack_size, (libsmm_acc_data_t)dtype, da, db, dc, m, n, k,
-      stream);
-    c_dbcsr_acc_memcpy_d2h(dc, host, (size_t)w.ts * nblocks * m * n, stream);
-    c_dbcsr_acc_stream_sync(stream);
-    err = host_check(&w, host, nblocks);
-    free(host);
+    top[ntop++] = order[i];
+  }
+  for (r = 0; r < TUNE_ROUNDS && EXIT_SUCCESS == result; ++r) {
+    for (i = 0; i < ntop; ++i) {
+      ts_s[i][r] = tune_time(top + i, &w, ds, da, db, dc, m, n, k, o->reps, stream);
+      ts_l[i][r] = (0 != have_long ? tune_time(top + i, &wl, ls, la, lb, lc, m, n, k, (o->reps + 3) / 4, stream) : 1);
+    }
+  }
+  {
+    double min_s = INFINITY, min_l = INFINITY;
+    for (i = 0; i < ntop; ++i) {
+      med_s[i] = median3(ts_s[i]);
+      med_l[i] = median3(ts_l[i]);
+      if (0 < med_s[i] && med_s[i] < min_s) min_s = med_s[i];
+      if (0 < med_l[i] && med_l[i] < min_l) min_l = med_l[i];
+    }
+    for (i = 0; i < ntop; ++i) {
+      score[i] = (0 < med_s[i] && 0 < med_l[i]) ? sqrt(med_s[i] / min_s * (med_l[i] / min_l)) : INFINITY;
+      top[i].gflops = (0 < med_s[i] ? flops / (med_s[i] * 1e6) : 0);
+    }
+    for (i = 0; i < ntop; ++i) { /* by score */
+      for (j = i + 1; j < ntop; ++j) {
+        if (score[j] < score[i]) {
+          const double t = score[i], u = med_s[i];
+          const libsmm_acc_smm_config_t c = top[i];
+          score[i] = score[j];
+          med_s[i] = med_s[j];
+          top[i] = top[j];
+          score[j] = t;
+          med_s[j] = u;
+          top[j] = c;
+        }
+      }
+    }
+  }
+  /* the first (by score) that passes the host check wins */
+  memset(&best, 0, sizeof(best));
+  for (i = 0; i < ntop && EXIT_SUCCESS == result; ++i) {
+    const double err = tune_error(top + i, &w, ds, da, db, dc, m, n, k, stream);
     if (err <= (8 == w.ts ? 1e-13 : 1e-5)) {
-      best = order[i];
+      best = top[i];
       break;
     }
     fprintf(stderr, "tune_multiply: %ix%ix%i configuration %i failed the check (error %g)\n", m, n, k, i, err);
@@ -328,9 +378,7 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
        split x bs entries are cut into pieces); the sweep file keeps the plain configurations only */
     static const int rbs[] = {4, 8, 16, 32}, rsplit[] = {1, 2};
     libsmm_acc_smm_config_t cand_best = best;
-    double t_best;
-    libxstream_profile_enable(1);
-    t_best = tune_time(&best, &w, ds, da, db, dc, m, n, k, o->reps, stream);
+    double t_best = tune_time(&best, &w, ds, da, db, dc, m, n, k, o->reps, stream);
     for (i = 0; i < 4 && 0 < t_best; ++i) {
       for (j = 0; j < 2; ++j) {
         libsmm_acc_smm_config_t cfg = best;
@@ -347,39 +395,18 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
       }
     }
     if (cand_best.bs != best.bs || cand_best.split != best.split) {
-      /* a refinement must not lose on a long stack: bs is scaled down for shorter stacks only, so the
-         configuration also serves stacks of many times the tuned size (the same shape at 16x S) */
-      tune_work_t w2;
-      void *da2 = NULL, *db2 = NULL, *dc2 = NULL, *ds2 = NULL;
-      const int s2 = (16 * w.stack_size < 500000 ? 16 * w.stack_size : 500000);
-      int short_pair = 0, ok = (s2 > w.stack_size && EXIT_SUCCESS == work_init(&w2, dtype, m, n, k, s2, 0x5eed2507ull ^ (uint64_t)m));
-      if (0 != ok) {
-        ok = (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&da2, (size_t)w2.ts * w2.na * m * k)) &&
-             (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&db2, (size_t)w2.ts * w2.nb * k * n)) &&
-             (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&dc2, (size_t)w2.ts * w2.nc * m * n)) &&
-             (EXIT_SUCCESS == c_dbcsr_acc_dev_mem_allocate(&ds2, sizeof(int) * 3 * (size_t)w2.stack_size));
-        if (0 != ok) {
-          c_dbcsr_acc_memcpy_h2d(w2.a, da2, (size_t)w2.ts * w2.na * m * k, stream);
-          c_dbcsr_acc_memcpy_h2d(w2.b, db2, (size_t)w2.ts * w2.nb * k * n, stream);
-          c_dbcsr_acc_memcpy_h2d(w2.stack, ds2, sizeof(int) * 3 * (size_t)w2.stack_size, stream);
-          c_dbcsr_acc_memset_zero(dc2, 0, (size_t)w2.ts * w2.nc * m * n, stream);
-          c_dbcsr_acc_stream_sync(stream);
-          {
-            const double t_plain = tune_time(&best, &w2, ds2, da2, db2, dc2, m, n, k, o->reps, stream);
-            const double t_cand = tune_time(&cand_best, &w2, ds2, da2, db2, dc2, m, n, k, o->reps, stream);
-            ok = (0 < t_plain && 0 < t_cand && t_cand <= 1.01 * t_plain);
-            if (0 == ok) { /* kept for stacks of at most S entries only */
-              printf("%ix%ix%i: bs=%i split=%i is faster at S=%i but not at S=%i (%.4f vs %.4f ms): short stacks only\n",
-                m, n, k, cand_best.bs, cand_best.split, w.stack_size, s2, t_cand, t_plain);
-              short_pair = 1;
-            }
-          }
+      /* a refinement must not lose on the long stack: bs is scaled down for shorter stacks only, so the
+         configuration also serves stacks of many times the tuned size */
+      int ok = 0, short_pair = 0;
+      if (0 != have_long) {
+        const double t_plain = tune_time(&best, &wl, ls, la, lb, lc, m, n, k, o->reps, stream);
+        const double t_cand = tune_time(&cand_best, &wl, ls, la, lb, lc, m, n, k, o->reps, stream);
+        ok = (0 < t_plain && 0 < t_cand && t_cand <= 1.01 * t_plain);
+        if (0 == ok) { /* kept for stacks of at most S entries only */
+          printf("%ix%ix%i: bs=%i split=%i is faster at S=%i but not at S=%i (%.4f vs %.4f ms): short stacks only\n",
+            m, n, k, cand_best.bs, cand_best.split, w.stack_size, wl.stack_size, t_cand, t_plain);
+          short_pair = 1;
         }
-        c_dbcsr_acc_dev_mem_deallocate(da2);
-        c_dbcsr_acc_dev_mem_deallocate(db2);
-        c_dbcsr_acc_dev_mem_deallocate(dc2);
-        c_dbcsr_acc_dev_mem_deallocate(ds2);
-        work_free(&w2);
       }
       if (0 != ok || 0 != short_pair) {
         const double err = tune_error(&cand_best, &w, ds, da, db, dc, m, n, k, stream);
@@ -393,8 +420,14 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
         }
       }
     }
-    libxstream_profile_enable(0);
   }
+  if (0 != have_long || NULL != la || NULL != lb || NULL != lc || NULL != ls) {
+    c_dbcsr_acc_dev_mem_deallocate(la);
+    c_dbcsr_acc_dev_mem_deallocate(lb);
+    c_dbcsr_acc_dev_mem_deallocate(lc);
+    c_dbcsr_acc_dev_mem_deallocate(ls);
+  }
+  work_free(&wl);
   if (0 < best.nt) {
     best.s = w.stack_size; /* bs was chosen for this stack size */
     libsmm_acc_config_set((libsmm_acc_data_t)dtype, m, n, k, &best);
@@ -446,7 +479,7 @@ int main(int argc, char* argv[]) {
   const double start = seconds_now();
   o.dtype = dbcsr_type_real_8;
   o.stack_size = 30000;
-  o.reps = 5;
+  o.reps = 20;
   o.csv = NULL;
   o.jsonl = NULL;
   o.device = "MI355X";

@@ -1,4 +1,6 @@
 """Shared helpers for the GPU tests: run a stack workload through the C ABI, compare with the oracle."""
+import os
+
 import numpy as np
 
 import libxstream_amd as lx
@@ -88,4 +90,22 @@ def cut_limit(tid, m, n, k, stack_size):
         bs, split = cfg.bs_short, cfg.split_short
     if bs > 1 and cfg.s > 1 and stack_size < cfg.s:
         bs = min(bs, max(1, -(-stack_size * max(bs, 32) // (cfg.s - 1))))
-    return (split if split > 0 else 2) * bs
+    # runs this short are never cut: 64 entries in stacks of at least 16384 entries unless the environment
+    # sets the floor for every stack
+    floor = int(os.environ.get("LIBXSTREAM_SMM_SPLIT_MIN") or 0) or (64 if stack_size >= 16384 else 1)
+    return max((split if split > 0 else 2) * bs, floor)
+
+
+def sampled_oracle_error(w, out, nblocks=16, seed=0, stack=None):
+    """Oracle check of a large stack on a seeded sample of C blocks: the entries that update those blocks
+    are run through the double-double oracle, and the sampled blocks of out are compared (max normalised
+    error over them)."""
+    stack = w.stack if stack is None else stack
+    cblk = (np.asarray(stack)[:, 2].astype(np.int64) - 1) // (w.m * w.n)
+    present = np.unique(cblk)
+    pick = np.random.default_rng(seed).choice(present, size=min(nblocks, present.size), replace=False)
+    sub = np.ascontiguousarray(np.asarray(stack)[np.isin(cblk, pick)])
+    hi, lo = oracle.stack_dd(w.m, w.n, w.k, sub, w.a, w.b, c_in=w.c.ravel())
+    mn = w.m * w.n
+    idx = (pick[:, None] * mn + np.arange(mn)[None, :]).ravel()
+    return oracle.error(np.ascontiguousarray(out.ravel()[idx]), hi.ravel()[idx], lo.ravel()[idx])

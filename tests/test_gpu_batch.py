@@ -222,3 +222,39 @@ def test_batch_tune_schedule(stream):
             batch.close()
     finally:
         res.free()
+
+
+def test_batch_full_workload(stream):
+    """The full mixed-shape workload (24 stacks, seed 25071975, lengths 1k-200k: about 915k entries), in
+    the schedule libsmm_acc_batch_tune picks and in one persistent launch: eager, then two graph replays.
+    Each stack's checksum after every application, and the double-double oracle on 8 sampled C blocks per
+    stack after the first."""
+    from tests.helpers import checksum, sampled_oracle_error
+
+    works = make_mixed_batch(24, seed=25071975, dtype=np.float64)
+    assert sum(w.stack_size for w in works) > 500000
+    totals = [checksum(w) for w in works]
+    res = Resident(works, stream)
+    try:
+        best, _ = lx.Batch.tune(lx.REAL_8, res.items)
+        for params in (best, lx.BatchParams(0, 0, 0)):
+            for i, w in enumerate(works):
+                res.items[i][7].upload(w.c)
+            batch = lx.Batch(lx.REAL_8, res.items, params)
+            for step in range(3):
+                if step == 0:
+                    batch.process(stream)
+                else:
+                    batch.replay(stream)
+                stream.sync()
+                for i, w in enumerate(works):
+                    out = res.download(i)
+                    total, scale = totals[i]
+                    got = float(np.sum(out))
+                    assert abs(got - (step + 1) * total) <= 1e-12 * (step + 1) * scale, (params.as_dict(), step, i)
+                    if step == 0:
+                        err = sampled_oracle_error(w, out, nblocks=8, seed=i)
+                        assert err[0] <= tol(w.dtype), (params.as_dict(), i, err)
+            batch.close()
+    finally:
+        res.free()
