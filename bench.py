@@ -118,9 +118,15 @@ class Resident:
             buf.free()
 
 
-def timed(lx, res, stream, steps, warmup):
+def timed(lx, res, stream, steps, warmup, settle_s=0.2):
     """(host ms per step, ms per launch between two HIP events on the launch stream around the same
-    timed launches)"""
+    timed launches). Before the warmup steps the stack runs untimed for settle_s seconds, so that the
+    device clocks have left their idle state when the timed steps begin."""
+    t_end = time.perf_counter() + settle_s
+    while time.perf_counter() < t_end:
+        for _ in range(10):
+            res.process(stream)
+        stream.sync()
     for _ in range(warmup):
         res.process(stream)
     stream.sync()
@@ -881,7 +887,8 @@ def main():
         "fp64_tflops": round(flops / (kernel_ms * 1e-3) * 1e-12, 3),
         "fp64_frac": round(flops / (kernel_ms * 1e-3) * 1e-12 / FP64_PEAK_TFLOPS, 4),
         "mfma_fp64_tflops": round(mfma / (kernel_ms * 1e-3) * 1e-12, 3) if mfma else None,
-        "mfma_busy": (round(pmc["SQ_VALU_MFMA_BUSY_CYCLES"][0] / (4.0 * ncus * pmc["GRBM_GUI_ACTIVE"][0]), 4)
+        # SQ_VALU_MFMA_BUSY_CYCLES sums over the SIMDs (4 per CU), GRBM_GUI_ACTIVE over the XCDs (32 CUs each)
+        "mfma_busy": (round(pmc["SQ_VALU_MFMA_BUSY_CYCLES"][0] / (4.0 * ncus * pmc["GRBM_GUI_ACTIVE"][0] / max(1, ncus // 32)), 4)
                       if "SQ_VALU_MFMA_BUSY_CYCLES" in pmc and pmc.get("GRBM_GUI_ACTIVE", (0,))[0] else None),
     }
     report["pmc"] = {c: {"per_launch": v[0], "copy_of_%d_bytes" % COPY_BYTES: v[1]} for c, v in pmc.items()}
