@@ -17,7 +17,9 @@
  * kept as the short-stack pair BSS/SPLITS (used for stacks of at most S entries) otherwise.
  *
  * usage: tune_multiply [-t f64|f32] [-s stack] [-r reps] [-o best.csv] [-j sweep.jsonl]
- *                      [-d device] [-T seconds] [-l model.txt [-k topk]] [-f triplets.txt | m n k [m n k ...]]
+ *                      [-d device] [-T seconds] [-l model.txt [-k topk]] [-x 1]
+ *                      [-f triplets.txt | m n k [m n k ...]]
+ * -x 1 only measures and records the sweep (training data for the learned predictor): no selection.
  * -T stops before the next shape once the time budget is spent (a resumed run continues there).
  * -l makes it learned-guided: only the model's topk (default 4) configurations are measured
  * (libsmm_acc_mlp_predict_topk) instead of every valid one.
@@ -35,6 +37,7 @@ typedef struct tune_opts_t {
   const char *csv, *jsonl, *device;
   libsmm_acc_mlp_t* model; /* learned-guided: measure only the model's topk configurations */
   int topk;
+  int sweep_only; /* -x: measure and record every configuration, choose nothing (training sweeps) */
 } tune_opts_t;
 
 static uint64_t splitmix64(uint64_t* state) {
@@ -292,6 +295,23 @@ static int tune_shape(const tune_opts_t* o, int m, int n, int k) {
         dtype, m, n, k, w.stack_size, cfg.variant, cfg.tm, cfg.tn, cfg.nt, cfg.bs, cfg.kt, cfg.flush, ms, cfg.gflops);
     }
   }
+  if (0 != o->sweep_only) { /* the sweep file is the product: mark the shape done */
+    if (NULL != sweep) {
+      double gbest = 0;
+      for (i = 0; i < nmeasured; ++i) gbest = (order[i].gflops > gbest ? order[i].gflops : gbest);
+      fprintf(sweep, "{\"done\": 1, \"dtype\": %i, \"m\": %i, \"n\": %i, \"k\": %i, \"S\": %i, \"gflops\": %.2f}\n", dtype,
+        m, n, k, w.stack_size, gbest);
+      fclose(sweep);
+    }
+    printf("%ix%ix%i %s: %i configurations measured\n", m, n, k, 8 == w.ts ? "f64" : "f32", nmeasured);
+    c_dbcsr_acc_dev_mem_deallocate(da);
+    c_dbcsr_acc_dev_mem_deallocate(db);
+    c_dbcsr_acc_dev_mem_deallocate(dc);
+    c_dbcsr_acc_dev_mem_deallocate(ds);
+    c_dbcsr_acc_stream_destroy(stream);
+    work_free(&w);
+    return result;
+  }
   /* fastest first */
   for (i = 0; i < nmeasured; ++i) {
     for (j = i + 1; j < nmeasured; ++j) {
@@ -485,6 +505,7 @@ int main(int argc, char* argv[]) {
   o.device = "MI355X";
   o.model = NULL;
   o.topk = 0;
+  o.sweep_only = 0;
   for (i = 1; i < argc && '-' == argv[i][0] && '\0' != argv[i][1] && i + 1 < argc; i += 2) {
     const char* const v = argv[i + 1];
     switch (argv[i][1]) {
@@ -498,6 +519,7 @@ int main(int argc, char* argv[]) {
       case 'T': budget = atof(v); break;
       case 'l': model_path = v; break;
       case 'k': o.topk = atoi(v); break;
+      case 'x': o.sweep_only = atoi(v); break;
       default: fprintf(stderr, "tune_multiply: unknown option %s\n", argv[i]); return EXIT_FAILURE;
     }
   }

@@ -187,6 +187,74 @@ def report(results, tids):
     return rep
 
 
+def _time_configs(configs, w, rounds=3, reps=10, stream=None):
+    """median ms per launch of every configuration on workload w: `rounds` rounds that each time every
+    configuration once (reps launches back to back between two HIP events), interleaved so that drift in
+    clock or temperature hits all configurations alike"""
+    tid = lx.typeid(w.dtype)
+    a, b, c = (lx.DeviceBuffer.from_array(v) for v in (w.a, w.b_device_layout(), w.c))
+    st = lx.DeviceBuffer.from_array(w.stack)
+    e0, e1 = lx.Event(), lx.Event()
+    times = [[] for _ in configs]
+    try:
+        for _ in range(rounds):
+            for i, cfg in enumerate(configs):
+                lx.process_config(cfg, st, w.stack_size, tid, a, b, c, w.m, w.n, w.k, stream)
+                e0.record(stream)
+                for _ in range(reps):
+                    lx.process_config(cfg, st, w.stack_size, tid, a, b, c, w.m, w.n, w.k, stream)
+                e1.record(stream)
+                stream.sync()
+                times[i].append(e0.elapsed_ms(e1) / reps)
+    finally:
+        e0.close()
+        e1.close()
+        for x in (a, b, c, st):
+            x.free()
+    return [float(np.median(t)) for t in times]
+
+
+def evaluate_live(model, sweep, keys, stack_size=30000, nbest=3, topk=4, stream=None, log=None):
+    """Held-out shapes timed now: the network's choice, its top-K and the sweep's nbest fastest runnable
+    configurations are timed together (interleaved rounds, medians) on a fresh S-entry stack of the shape.
+    The exhaustive best is the fastest of the sweep's nbest re-timed (one noisy sweep sample no longer
+    decides it); each method's fraction is t_best / t_method."""
+    from libxstream_amd.workload import make_stack_workload
+
+    out = []
+    for key in keys:
+        tid, m, n, k = key
+        rows = sorted(runnable(sweep[key]), key=lambda r: -r["gflops"])[:nbest]
+        cands, index = [], {}
+
+        def add(cfg):
+            ck = config_key(cfg)
+            if ck not in index:
+                index[ck] = len(cands)
+                cands.append(cfg)
+            return index[ck]
+
+        best_ix = [add(to_config(r)) for r in rows]
+        ranked = model.predict_topk(*key, topk)
+        top_ix = [add(c) for c, _ in ranked]
+        w = make_stack_workload(m, n, k, stack_size, dtype=lx.nptype(tid), seed=m * 7919 + n * 104729 + k)
+        t = _time_configs(cands, w, stream=stream)
+        t_best = min(t[i] for i in best_ix)
+        entry = {"key": list(key), "best_ms": t_best, "learned_ms": t[top_ix[0]],
+                 "learned": t_best / t[top_ix[0]], "learned_top%d" % topk: t_best / min(t[i] for i in top_ix),
+                 "confidence": ranked[0][1], "margin": ranked[0][1] - (ranked[1][1] if len(ranked) > 1 else 0.0)}
+        out.append(entry)
+        if log is not None:
+            log("%s learned %.3f top%d %.3f" % (key, entry["learned"], topk, entry["learned_top%d" % topk]))
+    return out
+
+
+def summary_live(results, field):
+    r = np.array([x[field] for x in results])
+    return {"mean": round(float(r.mean()), 4), "median": round(float(np.median(r)), 4), "min": round(float(r.min()), 4),
+            "within_10pct": round(float((r >= 0.9).mean()), 4), "geomean": round(float(np.exp(np.log(r).mean())), 4)}
+
+
 def write_triplets(path, count=500, seed=25071975):
     """the sweep's triplet file (one 'm n k' per line, tune_multiply -f): 400 training, then 100 held out"""
     with open(path, "w") as f:
@@ -210,6 +278,10 @@ def main(argv):
     e = sub.add_parser("evaluate", help="evaluate a model on the held-out shapes of the sweeps")
     e.add_argument("sweeps", nargs="+")
     e.add_argument("--model", required=True)
+    v = sub.add_parser("live", help="time the held-out choices now against the sweeps' fastest (GPU)")
+    v.add_argument("sweeps", nargs="+")
+    v.add_argument("--model", required=True)
+    v.add_argument("--report", help="output JSON report")
     args = p.parse_args(argv)
     if args.cmd == "triplets":
         write_triplets(args.path)
@@ -220,6 +292,24 @@ def main(argv):
     tids = sorted({key[0] for key in sweep})
     train_keys = [(tid,) + t for tid in tids for t in train_t if (tid,) + t in sweep]
     held_keys = [(tid,) + t for tid in tids for t in held_t if (tid,) + t in sweep]
+    if args.cmd == "live":
+        model = lx.Mlp(path=args.model)
+        stream = lx.Stream("live")
+        results = evaluate_live(model, sweep, held_keys, stream=stream, log=lambda x: print(x, flush=True))
+        rep = {"heldout_shapes": len(results)}
+        for f in ("learned", "learned_top4"):
+            rep[f] = summary_live(results, f)
+            for tid in tids:
+                rep.setdefault("type%d" % tid, {})[f] = summary_live([x for x in results if x["key"][0] == tid], f)
+        print(json.dumps(rep, indent=1))
+        if args.report:
+            rep["results"] = results
+            with open(args.report, "w") as f:
+                json.dump(rep, f)
+        stream.close()
+        model.close()
+        lx.finalize()
+        return 0
     if args.cmd == "train":
         model, loss = train(sweep, train_keys, nnets=args.nets, epochs=args.epochs)
         model.save(args.model)
