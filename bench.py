@@ -310,6 +310,23 @@ def batched(lx, stream, steps, warmup):
     stream.sync()
     t1 = time.perf_counter()
     prof = lx.profile_get("batch")
+    lx.profile_enable(False)
+    # every schedule kind replayed the same way: the persistent kernel (default and tuned chunking), the
+    # per-stack launches in order, and the per-stack launches on concurrent lanes
+    schedules = {}
+    for name, params in (("persistent_default", lx.BatchParams(0, 0, 0)), ("per_stack_in_order", lx.BatchParams(0, 0, 1)),
+                         ("per_stack_lanes", lx.BatchParams(0, 0, 2))):
+        bb = lx.Batch(lx.REAL_8, items, params)
+        for _ in range(warmup):
+            bb.replay(stream)
+        stream.sync()
+        ta = time.perf_counter()
+        for _ in range(steps):
+            bb.replay(stream)
+        stream.sync()
+        schedules[name] = round((time.perf_counter() - ta) * 1e3 / steps, 4)
+        bb.close()
+    lx.profile_enable(True)
     for _ in range(warmup):
         for r in res:
             r.process(stream)
@@ -346,6 +363,7 @@ def batched(lx, stream, steps, warmup):
         "graph_kernel_ms": round(prof["ms"] / max(prof["count"], 1), 4),
         "schedule": dict(best.as_dict(), source="libsmm_acc_batch_tune"),
         "default_schedule_graph_kernel_ms": round(prof0["ms"] / max(prof0["count"], 1), 4),
+        "graph_ms_by_schedule": schedules,
         "alg_gbs": round(info["bytes"] / (prof["ms"] / max(prof["count"], 1) * 1e-3) * 1e-9, 1),
         "separate_launches_ms_per_step": round(single_ms, 4),
         "separate_launches_gflops": round(info["flops"] / (single_ms * 1e-3) * 1e-9, 1),
@@ -837,9 +855,11 @@ def main():
     if extras and not args.no_batch:
         report["batched_mixed"] = b = batched(lx, stream, 20, 3)
         secondary["batched_mixed"] = {"stacks": 24, "entries": b["entries"], "graph_ms": b["graph_ms_per_step"],
-                                      "graph_gflops": b["graph_gflops"], "schedule": b["schedule"].get("per_stack")
-                                      and "per-stack launches" or "persistent kernel",
-                                      "separate_launches_ms": b["separate_launches_ms_per_step"]}
+                                      "graph_gflops": b["graph_gflops"], "schedule": {0: "persistent kernel", 1: "per-stack launches",
+                                                   2: "per-stack launches on 4 concurrent lanes"}[
+                                          b["schedule"].get("per_stack") or 0],
+                                      "separate_launches_ms": b["separate_launches_ms_per_step"],
+                                      "graph_ms_by_schedule": b["graph_ms_by_schedule"]}
         print("bench: batched %.4f ms" % b["graph_ms_per_step"], file=sys.stderr, flush=True)
     if extras and not args.no_ozaki:
         report["ozaki"] = oz = ozaki_section(lx, stream, [int(x) for x in args.ozaki_sizes.split(",") if x])

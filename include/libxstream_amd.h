@@ -116,7 +116,9 @@ typedef struct libsmm_acc_batch_t libsmm_acc_batch_t;
 typedef struct libsmm_acc_batch_params_t {
   int chunk_elements, blocks_per_cu;
   int per_stack; /* 1: one libsmm_acc_process launch per stack (its own configuration) instead of the
-                    persistent kernel; libsmm_acc_batch_replay captures all of them in one graph */
+                    persistent kernel; libsmm_acc_batch_replay captures all of them in one graph;
+                    2: the same launches spread over 4 side streams that fork from the caller's stream and
+                    join it again (independent branches of one graph), so that short stacks overlap */
 } libsmm_acc_batch_params_t;
 /** plan the batch and upload the plan (allocates device memory; not for use under capture) */
 LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,

@@ -13,6 +13,7 @@
 #include <string.h>
 
 #define LXS_BATCH_CHUNK_ELEMENTS (32 * 2 * 23 * 23)
+#define LXS_BATCH_LANES 4 /* side streams of the concurrent per-stack mode */
 
 struct libsmm_acc_batch_t {
   int dtype, nstacks, ntasks, nblocks;
@@ -24,6 +25,11 @@ struct libsmm_acc_batch_t {
   double flops, bytes;
   int per_stack;                 /* launch mode, libsmm_acc_batch_params_t */
   libsmm_acc_batch_stack_t* host; /* the stacks as given (per-stack mode) */
+  /* per_stack 2: the launches spread over LXS_BATCH_LANES side streams that fork from the caller's stream
+     and join it again (one graph with independent branches under replay); lane[i]: stack i's stream */
+  hipStream_t side[LXS_BATCH_LANES];
+  hipEvent_t fork, join[LXS_BATCH_LANES];
+  int* lane;
 };
 
 typedef struct lxs_chunk_t {
@@ -39,6 +45,22 @@ static int lxs_chunk_cmp(const void* x, const void* y) {
 LIBXSTREAM_API int libsmm_acc_batch_create(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
   int nstacks, libsmm_acc_batch_t** batch) {
   return libsmm_acc_batch_create_ex(datatype, stacks, nstacks, NULL, batch);
+}
+
+/* everything a batch holds besides its graph */
+static void lxs_batch_release(libsmm_acc_batch_t* b) {
+  int i;
+  if (NULL != b->capture) (void)hipStreamDestroy(b->capture);
+  for (i = 0; i < LXS_BATCH_LANES; ++i) {
+    if (NULL != b->side[i]) (void)hipStreamDestroy(b->side[i]);
+    if (NULL != b->join[i]) (void)hipEventDestroy(b->join[i]);
+  }
+  if (NULL != b->fork) (void)hipEventDestroy(b->fork);
+  if (NULL != b->desc) (void)hipFree(b->desc);
+  if (NULL != b->tasks) (void)hipFree(b->tasks);
+  free(b->lane);
+  free(b->host);
+  free(b);
 }
 
 LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const libsmm_acc_batch_stack_t* stacks,
@@ -138,7 +160,7 @@ LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const 
     b->dtype = (int)datatype;
     b->nstacks = nstacks;
     b->ntasks = ntasks;
-    b->per_stack = (NULL != params && 0 != params->per_stack);
+    b->per_stack = (NULL != params && 0 < params->per_stack ? (2 <= params->per_stack ? 2 : 1) : 0);
     b->host = (libsmm_acc_batch_stack_t*)malloc(sizeof(*stacks) * (size_t)nstacks);
     if (NULL == b->host) result = lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: out of memory");
     else memcpy(b->host, stacks, sizeof(*stacks) * (size_t)nstacks);
@@ -148,6 +170,39 @@ LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const 
       libsmm_acc_smm_config_t cfg;
       result = libsmm_acc_config_get(datatype, stacks[i].m, stacks[i].n, stacks[i].k, &cfg, NULL);
     }
+  }
+  if (EXIT_SUCCESS == result && 2 == b->per_stack) { /* lanes: stacks by flops, each to the least loaded */
+    double load[LXS_BATCH_LANES] = {0};
+    int* const order = (int*)malloc(sizeof(int) * (size_t)nstacks);
+    b->lane = (int*)malloc(sizeof(int) * (size_t)nstacks);
+    if (NULL == order || NULL == b->lane) result = lxs_error(EXIT_FAILURE, "libsmm_acc_batch_create: out of memory");
+    for (i = 0; i < nstacks && EXIT_SUCCESS == result; ++i) order[i] = i;
+    for (i = 0; i < nstacks && EXIT_SUCCESS == result; ++i) { /* descending flops (insertion sort) */
+      const int x = order[i];
+      const double fx = (double)stacks[x].m * stacks[x].n * stacks[x].k * stacks[x].stack_size;
+      int j = i;
+      while (0 < j && (double)stacks[order[j - 1]].m * stacks[order[j - 1]].n * stacks[order[j - 1]].k *
+                          stacks[order[j - 1]].stack_size < fx) {
+        order[j] = order[j - 1];
+        --j;
+      }
+      order[j] = x;
+    }
+    for (i = 0; i < nstacks && EXIT_SUCCESS == result; ++i) {
+      const int x = order[i];
+      int l, best = 0;
+      for (l = 1; l < LXS_BATCH_LANES; ++l) best = (load[l] < load[best] ? l : best);
+      b->lane[x] = best;
+      load[best] += (double)stacks[x].m * stacks[x].n * stacks[x].k * stacks[x].stack_size;
+    }
+    free(order);
+    for (i = 0; i < LXS_BATCH_LANES && EXIT_SUCCESS == result; ++i) {
+      result = lxs_hip(hipStreamCreateWithFlags(&b->side[i], hipStreamNonBlocking), "batch: side stream");
+      if (EXIT_SUCCESS == result) {
+        result = lxs_hip(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "batch: join event");
+      }
+    }
+    if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "batch: fork event");
   }
   if (EXIT_SUCCESS == result) {
     b->nblocks = (0 < lxs_cfg.ncus ? blocks_per_cu * lxs_cfg.ncus : 256 * blocks_per_cu);
@@ -171,10 +226,7 @@ LIBXSTREAM_API int libsmm_acc_batch_create_ex(libsmm_acc_data_t datatype, const 
     *batch = b;
   }
   else {
-    if (NULL != b->desc) (void)hipFree(b->desc);
-    if (NULL != b->tasks) (void)hipFree(b->tasks);
-    free(b->host);
-    free(b);
+    lxs_batch_release(b);
   }
   return result;
 }
@@ -184,11 +236,26 @@ static int lxs_batch_enqueue(libsmm_acc_batch_t* b, hipStream_t hs) {
   if (0 == b->per_stack) {
     code = lxs_batch_launch(b->dtype, b->desc, b->tasks, b->ntasks, b->nblocks, (void*)hs);
   }
-  else {
+  else if (1 == b->per_stack) {
     for (i = 0; i < b->nstacks && 0 == code; ++i) {
       const libsmm_acc_batch_stack_t* const s = b->host + i;
       code = lxs_smm_enqueue(b->dtype, s->dev_param_stack, s->stack_size, s->dev_a_data, s->dev_b_data, s->dev_c_data,
         s->m, s->n, s->k, (void*)hs);
+    }
+  }
+  else { /* fork to the side streams, one launch per stack on its lane, join (the C flush is atomic, so
+            stacks that share a C pool may overlap) */
+    int l;
+    code = (int)hipEventRecord(b->fork, hs);
+    for (l = 0; l < LXS_BATCH_LANES && 0 == code; ++l) code = (int)hipStreamWaitEvent(b->side[l], b->fork, 0);
+    for (i = 0; i < b->nstacks && 0 == code; ++i) {
+      const libsmm_acc_batch_stack_t* const s = b->host + i;
+      code = lxs_smm_enqueue(b->dtype, s->dev_param_stack, s->stack_size, s->dev_a_data, s->dev_b_data, s->dev_c_data,
+        s->m, s->n, s->k, (void*)b->side[b->lane[i]]);
+    }
+    for (l = 0; l < LXS_BATCH_LANES && 0 == code; ++l) {
+      code = (int)hipEventRecord(b->join[l], b->side[l]);
+      if (0 == code) code = (int)hipStreamWaitEvent(hs, b->join[l], 0);
     }
   }
   return (0 <= code ? lxs_hip((hipError_t)code, "batch launch") : lxs_error(code, "batch launch"));
@@ -249,11 +316,8 @@ LIBXSTREAM_API int libsmm_acc_batch_destroy(libsmm_acc_batch_t* batch) {
   if (NULL != batch) {
     if (NULL != batch->exec) result = lxs_hip(hipGraphExecDestroy(batch->exec), "hipGraphExecDestroy");
     if (NULL != batch->graph) (void)hipGraphDestroy(batch->graph);
-    if (NULL != batch->capture) (void)hipStreamDestroy(batch->capture);
-    if (NULL != batch->desc) (void)hipFree(batch->desc);
-    if (NULL != batch->tasks) (void)hipFree(batch->tasks);
-    free(batch->host);
-    free(batch);
+    if (2 == batch->per_stack) (void)hipDeviceSynchronize(); /* side streams may still run the last process */
+    lxs_batch_release(batch);
   }
   return result;
 }
@@ -309,7 +373,7 @@ LIBXSTREAM_API int libsmm_acc_batch_tune(libsmm_acc_data_t datatype, const libsm
   if (EXIT_SUCCESS == result) result = lxs_hip(hipStreamCreateWithFlags(&hs, hipStreamNonBlocking), "hipStreamCreate");
   if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreate(&e0), "hipEventCreate");
   if (EXIT_SUCCESS == result) result = lxs_hip(hipEventCreate(&e1), "hipEventCreate");
-  for (i = 0; i <= (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs)) && EXIT_SUCCESS == result; ++i) {
+  for (i = 0; i <= (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs)) + 1 && EXIT_SUCCESS == result; ++i) {
     const int last = (int)(sizeof(scales) / sizeof(*scales)) * (int)(sizeof(wgs) / sizeof(*wgs));
     libsmm_acc_batch_params_t p;
     libsmm_acc_batch_t* b = NULL;
@@ -320,7 +384,7 @@ LIBXSTREAM_API int libsmm_acc_batch_tune(libsmm_acc_data_t datatype, const libsm
       p.chunk_elements = scales[i / 3] * 8 * 2 * 23 * 23;
       p.blocks_per_cu = wgs[i % 3];
     }
-    else p.per_stack = 1; /* last candidate: one launch per stack with its own configuration */
+    else p.per_stack = 1 + (i - last); /* one launch per stack with its own configuration, in order or in lanes */
     result = libsmm_acc_batch_create_ex(datatype, trial, nstacks, &p, &b);
     if (EXIT_SUCCESS == result) result = lxs_batch_enqueue(b, hs); /* warm-up */
     if (EXIT_SUCCESS == result) result = lxs_hip(hipEventRecord(e0, hs), "hipEventRecord");

@@ -142,13 +142,15 @@ def test_batch_long_runs(stream):
         res.free()
 
 
-def test_batch_per_stack_mode(stream):
-    """per_stack = 1: one libsmm_acc_process launch per stack; eager, then three captured-graph replays,
-    each checked with eager work in between."""
+@pytest.mark.parametrize("mode", [1, 2], ids=["in_order", "lanes"])
+def test_batch_per_stack_mode(stream, mode):
+    """per_stack = 1: one libsmm_acc_process launch per stack, in order; 2: the same launches spread over
+    side streams that fork from the caller's stream and join it (concurrent branches of one graph). Eager,
+    then three captured-graph replays, each checked with eager work in between."""
     works = make_mixed_batch(12, seed=17, dtype=np.float64, len_range=(20, 3000), c_init="uniform")
     res = Resident(works, stream)
     try:
-        batch = lx.Batch(lx.REAL_8, res.items, lx.BatchParams(per_stack=1))
+        batch = lx.Batch(lx.REAL_8, res.items, lx.BatchParams(per_stack=mode))
         c_ref = [w.c.ravel().copy() for w in works]
         for step in range(4):
             if step == 0:
@@ -166,7 +168,7 @@ def test_batch_per_stack_mode(stream):
         res.free()
 
 
-@pytest.mark.parametrize("per_stack", [0, 1])
+@pytest.mark.parametrize("per_stack", [0, 1, 2])
 def test_batch_unsorted_stacks(stream, per_stack):
     """Stacks whose C-runs are interleaved (legal input): both batch modes flush with atomic adds, so
     every stack still matches the oracle, whatever configuration the per-stack mode resolves."""
@@ -208,7 +210,7 @@ def test_batch_tune_schedule(stream):
         best, ms = lx.Batch.tune(lx.REAL_8, res.items)
         # either schedule may win: per-stack launches, or the persistent kernel with its chunk size and
         # work-groups per CU
-        assert ms > 0 and (best.per_stack == 1 or (best.chunk_elements > 0 and best.blocks_per_cu > 0))
+        assert ms > 0 and (best.per_stack in (1, 2) or (best.chunk_elements > 0 and best.blocks_per_cu > 0))
         for i in range(len(works)):
             assert np.array_equal(res.download(i), before[i]), "tuning wrote the caller's C (stack %d)" % i
         expected = [oracle.stack_dd(w.m, w.n, w.k, w.stack, w.a, w.b, c_in=w.c.ravel()) for w in works]
@@ -237,7 +239,7 @@ def test_batch_full_workload(stream):
     res = Resident(works, stream)
     try:
         best, _ = lx.Batch.tune(lx.REAL_8, res.items)
-        for params in (best, lx.BatchParams(0, 0, 0)):
+        for params in (best, lx.BatchParams(0, 0, 0), lx.BatchParams(0, 0, 2)):
             for i, w in enumerate(works):
                 res.items[i][7].upload(w.c)
             batch = lx.Batch(lx.REAL_8, res.items, params)
