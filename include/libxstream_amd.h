@@ -188,7 +188,8 @@ LIBXSTREAM_API int libsmm_acc_mlp_predict_config(libsmm_acc_mlp_t* model, libsmm
 LIBXSTREAM_API int libsmm_acc_mlp_predict_topk(libsmm_acc_mlp_t* model, libsmm_acc_data_t datatype, int m, int n,
   int k, int topk, libsmm_acc_smm_config_t* configs, double* scores, int* count);
 /** install (or remove: NULL) the model as predictor for shapes without a tuned configuration; a
- *  prediction below min_confidence leaves the built-in default in place */
+ *  prediction below min_confidence leaves the built-in default in place. The caller keeps ownership:
+ *  destroy the model only after removing it or after libsmm_acc_finalize */
 LIBXSTREAM_API int libsmm_acc_learned_set(libsmm_acc_mlp_t* model, double min_confidence);
 
 /** profiling: per-kernel launch records timed by HIP events on the launch stream */

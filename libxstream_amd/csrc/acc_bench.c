@@ -41,14 +41,16 @@ static void bench_init_mat(ELEM_TYPE* mat, int seed, int rows, int cols, double 
   for (i = 0; i < rows * cols; ++i) mat[i] = (ELEM_TYPE)(s1 * (i + 1));
 }
 
-/* runs of stack_size / nc +- (that - 4) entries per C block, a and b drawn from the table (acc_bench.h:48-79) */
+/* runs of stack_size / nc +- (that - 4) entries per C block, a and b drawn from the table (acc_bench.h:48-79);
+   a run takes at least one entry: with stack_size / nc = 1 the macro's step can be 0 and it never ends */
 static void bench_init_stack(int* stack, int stack_size, const int* rnd, int nrnd, int mn, int mk, int kn, int nc,
   int na, int nb) {
   const int navg = stack_size / nc, nimb = (1 < navg - 4 ? navg - 4 : 1);
   int i = 0, c = 0, ntop = 0;
   while (i < stack_size) {
     const int r = rnd[i % nrnd], next = c + 1;
-    ntop += navg + (r % (2 * nimb) - nimb);
+    const int step = navg + (r % (2 * nimb) - nimb);
+    ntop += (0 < step ? step : 1);
     if (stack_size < ntop) ntop = stack_size;
     for (; i < ntop; ++i) {
       stack[3 * i + 0] = (rnd[(2 * i + 0) % nrnd] % na) * mk + 1;

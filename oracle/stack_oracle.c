@@ -157,7 +157,8 @@ int oracle_init_stack(int* stack, int stack_size, int rnd_size, const int* rnd, 
   while (i < stack_size) {
     const int r = rnd[i % rnd_size];
     const int next = c + 1;
-    ntop += navg + (r % (2 * nimb) - nimb);
+    const int step = navg + (r % (2 * nimb) - nimb);
+    ntop += (0 < step ? step : 1); /* the macro's step is 0 for an even r when navg = 1: it never ends there */
     if (stack_size < ntop) ntop = stack_size;
     for (; i < ntop; ++i) {
       const int ia = rnd[(2 * i + 0) % rnd_size] % na;

@@ -3,8 +3,8 @@
  * learned-predictor slot and the thread-local error state, driven from 8 threads at once the way
  * concurrent libsmm_acc_process calls drive them (samples/smm/smm_kernel.c:91,109-113: the reference
  * guards its registry with lock stripes; libsmm_acc_is_thread_safe returns 1). Built with
- * -fsanitize=thread over the C host sources (tests/native/Makefile); the test fails on any
- * ThreadSanitizer report or a wrong read-back.
+ * -fsanitize=thread, and again with -fsanitize=address,undefined, over the C host sources
+ * (tests/native/Makefile); the tests fail on any sanitizer report (leaks included) or a wrong read-back.
  * usage: threads_host <learned model file>
  */
 #include <pthread.h>
@@ -69,6 +69,7 @@ int main(int argc, char* argv[]) {
   for (i = 0; i < NTHREADS; ++i) pthread_create(threads + i, NULL, work, (void*)(size_t)i);
   for (i = 0; i < NTHREADS; ++i) pthread_join(threads[i], NULL);
   libsmm_acc_finalize();
+  libsmm_acc_mlp_destroy(model); /* the caller keeps ownership of an installed model */
   printf("threads_host: %d threads x %d iterations, %d learned lookups, %d failures\n", NTHREADS, NITER, learned,
     failures);
   return 0 == failures ? 0 : 1;

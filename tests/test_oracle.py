@@ -187,7 +187,7 @@ def init_stack_py(stack_size, rnd, mn, mk, kn, nc, na, nb):
     while i < stack_size:
         r = rnd[i % len(rnd)]
         nxt = c + 1
-        ntop += navg + (r % (2 * nimb) - nimb)
+        ntop += max(1, navg + (r % (2 * nimb) - nimb))  # at least one entry (the macro loops for navg = 1)
         ntop = min(ntop, stack_size)
         while i < ntop:
             out.append((rnd[(2 * i) % len(rnd)] % na * mk + 1, rnd[(2 * i + 1) % len(rnd)] % nb * kn + 1, c * mn + 1))
@@ -222,3 +222,33 @@ def test_cpu_baseline_matches_oracle():
     oracle.cpu_stack(23, 23, 23, w.stack, w.a, w.b, c, nrepeat=1, warmup=0, nthreads=2)
     hi, lo = oracle.stack_dd(23, 23, 23, w.stack, w.a, w.b, c_in=w.c.ravel())
     assert oracle.error(c.ravel(), hi, lo)[0] < 1e-13
+
+
+def test_init_stack_invariants():
+    """INIT_STACK's structure read back from the stack it made (samples/smm/acc_bench.h:48-79): every entry's
+    A and B block are rnd[2i] mod na and rnd[2i+1] mod nb of the supplied table, C blocks are 0, 1, ... in
+    order, each run but the last is navg + (rnd[start] mod 2 nimb) - nimb entries long (start: its first
+    entry), and the last C block takes the rest. A transposed index, a wrong base or a dropped term in the
+    restatement breaks one of these."""
+    for S, mk, kn, mn, nc, na, nb, R in ((30000, 529, 529, 529, 1875, 18750, 18750, 30000),
+                                         (5000, 88, 198, 36, 250, 777, 555, 997), (64, 4, 6, 6, 32, 3, 2, 7)):
+        rnd = oracle.glibc_rand(25071975 + S, R)
+        st = oracle.init_stack(S, rnd, mn, mk, kn, nc, na, nb).astype(np.int64)
+        i = np.arange(S)
+        assert np.array_equal((st[:, 0] - 1) // mk, rnd[(2 * i) % R] % na)
+        assert np.array_equal((st[:, 1] - 1) // kn, rnd[(2 * i + 1) % R] % nb)
+        assert np.all((st[:, 0] - 1) % mk == 0) and np.all((st[:, 1] - 1) % kn == 0) and np.all((st[:, 2] - 1) % mn == 0)
+        cblk = (st[:, 2] - 1) // mn
+        assert cblk[0] == 0 and np.all(np.diff(cblk) >= 0) and np.all(np.diff(cblk) <= 1)
+        starts = np.flatnonzero(np.diff(np.concatenate([[-1], cblk])))
+        lengths = np.diff(np.concatenate([starts, [S]]))
+        navg = S // nc
+        nimb = max(1, navg - 4)
+        expect = navg + rnd[starts[:-1] % R] % (2 * nimb) - nimb
+        assert np.array_equal(lengths[:-1], expect)
+        assert cblk[-1] == min(nc - 1, len(starts) - 1)
+    # navg = 1 (nc = S): the macro's step is 0 for an even draw and it would never end; one entry per run here
+    rnd = oracle.glibc_rand(7, 50)
+    st = oracle.init_stack(50, rnd, 1, 1, 1, 50, 5, 5)
+    assert np.array_equal(st, init_stack_py(50, list(rnd), 1, 1, 1, 50, 5, 5))
+    assert np.array_equal(st[:, 2], np.arange(1, 51))
