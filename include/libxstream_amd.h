@@ -81,8 +81,14 @@ LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* q
  *  kernel's fixed k per stage, 0 = any); returns their number, fills up to cap */
 LIBXSTREAM_API int libsmm_acc_kernel_candidates(libsmm_acc_data_t datatype, int m, int n, int k, int* quintuples,
   int cap);
-/** load tuned parameters from a CSV file (reference or MI355X header); returns #rows loaded or <0 */
+/** load tuned parameters from a CSV file (reference or MI355X header); returns #rows loaded or <0. With a
+ *  DEVICE column, the rows of the device whose name best matches the active device (share of the row's
+ *  name words found in its marketing and gcnArch name) take precedence; rows of other devices fill the
+ *  remaining shapes, and among rows of equal standing the higher GFLOPS wins (smm_acc.c:98-235) */
 LIBXSTREAM_API int libsmm_acc_params_load(const char* filename);
+/** as libsmm_acc_params_load, matched against the given device name instead of the active device's
+ *  (NULL: the active device; "": no device matching) */
+LIBXSTREAM_API int libsmm_acc_params_load_device(const char* filename, const char* device);
 /** write the registry as an MI355X CSV (header line + one row per shape); returns #rows or <0 */
 LIBXSTREAM_API int libsmm_acc_params_save(const char* filename, const char* device);
 

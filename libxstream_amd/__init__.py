@@ -144,6 +144,7 @@ def _declare(L):
         "libsmm_acc_kernel_variants": (i, [i, pi, i]),
         "libsmm_acc_kernel_candidates": (i, [i, i, i, i, pi, i]),
         "libsmm_acc_params_load": (i, [ctypes.c_char_p]),
+        "libsmm_acc_params_load_device": (i, [ctypes.c_char_p, ctypes.c_char_p]),
         "libsmm_acc_params_save": (i, [ctypes.c_char_p, ctypes.c_char_p]),
         "libsmm_acc_read_ref_params": (i, [ctypes.c_char_p, i, i, ctypes.c_char_p, sz, pi, pi, ctypes.POINTER(ctypes.c_double)]),
         "libsmm_acc_batch_create": (i, [i, ctypes.POINTER(BatchStack), i, pvp]),

@@ -36,6 +36,7 @@ typedef struct lxs_config_t {
                     contiguous run, which lets configurations with flush 1 write C without atomics (default 0) */
   int ncus;      /* compute units of the active device */
   char arch[256]; /* gcnArchName of the active device */
+  char devname[512]; /* marketing name and gcnArchName of the active device ("" before one is active) */
   pthread_mutex_t lock;
 } lxs_config_t;
 

@@ -80,6 +80,7 @@ static int lxs_activate(int device_id) {
     lxs_cfg.device = device_id;
     lxs_cfg.ncus = props.multiProcessorCount;
     snprintf(lxs_cfg.arch, sizeof(lxs_cfg.arch), "%s", props.gcnArchName);
+    snprintf(lxs_cfg.devname, sizeof(lxs_cfg.devname), "%s %s", props.name, props.gcnArchName);
     lxs_tls_device = device_id;
     if (2 <= lxs_cfg.verbosity) {
       fprintf(stderr, "INFO ACC/HIP: ndevices=%i device%i=\"%s\" arch=%s cus=%i\n", lxs_cfg.ndevices, device_id,

@@ -12,6 +12,7 @@
 #  define _GNU_SOURCE /* dladdr */
 #endif
 #include "lxs_internal.h"
+#include <ctype.h>
 #include <dlfcn.h>
 #include <string.h>
 #include <strings.h>
@@ -24,6 +25,7 @@
 
 typedef struct lxs_smm_slot_t {
   int type, m, n, k, used, source;
+  int devmatch; /* 1: a CSV row of the device that best matches the active one */
   libsmm_acc_smm_config_t config;
 } lxs_smm_slot_t;
 
@@ -234,7 +236,7 @@ static int lxs_smm_env(libsmm_acc_smm_config_t* config) {
 }
 
 static int lxs_smm_register(int datatype, int m, int n, int k, const libsmm_acc_smm_config_t* config, int source,
-  int keep_best);
+  int keep_best, int devmatch);
 
 static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config, int* source) {
   const lxs_smm_slot_t* slot;
@@ -250,7 +252,7 @@ static void lxs_smm_resolve(int datatype, int m, int n, int k, libsmm_acc_smm_co
     if (EXIT_SUCCESS == lxs_learned_predict(datatype, m, n, k, config)) { /* cached for the next lookup */
       src = libsmm_acc_config_learned;
       config->s = LXS_SMM_S; /* the model learned from sweeps at this stack size */
-      (void)lxs_smm_register(datatype, m, n, k, config, libsmm_acc_config_learned, 0);
+      (void)lxs_smm_register(datatype, m, n, k, config, libsmm_acc_config_learned, 0, 0);
     }
     else {
       lxs_smm_default(datatype, m, n, k, config);
@@ -288,17 +290,21 @@ LIBXSTREAM_API int libsmm_acc_config_get(libsmm_acc_data_t datatype, int m, int 
   return (0 < config->nt ? EXIT_SUCCESS : EXIT_FAILURE);
 }
 
+/* keep_best = 0 replaces any entry. Otherwise a row of the device matching the active one (devmatch) wins
+   over rows of other devices whatever their speed, and among rows of equal standing a duplicate replaces
+   the entry only when it is faster (smm_acc.c:117-119, :204-235) */
 static int lxs_smm_register(int datatype, int m, int n, int k, const libsmm_acc_smm_config_t* config, int source,
-  int keep_best) {
+  int keep_best, int devmatch) {
   lxs_smm_slot_t* slot;
   int result = EXIT_SUCCESS;
   pthread_mutex_lock(&lxs_smm_lock);
   slot = lxs_smm_lookup(datatype, m, n, k, 1);
   if (NULL != slot) {
-    /* a duplicate row replaces the entry only when it is faster (smm_acc.c:117-119) */
-    if (0 == keep_best || 0 == slot->config.nt || slot->config.gflops < config->gflops) {
+    if (0 == keep_best || 0 == slot->config.nt || slot->devmatch < devmatch ||
+        (slot->devmatch == devmatch && slot->config.gflops < config->gflops)) {
       slot->config = *config;
       slot->source = source;
+      slot->devmatch = devmatch;
     }
   }
   else {
@@ -313,7 +319,7 @@ LIBXSTREAM_API int libsmm_acc_config_set(libsmm_acc_data_t datatype, int m, int 
   if (EXIT_SUCCESS != libsmm_acc_config_valid(datatype, m, n, k, config)) {
     return lxs_error(EXIT_FAILURE, "libsmm_acc_config_set: configuration not valid for this shape");
   }
-  return lxs_smm_register((int)datatype, m, n, k, config, libsmm_acc_config_tuned, 0);
+  return lxs_smm_register((int)datatype, m, n, k, config, libsmm_acc_config_tuned, 0, 0);
 }
 
 LIBXSTREAM_API int libsmm_acc_kernel_variants(libsmm_acc_data_t datatype, int* quintuples, int cap) {
@@ -384,9 +390,52 @@ static int lxs_column(char names[][16], int ncolumns, const char* name) {
   return -1;
 }
 
+/* lower-case alphanumeric words of a device name, without the "[0x...]" device id of the reference's rows */
+static int lxs_device_words(const char* name, char words[][32], int cap) {
+  int n = 0;
+  while (NULL != name && '\0' != *name && n < cap) {
+    int len = 0;
+    while ('\0' != *name && !isalnum((unsigned char)*name)) ++name;
+    while (isalnum((unsigned char)*name)) {
+      if (len < 31) words[n][len++] = (char)tolower((unsigned char)*name);
+      ++name;
+    }
+    words[n][len] = '\0';
+    if (0 < len && !('0' == words[n][0] && 'x' == words[n][1])) ++n;
+  }
+  return n;
+}
+
+/* share of a row's device words found in the active device's name (0 .. 1) */
+static double lxs_device_score(const char* row, char active[][32], int nactive) {
+  char words[16][32];
+  const int n = lxs_device_words(row, words, 16);
+  int i, j, hits = 0;
+  for (i = 0; i < n; ++i) {
+    for (j = 0; j < nactive && 0 != strcmp(words[i], active[j]); ++j);
+    hits += (j < nactive);
+  }
+  return 0 < n ? (double)hits / n : 0.0;
+}
+
+/* value of the given column of a CSV row ("" if the row is shorter) */
+static void lxs_row_device(const char* line, int column, char* device, size_t size) {
+  int i;
+  for (i = 0; i < column && '\0' != *line; ++i) {
+    line += strcspn(line, lxs_delims);
+    if ('\0' != *line) ++line;
+  }
+  snprintf(device, size, "%.*s", (int)strcspn(line, lxs_delims), line);
+}
+
 LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
-  char line[1024], names[32][16];
-  int ncolumns = 0, nrows = 0, native, with_device, with_perf;
+  return libsmm_acc_params_load_device(filename, NULL);
+}
+
+LIBXSTREAM_API int libsmm_acc_params_load_device(const char* filename, const char* device) {
+  char line[1024], names[32][16], best[256] = "", active[16][32];
+  int ncolumns = 0, nrows = 0, native, with_device, with_perf, nactive, devcol;
+  long body;
   FILE* const file = (NULL != filename ? fopen(filename, "r") : NULL);
   if (NULL == file) return -1;
   if (NULL == fgets(line, sizeof(line), file)) {
@@ -402,14 +451,41 @@ LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
   native = (0 <= lxs_column(names, ncolumns, "TM") && 0 <= lxs_column(names, ncolumns, "NT"));
   with_device = (0 <= lxs_column(names, ncolumns, "DEVICE"));
   with_perf = (0 <= lxs_column(names, ncolumns, "GFLOPS"));
-  if (0 > lxs_column(names, ncolumns, "TYPEID")) {
+  devcol = lxs_column(names, ncolumns, "DEVICE");
+  if (0 > lxs_column(names, ncolumns, "TYPEID") || (0 != with_device && 0 == native && 0 != devcol)) {
     fclose(file);
     return -1;
   }
+  /* rows of several devices: those of the device whose name best matches the active one take precedence
+     (smm_acc.c:150-235); the other devices' rows fill the remaining shapes, the fastest row per shape */
+  nactive = lxs_device_words(NULL != device ? device : lxs_cfg.devname, active, 16);
+  body = ftell(file);
+  if (0 != with_device && 0 < nactive && 0 <= body) {
+    double best_score = 0;
+    while (NULL != fgets(line, sizeof(line), file)) {
+      char row[256];
+      double score;
+      lxs_row_device(line, devcol, row, sizeof(row));
+      score = lxs_device_score(row, active, nactive);
+      if (best_score < score) {
+        best_score = score;
+        snprintf(best, sizeof(best), "%s", row);
+      }
+    }
+    if (0 != fseek(file, body, SEEK_SET)) best[0] = '\0';
+    if (0 != lxs_cfg.verbosity && '\0' != best[0]) {
+      fprintf(stderr, "INFO LIBSMM: tuned parameters of \"%s\" take precedence\n", best);
+    }
+  }
   while (NULL != fgets(line, sizeof(line), file)) {
     libsmm_acc_smm_config_t config;
-    int key[4], ok = 0;
+    int key[4], ok = 0, devmatch = 0;
     if ('\0' == line[strspn(line, lxs_delims)]) continue; /* blank line */
+    if ('\0' != best[0]) {
+      char row[256];
+      lxs_row_device(line, devcol, row, sizeof(row));
+      devmatch = (0 == strcmp(row, best));
+    }
     memset(&config, 0, sizeof(config));
     if (0 != native) {
       char *save = NULL, *tok, values[32][64];
@@ -451,7 +527,10 @@ LIBXSTREAM_API int libsmm_acc_params_load(const char* filename) {
       }
     }
     if (0 != ok) {
-      if (EXIT_SUCCESS != lxs_smm_register(key[0], key[1], key[2], key[3], &config, libsmm_acc_config_tuned, 1)) break;
+      if (EXIT_SUCCESS != lxs_smm_register(key[0], key[1], key[2], key[3], &config, libsmm_acc_config_tuned, 1,
+                             devmatch)) {
+        break;
+      }
       ++nrows;
     }
     else if (0 != lxs_cfg.verbosity) {

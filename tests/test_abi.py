@@ -186,6 +186,32 @@ def test_params_load_reference_and_roundtrip(tmp_path, monkeypatch):
         assert L.libsmm_acc_finalize() == 0
 
 
+def test_params_load_device_match(tmp_path, monkeypatch):
+    """A CSV with rows of several devices (the reference's multi-device files): the rows of the device that best
+    matches the active one take precedence even when slower, other devices' rows fill the remaining shapes with
+    their fastest row, and without a device to match the fastest row of every shape wins (smm_acc.c:98-235)."""
+    L = lx.lib()
+    head = "DEVICE;TYPEID;M;N;K;S;GFLOPS;BS;BM;BN;BK;WS;WG;LU;NZ;AL;TB;TC;AP;AA;AB;AC\n"
+    tail = ";23;1;10;23;-2;-2;0;1;1;1;1;2;2;0;0\n"
+    rows = ["NVIDIA A100 80GB PCIe [0x1f79];3;23;23;23;30000;3000.0;50" + tail,
+            "AMD Instinct MI355X [0x75a3];3;23;23;23;30000;1000.0;20" + tail,
+            "AMD Instinct MI250X [0x740c];3;23;23;23;30000;2000.0;30" + tail,
+            "NVIDIA A100 80GB PCIe [0x1f79];3;13;13;13;30000;900.0;41" + tail.replace(";23;", ";13;", 1),
+            "AMD Instinct MI250X [0x740c];3;13;13;13;30000;800.0;42" + tail.replace(";23;", ";13;", 1)]
+    csv = tmp_path / "multi.csv"
+    csv.write_text(head + "".join(rows))
+    monkeypatch.setenv("OPENCL_LIBSMM_SMM_PARAMS", "0")
+    for device, bs23, bs13 in ((b"AMD Instinct MI355X gfx950:sramecc+:xnack-", 20, 41),
+                               (b"AMD Instinct MI250X gfx90a", 30, 42), (b"", 50, 41)):
+        assert L.libsmm_acc_init() == 0
+        try:
+            assert L.libsmm_acc_params_load_device(str(csv).encode(), device) == 5
+            assert lx.config_get(lx.REAL_8, 23, 23, 23)[0].bs == bs23, device
+            assert lx.config_get(lx.REAL_8, 13, 13, 13)[0].bs == bs13, device
+        finally:
+            assert L.libsmm_acc_finalize() == 0
+
+
 @pytest.mark.parametrize("shape", [(23, 23, 23), (5, 5, 5), (64, 64, 64), (45, 45, 13), (4, 9, 22), (80, 80, 80),
                                    (80, 80, 500), (7, 11, 3), (1, 1, 1), (17, 33, 2)])
 @pytest.mark.parametrize("tid", [lx.REAL_8, lx.REAL_4])

@@ -43,3 +43,68 @@ def test_report_and_host_check(binary, bar, args):
     gflops = float(re.search(r"device: \S+ ms (\S+) GFLOPS/s", out).group(1))
     diff = float(re.search(r"diff.cur: (\S+)", out).group(1))
     assert gflops > 0 and diff <= bar, out
+
+
+def _reports(out):
+    """(header fields, diff.cur) of every benchmark in a report"""
+    lines = out.splitlines()
+    heads = [i for i, line in enumerate(lines) if line.split(" ")[0].endswith(("acc_bench", "acc_bench_f32"))]
+    res = []
+    for i in heads:
+        fields = [int(x) for x in lines[i].split()[1:]]
+        diff = next((float(line.split()[1]) for line in lines[i:] if line.startswith("diff.cur:")), None)
+        res.append((fields, diff))
+    return res
+
+
+@pytest.mark.gpu
+def test_shape_file_and_diff_max(tmp_path):
+    """a file of argument lists (blank lines skipped, "MxNxK" or "M N K"): one benchmark per line and diff.max
+    over them (samples/smm/acc_bench.c:148, :442-462, :491-557)"""
+    f = tmp_path / "shapes.txt"
+    f.write_text("2 3000 23\n\n2 2000 4x9x22\n5x5x5 100\n")
+    rc, out, err = _run(BIN, [f], env={"CHECK": "1"})
+    assert rc == 0, out + err
+    reps = _reports(out)
+    assert [r[0][:5] for r in reps] == [[2, 3000, 23, 23, 23], [2, 2000, 4, 9, 22], [3, 30000, 5, 5, 5]], out
+    assert reps[2][0][5] == 100  # nc from the line, na = nb = 10 nc
+    diffs = [r[1] for r in reps]
+    assert all(d is not None and d <= 1e-13 for d in diffs), out
+    dmax = float(re.search(r"diff.max: (\S+)", out).group(1))
+    assert dmax == pytest.approx(max(diffs), rel=1e-5, abs=1e-300)
+
+
+@pytest.mark.gpu
+def test_performance_floors():
+    """CHECK_DEV / CHECK_HST / CHECK_H2D: the run fails when the device, host or copy-in rate is below the floor
+    (samples/smm/acc_bench.c:463-477)"""
+    rc, out, err = _run(BIN, [2, 3000, 23], env={"CHECK": "1", "CHECK_DEV": "1e9"})
+    assert rc != 0 and "FAILED" in err, out + err
+    rc, out, err = _run(BIN, [2, 3000, 23], env={"CHECK": "1", "CHECK_HST": "1e9"})
+    assert rc != 0 and "FAILED" in err, out + err
+    rc, out, err = _run(BIN, [2, 3000, 23], env={"CHECK": "1", "CHECK_H2D": "1e9"})
+    assert rc != 0 and "FAILED" in err, out + err
+    rc, out, err = _run(BIN, [2, 3000, 23], env={"CHECK": "1", "CHECK_DEV": "1", "CHECK_HST": "1e-3", "CHECK_H2D": "1e-3"})
+    assert rc == 0, out + err
+
+
+@pytest.mark.gpu
+def test_stack_size_forms():
+    """a k/m/g byte budget (the smallest stack whose pools and stack reach it), BATCHSIZE_SMM=<size>,<nrepeat>,
+    a negative size (a random multiple of 100 up to its magnitude) and CHECK=0 (no host run)"""
+    rc, out, err = _run(BIN, [2, "4m", 23], env={"CHECK": "1"})
+    assert rc == 0, out + err
+    (nr, s, m, n, k, nc, na, nb), diff = _reports(out)[0]
+
+    def size(s):
+        nc = max(s // 16, 1)
+        return 8 * (m * k + k * n) * 10 * nc + 12 * s
+
+    assert size(s) >= 4 << 20 > size(s - 1) and diff <= 1e-13, out
+    rc, out, err = _run(BIN, [2, 500, 13], env={"CHECK": "1", "BATCHSIZE_SMM": "2000,4"})
+    assert rc == 0 and _reports(out)[0][0][:2] == [4, 2000], out + err
+    rc, out, err = _run(BIN, [2, -5000, 23], env={"CHECK": "1"})
+    (nr, s, *_), diff = _reports(out)[0]
+    assert rc == 0 and s % 100 == 0 and 100 <= s <= 5000 and nr >= 3 and diff <= 1e-13, out + err
+    rc, out, err = _run(BIN, ["", 3000, 23], env={"CHECK": "0"})
+    assert rc == 0 and "host:" not in out and _reports(out)[0][0][0] == 66, out + err
