@@ -55,6 +55,36 @@ __device__ __forceinline__ T gload(const T* p) {
   return *(const __attribute__((address_space(1))) T*)p;
 }
 
+/* two consecutive elements at the element type's alignment (one 16-byte load for FP64, 8-byte for FP32;
+   the operand blocks only guarantee the element alignment) */
+typedef double d2u_t __attribute__((ext_vector_type(2), aligned(8)));
+typedef float f2u_t __attribute__((ext_vector_type(2), aligned(4)));
+template <typename T>
+struct pair_of;
+template <>
+struct pair_of<double> {
+  typedef d2u_t type;
+};
+template <>
+struct pair_of<float> {
+  typedef f2u_t type;
+};
+template <typename T>
+__device__ __forceinline__ typename pair_of<T>::type gload2(const T* p) {
+  return *(const __attribute__((address_space(1))) typename pair_of<T>::type*)p;
+}
+
+/* Paired tiles: output tiles 2p and 2p+1 of one dimension interleave the 32 rows (or columns) from 32p on,
+   so that one two-element load per lane feeds both tiles. Lane l of the pair loads the elements at
+   s = min(32p + 2l, extent - 2) and s + 1 (clamped so both stay inside the extent) for tiles 2p and 2p+1.
+   Output row l of tile 2p+q is then element s + q; it is written by exactly one lane: an unclamped pair,
+   or for an odd extent the lane whose pair would start at extent - 1 (tile 2p+1 holds that last element). */
+__device__ __forceinline__ int pair_start(int p, int l, int extent) { return min(32 * p + 2 * l, extent - 2); }
+__device__ __forceinline__ bool pair_owner(int p, int q, int l, int extent) {
+  const int base = 32 * p + 2 * l;
+  return base <= extent - 2 || (1 == q && base == extent - 1);
+}
+
 /* Long C-runs are cut into pieces at the segment boundaries of the stack (multiples of the
    segment size bs, where one work-group's entries begin). Boundary entry i starts a new piece
    when the entry tsplit places earlier has the same C block ci: in a stack sorted by C, the run

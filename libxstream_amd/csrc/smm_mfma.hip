@@ -26,6 +26,7 @@
  * Fragment and output-tile lane layouts: lxs_mfma.h.
  */
 #include <hip/hip_runtime.h>
+#include <climits>
 #include "lxs_kernels.h"
 #include "lxs_mfma.h"
 
@@ -264,7 +265,7 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
  * next run stay in flight across them.
  */
 template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
-__global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, const T* __restrict__ adata,
+__global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
   int kt_unused, int atomic, int ps, int pz, int tsplit) {
   typedef typename mfma<T>::acc_t acc_t;
@@ -282,8 +283,58 @@ __global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, co
 
   const int begin = blockIdx.x * bs;
   const int end = min(begin + bs, stack_size);
-  int h = end; /* first piece head in [begin, end) */
-  for (int base = begin; base < end; base += 64) {
+  /* WIN: the stack around the work-group's entries is held in registers: lane l has the C, A and B offsets of
+     entry begin + l and the C offset of entry begin - 64 + l (NONE outside the stack). The head and run-end
+     scans, the cut tests and the entries' A and B offsets then come from these registers (shuffles and
+     readlanes in wave-uniform code) instead of chains of dependent loads; entries outside the window are
+     loaded. Kernels of 16 tiles or more keep loading: their accumulators leave no registers for the window
+     (a spilled window register reloaded under a partial EXEC mask would feed the readlanes stale lanes). */
+  constexpr bool WIN = (NTILE < 16);
+  constexpr int NONE = INT_MIN; /* no C block: differs from every offset */
+  const int wi = begin + lane, pi = begin - 64 + lane;
+  const bool win = (wi < stack_size);
+  int wc = NONE, wa = 0, wb = 0, pc = NONE;
+  if constexpr (WIN) {
+    wc = (win ? stack[ps * wi + 2] : NONE);
+    wa = (win ? stack[ps * wi + 0] - pz : 0);
+    wb = (win ? stack[ps * wi + 1] - pz : 0);
+    pc = (0 <= pi ? stack[ps * pi + 2] : NONE);
+  }
+  /* C offset of a wave-uniform entry j (NONE outside the stack) */
+  const auto c_uni = [&](int j) -> int {
+    if constexpr (WIN) {
+      if (begin <= j && j < begin + 64) return __builtin_amdgcn_readlane(wc, j - begin);
+      if (begin - 64 <= j && j < begin) return __builtin_amdgcn_readlane(pc, j - begin + 64);
+    }
+    return (0 <= j && j < stack_size) ? stack[ps * j + 2] : NONE;
+  };
+  /* A and B offsets of a wave-uniform entry e of the stack */
+  const auto a_uni = [&](int e) -> int {
+    if constexpr (WIN) {
+      if (begin <= e && e < begin + 64) return __builtin_amdgcn_readlane(wa, e - begin);
+    }
+    return stack[ps * e + 0] - pz;
+  };
+  const auto b_uni = [&](int e) -> int {
+    if constexpr (WIN) {
+      if (begin <= e && e < begin + 64) return __builtin_amdgcn_readlane(wb, e - begin);
+    }
+    return stack[ps * e + 1] - pz;
+  };
+  /* first piece head in [begin, end): a run head, or begin where it cuts a long run (lxs_cut) */
+  int h = end;
+  int base = begin;
+  if constexpr (WIN) {
+    /* C offset of the entry before each lane's: lanes 1..63 read lane l - 1 of wc, lane 0 reads lane 63 of
+       the source, which holds entry begin - 1 (one shuffle, no lane-dependent select around a cross-lane read) */
+    const int cprev = __shfl(63 == lane ? pc : wc, (lane + 63) & 63);
+    const int ccut = ((0 < tsplit && tsplit <= begin) ? c_uni(begin - tsplit) : NONE);
+    const bool head = (wi < end) && (0 == wi || wc != cprev || (wi == begin && ccut == wc));
+    const unsigned long long mask = __ballot(head);
+    if (0 != mask) h = begin + __ffsll((long long)mask) - 1;
+    base = begin + 64;
+  }
+  for (; h >= end && base < end; base += 64) {
     const int i = base + lane;
     bool head = false;
     if (i < end) {
@@ -291,65 +342,114 @@ __global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, co
              (i == begin && lxs_cut(stack, ps, i, stack[ps * i + 2], tsplit));
     }
     const unsigned long long mask = __ballot(head);
-    if (0 != mask) {
-      h = base + __ffsll((long long)mask) - 1;
-      break;
-    }
+    if (0 != mask) h = base + __ffsll((long long)mask) - 1;
   }
   if (h >= end) return; /* the same for every wavefront of the work-group */
-
-  int aoff[TI], boff[TJ];
-#pragma unroll
-  for (int ti = 0; ti < TI; ++ti) aoff[ti] = min(16 * ti + li, M - 1);
-#pragma unroll
-  for (int tj = 0; tj < TJ; ++tj) boff[tj] = min(16 * tj + li, N - 1);
-  const T* const zeros = lxs::zeros<T>();
-  acc_t acc[TI][TJ], prod[TI][TJ];
-  T fa[KB][TI], fb[KB][TJ];
-  const acc_t zero = {T(0), T(0), T(0), T(0)};
-#pragma unroll
-  for (int ti = 0; ti < TI; ++ti) {
-#pragma unroll
-    for (int tj = 0; tj < TJ; ++tj) acc[ti][tj] = prod[ti][tj] = zero;
-  }
-
-#define LXS_LOAD_KSTEP(KBI, A0, B0, KSTEP) \
-  do { \
-    const int k_ = 4 * (KSTEP) + lk; \
-    const bool kv_ = (k_ < K); \
-    _Pragma("unroll") for (int ti = 0; ti < TI; ++ti) { \
-      fa[KBI][ti] = gload(kv_ ? (adata + ((A0) + k_ * M + aoff[ti])) : zeros); \
-    } \
-    _Pragma("unroll") for (int tj = 0; tj < TJ; ++tj) { \
-      fb[KBI][tj] = gload(kv_ ? (bdata + ((B0) + k_ * N + boff[tj])) : zeros); \
-    } \
-  } while (0)
-
-  int e = -1, a0 = 0, b0 = 0; /* entry whose first k-stage is in flight in the fragment registers */
-  for (;;) {                  /* one run (or piece of a cut run) [h, r) per iteration */
-    const int c0 = stack[ps * h + 2];
-    int r = stack_size;
-    for (int base = h + 1; base < stack_size; base += 64) {
-      const int i = base + lane;
+  /* end of the piece that starts at h: the first i > h with another C block, or a segment boundary past
+     end where lxs_cut cuts the run */
+  const auto run_end = [&](int h0, int c0) -> int {
+    int from = h0 + 1;
+    if constexpr (WIN) {
+      if (h0 + 1 < begin + 64) {
+        const int jc = wi - tsplit;
+        const bool bnd = (end <= wi && 0 == (wi - end) % bs && 0 < tsplit && tsplit <= wi);
+        const int x = __shfl(pc, (jc - begin + 64) & 63), y = __shfl(wc, (jc - begin) & 63);
+        const bool need = (bnd && wc == c0); /* the cut test decides this lane */
+        const bool inwin = (begin - 64 <= jc && jc < begin + 64);
+        int cj = (jc < begin ? x : y); /* C offset of entry jc where the test needs it */
+        if (need && !inwin) cj = stack[ps * jc + 2];
+        const bool stop = (h0 < wi) && (!win || wc != c0 || (need && cj == c0));
+        const unsigned long long mask = __ballot(stop);
+        if (0 != mask) return begin + __ffsll((long long)mask) - 1;
+        from = begin + 64;
+      }
+    }
+    for (int b0 = from; b0 < stack_size; b0 += 64) {
+      const int i = b0 + lane;
       bool stop = (i >= stack_size);
       if (!stop) {
         stop = (stack[ps * i + 2] != c0) || (end <= i && 0 == (i - end) % bs && lxs_cut(stack, ps, i, c0, tsplit));
       }
       const unsigned long long mask = __ballot(stop);
-      if (0 != mask) {
-        r = base + __ffsll((long long)mask) - 1;
-        break;
+      if (0 != mask) return b0 + __ffsll((long long)mask) - 1;
+    }
+    return stack_size;
+  };
+
+  /* PAIR (even tile counts): paired tiles (lxs::pair_start), one two-element load per lane feeds two tiles,
+     which halves the fragment load instructions and the cache lines they touch */
+  constexpr bool PAIR = (0 == TI % 2) && (0 == TJ % 2);
+  constexpr int PI = (PAIR ? TI / 2 : 1), PJ = (PAIR ? TJ / 2 : 1);
+  typedef typename lxs::pair_of<T>::type v2_t;
+  int aoff[TI], boff[TJ];
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti) aoff[ti] = (PAIR ? lxs::pair_start(ti >> 1, li, M) : min(16 * ti + li, M - 1));
+#pragma unroll
+  for (int tj = 0; tj < TJ; ++tj) boff[tj] = (PAIR ? lxs::pair_start(tj >> 1, li, N) : min(16 * tj + li, N - 1));
+  const T* const zeros = lxs::zeros<T>();
+  /* the matrix core accumulates up to SEG_LEN entries in prod; each segment is then added to the run total,
+     which lives in this wavefront's partial block in LDS (red, slot q): total = 0 + prod first, total + prod
+     after, the same additions a register total would make, without its registers */
+  acc_t prod[TI][TJ];
+  T fa[KB][TI], fb[KB][TJ];
+  v2_t pa[KB][PI], pb[KB][PJ];
+  const acc_t zero = {T(0), T(0), T(0), T(0)};
+#pragma unroll
+  for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+    for (int tj = 0; tj < TJ; ++tj) prod[ti][tj] = zero;
+  }
+  bool folded = false; /* the run total in LDS holds a segment */
+  const auto fold = [&]() {
+#pragma unroll
+    for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+      for (int tj = 0; tj < TJ; ++tj) {
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          T* const slot = red + ((q * NTILE + ti * TJ + tj) * 4 + x) * 64 + lane;
+          *slot = (folded ? *slot : T(0)) + prod[ti][tj][x];
+        }
+        prod[ti][tj] = zero;
       }
     }
+    folded = true;
+  };
+
+#define LXS_LOAD_KSTEP(KBI, A0, B0, KSTEP) \
+  do { \
+    const int k_ = 4 * (KSTEP) + lk; \
+    const bool kv_ = (k_ < K); \
+    if constexpr (PAIR) { \
+      _Pragma("unroll") for (int p_ = 0; p_ < PI; ++p_) { \
+        pa[KBI][p_] = lxs::gload2(kv_ ? (adata + ((A0) + k_ * M + aoff[2 * p_])) : zeros); \
+      } \
+      _Pragma("unroll") for (int p_ = 0; p_ < PJ; ++p_) { \
+        pb[KBI][p_] = lxs::gload2(kv_ ? (bdata + ((B0) + k_ * N + boff[2 * p_])) : zeros); \
+      } \
+    } \
+    else { \
+      _Pragma("unroll") for (int ti = 0; ti < TI; ++ti) { \
+        fa[KBI][ti] = gload(kv_ ? (adata + ((A0) + k_ * M + aoff[ti])) : zeros); \
+      } \
+      _Pragma("unroll") for (int tj = 0; tj < TJ; ++tj) { \
+        fb[KBI][tj] = gload(kv_ ? (bdata + ((B0) + k_ * N + boff[tj])) : zeros); \
+      } \
+    } \
+  } while (0)
+
+  int e = -1, a0 = 0, b0 = 0; /* entry whose first k-stage is in flight in the fragment registers */
+  for (;;) {                  /* one run (or piece of a cut run) [h, r) per iteration */
+    const int c0 = c_uni(h);
+    const int r = run_end(h, c0);
     /* a piece that starts or ends inside a run shares its C block with another work-group */
-    const bool flush_atomic = atomic || (0 < h && stack[ps * (h - 1) + 2] == c0) ||
-                              (r < stack_size && stack[ps * r + 2] == c0);
+    const bool flush_atomic = atomic || (0 < h && c_uni(h - 1) == c0) || (r < stack_size && c_uni(r) == c0);
     const bool own_next = (r < end);
     int cur = h + q;
     if (cur < r) {
       if (cur != e) { /* not prefetched: load the first k-stage now */
-        a0 = stack[ps * cur + 0] - pz;
-        b0 = stack[ps * cur + 1] - pz;
+        a0 = a_uni(cur);
+        b0 = b_uni(cur);
 #pragma unroll
         for (int kb = 0; kb < KB; ++kb) {
           if (kb < KS) LXS_LOAD_KSTEP(kb, a0, b0, kb);
@@ -363,7 +463,7 @@ __global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, co
         const int cand = r + q;
         const bool pre = !in_run && own_next && cand < stack_size;
         const int nxt = (in_run ? nx : (pre ? cand : -1));
-        const int na = (0 <= nxt ? stack[ps * nxt + 0] - pz : a0), nb = (0 <= nxt ? stack[ps * nxt + 1] - pz : b0);
+        const int na = (0 <= nxt ? a_uni(nxt) : a0), nb = (0 <= nxt ? b_uni(nxt) : b0);
         for (int s = 0; s < NS; ++s) {
           const bool last = (s + 1 == NS);
           const int la = (last ? na : a0), lb = (last ? nb : b0), ns = (last ? 0 : s + 1);
@@ -374,7 +474,14 @@ __global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, co
 #pragma unroll
               for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
-                for (int tj = 0; tj < TJ; ++tj) prod[ti][tj] = mfma<T>::mma(fa[kb][ti], fb[kb][tj], prod[ti][tj]);
+                for (int tj = 0; tj < TJ; ++tj) {
+                  if constexpr (PAIR) {
+                    prod[ti][tj] = mfma<T>::mma(pa[kb][ti >> 1][ti & 1], pb[kb][tj >> 1][tj & 1], prod[ti][tj]);
+                  }
+                  else {
+                    prod[ti][tj] = mfma<T>::mma(fa[kb][ti], fb[kb][tj], prod[ti][tj]);
+                  }
+                }
               }
             }
             /* refill the slot with the next unit while the matrix core works */
@@ -382,14 +489,7 @@ __global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, co
           }
         }
         if (SEG_LEN == ++nent) { /* fold the segment into the run total */
-#pragma unroll
-          for (int ti = 0; ti < TI; ++ti) {
-#pragma unroll
-            for (int tj = 0; tj < TJ; ++tj) {
-              acc[ti][tj] += prod[ti][tj];
-              prod[ti][tj] = zero;
-            }
-          }
+          fold();
           nent = 0;
         }
         a0 = na;
@@ -401,31 +501,29 @@ __global__ void __launch_bounds__(64 * EP) smm_mfma_ep(T* __restrict__ cdata, co
         cur = nx;
       }
     }
-    /* this wavefront's partial block into LDS, then the fixed-order sum and the flush, tiles shared out */
-#pragma unroll
-    for (int ti = 0; ti < TI; ++ti) {
-#pragma unroll
-      for (int tj = 0; tj < TJ; ++tj) {
-        const acc_t sum = acc[ti][tj] + prod[ti][tj];
-        acc[ti][tj] = prod[ti][tj] = zero;
-#pragma unroll
-        for (int x = 0; x < 4; ++x) red[((q * NTILE + ti * TJ + tj) * 4 + x) * 64 + lane] = sum[x];
-      }
-    }
+    /* this wavefront's partial block (run total plus the open segment) in LDS, then the fixed-order sum and
+       the flush, tiles shared out */
+    fold();
+    folded = false;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
     {
       T* const cblk = cdata + (c0 - pz);
       for (int t = q; t < NTILE; t += EP) {
-        const int ti = t / TJ, tj = t - ti * TJ, j = 16 * tj + li;
+        const int ti = t / TJ, tj = t - ti * TJ;
+        /* element of the block behind output row/column l of a tile, and whether this lane writes it */
+        const int j = (PAIR ? lxs::pair_start(tj >> 1, li, N) + (tj & 1) : 16 * tj + li);
+        const bool jown = (PAIR ? lxs::pair_owner(tj >> 1, tj & 1, li, N) : j < N);
 #pragma unroll
         for (int x = 0; x < 4; ++x) {
           T v = red[(t * 4 + x) * 64 + lane];
 #pragma unroll
           for (int p = 1; p < EP; ++p) v += red[((p * NTILE + t) * 4 + x) * 64 + lane];
-          const int i = 16 * ti + mfma<T>::row(lane, x);
-          if (i < M && j < N) {
+          const int ii = mfma<T>::row(lane, x);
+          const int i = (PAIR ? lxs::pair_start(ti >> 1, ii, M) + (ti & 1) : 16 * ti + ii);
+          const bool iown = (PAIR ? lxs::pair_owner(ti >> 1, ti & 1, ii, M) : i < M);
+          if (iown && jown) {
             T* const ptr = cblk + j * M + i;
             if (flush_atomic) {
               unsafeAtomicAdd(ptr, v);
