@@ -98,13 +98,16 @@ int lxs_oz_scale(int dtype, void* c, int ldc, int M, int N, double beta, void* h
 int lxs_oz_static_scratch(void** ptr);
 /* Ozaki scheme 2 (CRT): moduli in use order, pairwise coprime and at most 256, so that symmetric residues
    fit int8; the first np of them serve a call. Residue planes: np x Rp x Kp int8 of the integers
-   trunc(|v| 2^(mb - trim - E)) (E: the row's exponent, mb = 53 or 24) with the sign of v. */
+   trunc(|v| 2^(mb - trim - E)) (E: the row's exponent, mb = 53 or 24) with the sign of v, times the
+   reconstruction weight w_p = ((M / m_p) mod m_p)^-1 mod m_p when weight != 0 (one operand carries it, so
+   the int8 GEMM's residues are the CRT coefficients). */
 #define LXS_OZ_NPRIMES_MAX 16
 #define LXS_OZ_MODULI 256, 255, 253, 251, 247, 241, 239, 233, 229, 227, 223, 211, 199, 197, 193, 191
-int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int trim, int Rp, int Kp,
-  int* e, signed char* planes, void* hip_stream);
-/* C = alpha 2^(ea + eb + sh0) x + beta C, x the integer (|x| < M / 2, M the product of the first np moduli) of
-   the residues r_p = sum of the residue bytes of the prime's items zb[p] .. zb[p + 1] - 1, mod m_p */
+int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int weight, int trim, int Rp,
+  int Kp, int* e, signed char* planes, void* hip_stream);
+/* C = alpha 2^(ea + eb + sh0) x + beta C, x the integer (|x| < M / 2, M the product of the first np moduli) whose
+   CRT coefficients c_p = x w_p mod m_p are the sums of the residue bytes of the prime's items zb[p] .. zb[p + 1] - 1,
+   mod m_p (the A operand's residues carry w_p) */
 int lxs_oz_crt_recombine(int dtype, const unsigned char* r8, const int* zb, int np, int Mp, int Np, const int* ea,
   const int* eb, int M, int N, int sh0, double alpha, double beta, void* c, int ldc, void* hip_stream);
 int lxs_dgemm(const double* a, long ars, long aks, const double* b, long bks, long bcs, int M, int N, int K,

@@ -457,10 +457,10 @@ static int oz_gemm_real(ozaki_context_t* ctx, hipStream_t hs, int ta, int tb, in
       const int nplanes = (0 != l.crt ? np : s);
       void* token = (NULL == fixed ? lxs_profile_begin(hs) : NULL);
       if (0 != l.crt) {
-        result = lxs_oz_crt_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, np, trim, l.Mp, Kp,
+        result = lxs_oz_crt_preprocess(dtype, da + ts * (size_t)k0 * aks, ars, aks, M, kk, np, 1, trim, l.Mp, Kp,
           (int*)(base + l.ea), (signed char*)(base + l.planes_a), hs);
         if (0 == result) {
-          result = lxs_oz_crt_preprocess(dtype, db + ts * (size_t)k0 * bks, bcs, bks, N, kk, np, trim, l.Np, Kp,
+          result = lxs_oz_crt_preprocess(dtype, db + ts * (size_t)k0 * bks, bcs, bks, N, kk, np, 0, trim, l.Np, Kp,
             (int*)(base + l.eb), (signed char*)(base + l.planes_b), hs);
         }
       }

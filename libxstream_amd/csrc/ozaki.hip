@@ -92,9 +92,15 @@ __device__ __forceinline__ void oz_load16(double* v, const T* __restrict__ p, lo
   int K) {
   const T* const q = p + (long)row * rs + (long)k * ks;
   if (row < R && k + 16 <= K) {
-    if (1 == ks) {
+    if (1 == ks) { /* 16-byte loads at the element's alignment */
+      constexpr int V = 16 / (int)sizeof(T);
+      typedef T vec_t __attribute__((ext_vector_type(V), aligned(sizeof(T))));
 #pragma unroll
-      for (int i = 0; i < 16; ++i) v[i] = oz_keep<T>(q[i]);
+      for (int i = 0; i < 16; i += V) {
+        const vec_t x = *(const __attribute__((address_space(1))) vec_t*)(q + i);
+#pragma unroll
+        for (int j = 0; j < V; ++j) v[i + j] = oz_keep<T>(x[j]);
+      }
     }
     else {
 #pragma unroll
@@ -535,6 +541,44 @@ constexpr oz_crt_tab_t oz_crt_make() {
 }
 __constant__ oz_crt_tab_t oz_crt = oz_crt_make();
 
+/*
+ * Residue tables: byte i (i = 0..7) of kb[w][p] is 2^(8 i) w_p mod m_p, where w_p is 1 (w = 0: operand B) or
+ * Garner's weight of the reconstruction for np = w primes, ((M / m_p) mod m_p)^-1 mod m_p (operand A: its
+ * residues carry the weight, so the int8 products modulo m_p are already the CRT coefficients c_p). lo is
+ * the least symmetric residue, (m + 1) / 2 - m; r31 = ceil(2^31 / m) (m > 128).
+ */
+struct oz_res_tab_t {
+  unsigned int kb[LXS_OZ_NPRIMES_MAX + 1][LXS_OZ_NPRIMES_MAX][2];
+  int lo[LXS_OZ_NPRIMES_MAX];
+  unsigned int r31[LXS_OZ_NPRIMES_MAX];
+};
+constexpr oz_res_tab_t oz_res_make() {
+  constexpr int mod[LXS_OZ_NPRIMES_MAX] = {LXS_OZ_MODULI};
+  oz_res_tab_t t{};
+  for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
+    const unsigned long long m = (unsigned long long)mod[p];
+    t.lo[p] = (mod[p] + 1) / 2 - mod[p];
+    t.r31[p] = (unsigned int)(((1ull << 31) + m - 1) / m);
+  }
+  for (int np = 0; np <= LXS_OZ_NPRIMES_MAX; ++np) {
+    unsigned __int128 mm = 1;
+    for (int p = 0; p < np; ++p) mm *= (unsigned __int128)mod[p];
+    for (int p = 0; p < LXS_OZ_NPRIMES_MAX; ++p) {
+      const unsigned long long m = (unsigned long long)mod[p];
+      const unsigned long long w = (0 < np && p < np)
+                                     ? (unsigned long long)oz_inv_mod((int)((mm / m) % (unsigned __int128)m), (int)m)
+                                     : 1ull;
+      unsigned long long pw = 1; /* 2^(8 i) mod m */
+      for (int i = 0; i < 8; ++i) {
+        t.kb[np][p][i >> 2] |= (unsigned int)((pw * w) % m) << (8 * (i & 3));
+        pw = (pw * 256) % m;
+      }
+    }
+  }
+  return t;
+}
+__constant__ oz_res_tab_t oz_res = oz_res_make();
+
 __device__ __forceinline__ unsigned int oz_mod18(unsigned int x, int p) { /* x mod m_p for x < 2^18 */
   return x - __umul24(__umulhi(x, oz_crt.rcp[p]), oz_crt.m[p]);
 }
@@ -546,21 +590,28 @@ __device__ __forceinline__ unsigned int oz_mod27(unsigned int x, int p) {
 /*
  * Residue planes of one 64 x 64 tile: every element v of row r with the row's exponent E (|v| < 2^E)
  * becomes the integer a = trunc(|v| 2^(mb - t - E)) (mb = 53 for FP64, 24 for FP32; t trimmed bits);
- * plane p holds the symmetric residue of sign(v) a mod m_p, in [-m/2, m/2), as int8.
- * A thread owns 16 consecutive k of one row and writes one 16-byte vector per plane.
+ * plane p holds the symmetric residue of sign(v) a w_p mod m_p, in [lo_p, lo_p + m_p), as int8 (w_p: 1, or
+ * the reconstruction weight for weight = np > 0, see oz_res_tab_t). A thread owns 16 consecutive k of one
+ * row and writes one 16-byte vector per plane.
+ * Per plane and element: x = sum of the 7 bytes of a times 2^(8 i) w mod m (two byte dot products,
+ * x < 7 * 255^2 < 2^19, x = a w mod m), y = x - lo or m 2^12 - x - lo by the sign (so y = sign a w - lo
+ * mod m and 0 <= y < 2^21; the sign enters through an all-ones mask, -x = (x ^ ~0) + 1), y mod m by one 24-bit Barrett step (floor(y / m) =
+ * (2 y ceil(2^31 / m)) >> 32: the excess below 2^-10 cannot reach the next integer, y / m being a multiple
+ * of 1 / m >= 2^-8), and the residue is lo + (y mod m).
  */
 template <typename T>
 __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long rs, long ks, int R, int K,
-  const int* __restrict__ e, int np, int mb, int trim, int Rp, int Kp, int8_t* __restrict__ planes) {
+  const int* __restrict__ e, int np, int weight, int mb, int trim, int Rp, int Kp, int8_t* __restrict__ planes) {
   const int r0 = blockIdx.x * OZ_TILE, k0 = blockIdx.y * OZ_TILE;
-  int r, kq;
-  oz_owner(rs, r, kq);
+  /* four lanes per row (k chunks of 16), 16 rows per wavefront, for either operand layout: each lane's 16
+     residue bytes of a plane continue its neighbour's, so a wavefront stores 1 KB contiguous per plane */
+  const int r = (int)threadIdx.x >> 2, kq = ((int)threadIdx.x & 3) * 16;
   const int row = r0 + r;
   const int ex = (row < R ? e[row] : OZ_EXP_NONE);
   double t[16];
   oz_load16<T>(t, p, rs, ks, row, k0 + kq, R, K);
-  unsigned int c0[16], c1[16], c2[16]; /* a = c2 2^36 + c1 2^18 + c0 */
-  unsigned int neg = 0;
+  unsigned int alo[16], ahi[16]; /* a = ahi 2^32 + alo, a < 2^53 */
+  unsigned int nm[16];           /* all ones for negative v, else 0 */
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     const double v = t[i];
@@ -569,26 +620,29 @@ __global__ void __launch_bounds__(256) oz_residues(const T* __restrict__ p, long
     const unsigned long long mant = (bits & 0xFFFFFFFFFFFFFull) | (1ull << 52);
     const int shift = (53 - mb) + trim + (ex - ev); /* a = mant >> shift */
     const unsigned long long a = (OZ_EXP_NONE != ev && OZ_EXP_NONE != ex && shift < 53) ? (mant >> shift) : 0ull;
-    c0[i] = (unsigned int)(a & 0x3FFFF);
-    c1[i] = (unsigned int)((a >> 18) & 0x3FFFF);
-    c2[i] = (unsigned int)(a >> 36);
-    neg |= (unsigned int)(bits >> 63) << i;
+    alo[i] = (unsigned int)a;
+    ahi[i] = (unsigned int)(a >> 32);
+    nm[i] = 0u - (unsigned int)(bits >> 63);
   }
+  const int w = (0 != weight ? np : 0);
 #pragma unroll
   for (int pl = 0; pl < LXS_OZ_NPRIMES_MAX; ++pl) {
     if (pl < np) {
-      const int m = (int)oz_crt.m[pl];
+      const unsigned int m = oz_crt.m[pl], klo = oz_res.kb[w][pl][0], khi = oz_res.kb[w][pl][1], r31 = oz_res.r31[pl];
+      const int lo = oz_res.lo[pl];
+      const unsigned int cbig = (m << 12) + 1u, clo = (unsigned int)-lo; /* -x = (x ^ ~0) + 1 */
       union {
         int8_t b[16];
         i4_t v;
       } d;
 #pragma unroll
       for (int i = 0; i < 16; ++i) {
-        /* a = c2 2^36 + c1 2^18 + c0 = c2 k36 + c1 k18 + c0 (mod m), which is below 2^26.6 */
-        const unsigned int x = __umul24(c2[i], oz_crt.k36[pl]) + __umul24(c1[i], oz_crt.k18[pl]) + c0[i];
-        int s = (int)oz_mod27(x, pl);
-        if (0 != ((neg >> i) & 1)) s = (0 != s ? m - s : 0);
-        d.b[i] = (int8_t)(s >= (m + 1) / 2 ? s - m : s);
+        const unsigned int x = __builtin_amdgcn_udot4(alo[i], klo, __builtin_amdgcn_udot4(ahi[i], khi, 0u, false), false);
+        const unsigned int y = (x ^ nm[i]) + (nm[i] & cbig) + clo; /* x - lo, or m 2^12 - x - lo */
+        /* both factors below 2^24 (masks state it for the compiler: v_mul_hi_u32_u24) */
+        const unsigned int q = (unsigned int)(((unsigned long long)((2u * y) & 0xFFFFFFu) * (r31 & 0xFFFFFFu)) >> 32);
+        /* lo + y mod m = lo + y - q m, of which only the low byte is kept: lo + y + q (256 - m) */
+        d.b[i] = (int8_t)(__umul24(q, 256u - m) + y + (unsigned int)lo);
       }
       *(i4_t*)(planes + oz_plane_off(pl, row, k0 + kq, Rp, Kp)) = d.v;
     }
@@ -700,7 +754,7 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
         for (int h = 0; h < 2; ++h) {
           const int p = p2 + h;
           if (p < np) {
-            const unsigned int mp = oz_crt.m[p], wp = oz_rec.w[np][p];
+            const unsigned int mp = oz_crt.m[p];
             const float rmp = oz_crt.rmf[p];
             /* r_p = sum of the items' bytes (each < m_p) mod m_p: min(t, t - m) is t mod m for t < 2 m */
             int z = lv.zb[p];
@@ -717,10 +771,8 @@ __global__ void __launch_bounds__(256) oz_crt_recombine(const uint8_t* __restric
             }
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-              /* cp = s w mod m_p, s w < 2^16: floor(s w / m) from FP32 is exact but at multiples of m (r = m) */
-              const unsigned int sw_ = __umul24(sv[e], wp);
-              const unsigned int r = (unsigned int)__mul24((int)((float)sw_ * rmp), -(int)mp) + sw_;
-              const unsigned int cp = min(r, r - mp);
+              /* the A residues carry w_p (oz_res_tab_t), so r_p is already c_p = r w_p mod m_p */
+              const unsigned int cp = sv[e];
               f[e] = fmaf((float)cp, rmp, f[e]); /* sum_p c_p / m_p < np: absolute error below 2^-14 */
               cpair[e] |= cp << (16 * h);
             }
@@ -1023,8 +1075,8 @@ extern "C" int lxs_oz_level_gemm(int crt, const signed char* pa, const signed ch
   return (int)hipGetLastError();
 }
 
-extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int trim, int Rp,
-  int Kp, int* e, signed char* planes, void* hip_stream) {
+extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks, int R, int K, int np, int weight,
+  int trim, int Rp, int Kp, int* e, signed char* planes, void* hip_stream) {
   const hipStream_t st = (hipStream_t)hip_stream;
   const dim3 egrid = oz_exp_grid(ks, Rp, Kp), grid((unsigned int)(Rp / OZ_TILE), (unsigned int)(Kp / OZ_TILE));
   const int mb = (3 == dtype ? 53 : 24);
@@ -1033,11 +1085,13 @@ extern "C" int lxs_oz_crt_preprocess(int dtype, const void* p, long rs, long ks,
   oz_init<<<(Rp + 255) / 256, 256, 0, st>>>(e, Rp, NULL);
   if (3 == dtype) {
     oz_exp<double><<<egrid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e);
-    oz_residues<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e, np, mb, trim, Rp, Kp, (int8_t*)planes);
+    oz_residues<double><<<grid, 256, 0, st>>>((const double*)p, rs, ks, R, K, e, np, weight, mb, trim, Rp, Kp,
+      (int8_t*)planes);
   }
   else if (1 == dtype) {
     oz_exp<float><<<egrid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e);
-    oz_residues<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e, np, mb, trim, Rp, Kp, (int8_t*)planes);
+    oz_residues<float><<<grid, 256, 0, st>>>((const float*)p, rs, ks, R, K, e, np, weight, mb, trim, Rp, Kp,
+      (int8_t*)planes);
   }
   else {
     return LXS_ENOVARIANT;
