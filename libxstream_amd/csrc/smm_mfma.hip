@@ -547,6 +547,7 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
 #define LXS_MG(TI, TJ, NWI, KB) {TI, TJ, NWI, KB, 0, 0, 0, 1, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, 0, 0, 0>},
 #define LXS_MS(M, N, K, TI, TJ, NWI, KB) {TI, TJ, NWI, KB, M, N, K, 1, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, M, N, K>},
 #define LXS_ME(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, KB, M, N, K, EP, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, M, N, K>},
+#define LXS_MEG(TI, TJ, EP, KB) {TI, TJ, 1, KB, 0, 0, 0, EP, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, 0, 0, 0>},
 
 const mfma_entry_t mfma_table[] = {
   /* runtime shape (m <= 16*ti, n <= 16*tj), 4 k-steps (16 of k) per stage: every tile grid up to
@@ -569,6 +570,17 @@ const mfma_entry_t mfma_table[] = {
   LXS_ME(13, 13, 13, 1, 1, 4, 4) LXS_ME(13, 13, 13, 1, 1, 8, 4) LXS_ME(32, 32, 32, 2, 2, 2, 8)
   LXS_ME(32, 32, 32, 2, 2, 4, 8) LXS_ME(64, 64, 64, 4, 4, 2, 4) LXS_ME(45, 45, 13, 3, 3, 2, 4)
   LXS_ME(5, 5, 5, 1, 1, 4, 2) LXS_ME(4, 9, 22, 1, 1, 4, 6)
+  /* entry-parallel, runtime shape (m <= 16*ti, n <= 16*tj), 4 k-steps (16 of k) per stage: 2 wavefronts per run
+     up to 16 tiles (the FP64 partial blocks fill 64 KB of LDS there), 4 up to 8 tiles, 8 up to 4 tiles */
+  LXS_MEG(1, 1, 2, 4) LXS_MEG(1, 2, 2, 4) LXS_MEG(1, 3, 2, 4) LXS_MEG(1, 4, 2, 4) LXS_MEG(1, 5, 2, 4)
+  LXS_MEG(2, 1, 2, 4) LXS_MEG(2, 2, 2, 4) LXS_MEG(2, 3, 2, 4) LXS_MEG(2, 4, 2, 4) LXS_MEG(2, 5, 2, 4)
+  LXS_MEG(3, 1, 2, 4) LXS_MEG(3, 2, 2, 4) LXS_MEG(3, 3, 2, 4) LXS_MEG(3, 4, 2, 4) LXS_MEG(3, 5, 2, 4)
+  LXS_MEG(4, 1, 2, 4) LXS_MEG(4, 2, 2, 4) LXS_MEG(4, 3, 2, 4) LXS_MEG(4, 4, 2, 4) LXS_MEG(5, 1, 2, 4)
+  LXS_MEG(5, 2, 2, 4) LXS_MEG(5, 3, 2, 4) LXS_MEG(1, 1, 4, 4) LXS_MEG(1, 2, 4, 4) LXS_MEG(1, 3, 4, 4)
+  LXS_MEG(1, 4, 4, 4) LXS_MEG(1, 5, 4, 4) LXS_MEG(2, 1, 4, 4) LXS_MEG(2, 2, 4, 4) LXS_MEG(2, 3, 4, 4)
+  LXS_MEG(2, 4, 4, 4) LXS_MEG(3, 1, 4, 4) LXS_MEG(3, 2, 4, 4) LXS_MEG(4, 1, 4, 4) LXS_MEG(4, 2, 4, 4)
+  LXS_MEG(5, 1, 4, 4) LXS_MEG(1, 1, 8, 4) LXS_MEG(1, 2, 8, 4) LXS_MEG(1, 3, 8, 4) LXS_MEG(1, 4, 8, 4)
+  LXS_MEG(2, 1, 8, 4) LXS_MEG(2, 2, 8, 4) LXS_MEG(3, 1, 8, 4) LXS_MEG(4, 1, 8, 4)
 };
 
 } /* namespace */

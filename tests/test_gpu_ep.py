@@ -107,3 +107,38 @@ def test_ep_bit_reproducible(stream, shape):
                 ref = out
                 assert oracle_error(w, out, stack=stack)[0] <= 1e-13
             assert np.array_equal(ref.view(np.uint64), out.view(np.uint64)), (cfg, bs, flush)
+
+
+GENERIC_SHAPES = [(17, 29, 11), (31, 23, 40), (6, 21, 16), (40, 33, 9)]
+
+
+@pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", GENERIC_SHAPES, ids=["%dx%dx%d" % s for s in GENERIC_SHAPES])
+def test_ep_runtime_shape_parity(stream, shape, dtype):
+    """The runtime-shape entry-parallel kernels (no kernel compiled for the shape itself): every one that fits
+    the shape, on sorted stacks with ragged runs, long runs cut into pieces and unsorted stacks"""
+    m, n, k = shape
+    tid = lx.typeid(dtype)
+    cfgs = [c for c in _ep_configs(tid, m, n, k)]
+    assert cfgs, shape
+    w = make_stack_workload(m, n, k, 2500, dtype=dtype, seed=sum(shape) * 11 + 1, c_init="uniform", spread=12)
+    long_runs = [400, 3, 70, 1, 129, 65] + [17] * 20 + [600]
+    wl = make_stack_workload(m, n, k, sum(long_runs), nc=len(long_runs), dtype=dtype, seed=sum(shape) * 11 + 2,
+                             c_init="uniform")
+    stack_long = _with_runs(wl, long_runs)
+    stack_unsorted = np.ascontiguousarray(w.stack[np.random.default_rng(sum(shape)).permutation(w.stack_size)])
+    ref_sorted = oracle.stack_dd(m, n, k, w.stack, w.a, w.b, c_in=w.c.ravel())
+    ref_long = oracle.stack_dd(m, n, k, stack_long, wl.a, wl.b, c_in=wl.c.ravel())
+    ref_unsorted = oracle.stack_dd(m, n, k, stack_unsorted, w.a, w.b, c_in=w.c.ravel())
+    for cfg in cfgs:
+        for bs in (1, 32, 100):
+            cfg.bs, cfg.s = bs, 0
+            for flush in (1, 2):
+                cfg.flush = flush
+                out = run_stack(stream, w, config=cfg, device_transpose=False)
+                assert oracle.error(out.ravel(), *ref_sorted)[0] <= tol(dtype), (cfg, "sorted")
+                out = run_stack(stream, wl, config=cfg, device_transpose=False, stack=stack_long)
+                assert oracle.error(out.ravel(), *ref_long)[0] <= tol(dtype), (cfg, "long runs")
+            cfg.flush = 2
+            out = run_stack(stream, w, config=cfg, device_transpose=False, stack=stack_unsorted)
+            assert oracle.error(out.ravel(), *ref_unsorted)[0] <= tol(dtype), (cfg, "unsorted")
