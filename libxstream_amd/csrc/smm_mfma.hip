@@ -41,8 +41,9 @@ namespace {
 
 typedef LXS_REAL real_t;
 struct mfma_entry_t {
-  int ti, tj, nwi, kb, m, n, k; /* m = n = k = 0: runtime shape with m <= 16*ti, n <= 16*tj */
+  int ti, tj, nwi, kb, m, n, k; /* m = n = k = 0: runtime shape with m <= 16*ti, n <= 16*tj (b4: 8*ti, 8*tj) */
   int ep;                       /* 1: smm_mfma; > 1: smm_mfma_ep with ep wavefronts per C-run */
+  int b4;                       /* 1: smm_mfma_eb (4x4 blocks, ti x tj instructions per k-step) */
   const void* fn;
 };
 
@@ -263,14 +264,25 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
  * (every wavefront computes it, so all take the same path through the barriers).
  * Barriers wait for LDS only (a fence restricted to the local address space), so fragment loads of the
  * next run stay in flight across them.
+ *
+ * B4 (FP64, smm_mfma_eb): the same run protocol on 4x4 blocks instead of 16x16 tiles. v_mfma_f64_4x4x4_4b
+ * multiplies four independent 4x4x4 blocks at the FP64 matrix-core rate of the 16x16x4 form (a quarter of
+ * its cycles), so a C block is padded to multiples of 8 instead of 16: 23 x 23 takes 24 x 24 (54 MFMA per
+ * entry of 23 in k, 16 cycles each) instead of 32 x 32 (24 MFMA of 64 cycles), 1.78x less matrix-core time.
+ * Lane l feeds k = l / 16 of block g = (l / 4) % 4, row l % 4 of its A block and column l % 4 of its B block;
+ * the result of block g for A row i and B column j is in lane 16 i + 4 g + j (checked on gfx950 with a
+ * probe). Block g covers the quadrant (g % 2, g / 2) of the padded 8 TI x 8 TJ C block, and instruction
+ * (r, c) of a k-step takes its block rows r and columns c with the rows interleaved: block row i of
+ * instruction r is C row 4 TI (g % 2) + TI i + r (columns likewise), so each lane's TI rows of A (TJ columns
+ * of B) are consecutive in memory and load as pairs.
  */
-template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
-__global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T* __restrict__ cdata, const T* __restrict__ adata,
+template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC, bool B4>
+__device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
   int kt_unused, int atomic, int ps, int pz, int tsplit) {
   typedef typename mfma<T>::acc_t acc_t;
   constexpr int SEG_LEN = 16; /* entries per matrix-core accumulation before the fold into the run total */
-  constexpr int NTILE = TI * TJ;
+  constexpr int NTILE = TI * TJ; /* 16x16 tiles, or (B4) 4x4-block instructions per k-step */
   const int M = (0 < MC ? MC : m_rt), N = (0 < NC ? NC : n_rt), K = (0 < KC ? KC : k_rt);
   const int KS = (K + 3) >> 2;       /* k-steps of 4 */
   const int NS = (KS + KB - 1) / KB; /* k-stages per entry */
@@ -278,7 +290,7 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
   extern __shared__ double lxs_smem[];
-  T* const red = reinterpret_cast<T*>(lxs_smem); /* [EP][NTILE][4][64] partial blocks */
+  T* const red = reinterpret_cast<T*>(lxs_smem); /* [EP][NTILE][4][64] partial blocks ([EP][NTILE][64] for B4) */
   (void)kt_unused;
 
   const int begin = blockIdx.x * bs;
@@ -378,7 +390,7 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
 
   /* PAIR (even tile counts): paired tiles (lxs::pair_start), one two-element load per lane feeds two tiles,
      which halves the fragment load instructions and the cache lines they touch */
-  constexpr bool PAIR = (0 == TI % 2) && (0 == TJ % 2);
+  constexpr bool PAIR = !B4 && (0 == TI % 2) && (0 == TJ % 2);
   constexpr int PI = (PAIR ? TI / 2 : 1), PJ = (PAIR ? TJ / 2 : 1);
   typedef typename lxs::pair_of<T>::type v2_t;
   int aoff[TI], boff[TJ];
@@ -387,17 +399,34 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
 #pragma unroll
   for (int tj = 0; tj < TJ; ++tj) boff[tj] = (PAIR ? lxs::pair_start(tj >> 1, li, N) : min(16 * tj + li, N - 1));
   const T* const zeros = lxs::zeros<T>();
+  /* B4: this lane's first A row r4 and first B column c4 (TI rows, TJ columns from there); element pairs start
+     at min(row, M - 2) so they stay inside the block, and a pair whose first row is the last row of the block
+     (a4x, b4x) holds that row in its second element (the first is the row before; read at use, so that no
+     select waits for a load in flight). Rows and columns past the block only feed padding outputs. */
+  constexpr int TI2 = TI / 2, TJ2 = TJ / 2;
+  const int g4 = (lane >> 2) & 3;
+  const int r4 = 4 * TI * (g4 & 1) + TI * (lane & 3), c4 = 4 * TJ * (g4 >> 1) + TJ * (lane & 3);
+  bool a4x[TI2 > 0 ? TI2 : 1], b4x[TJ2 > 0 ? TJ2 : 1];
+#pragma unroll
+  for (int p = 0; p < TI2; ++p) a4x[p] = (r4 + 2 * p == M - 1);
+#pragma unroll
+  for (int p = 0; p < TJ2; ++p) b4x[p] = (c4 + 2 * p == N - 1);
+  v2_t qa[KB][TI2 > 0 ? TI2 : 1], qb[KB][TJ2 > 0 ? TJ2 : 1]; /* B4 element pairs */
   /* the matrix core accumulates up to SEG_LEN entries in prod; each segment is then added to the run total,
      which lives in this wavefront's partial block in LDS (red, slot q): total = 0 + prod first, total + prod
      after, the same additions a register total would make, without its registers */
   acc_t prod[TI][TJ];
+  T prod4[TI][TJ]; /* B4 */
   T fa[KB][TI], fb[KB][TJ];
   v2_t pa[KB][PI], pb[KB][PJ];
   const acc_t zero = {T(0), T(0), T(0), T(0)};
 #pragma unroll
   for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
-    for (int tj = 0; tj < TJ; ++tj) prod[ti][tj] = zero;
+    for (int tj = 0; tj < TJ; ++tj) {
+      prod[ti][tj] = zero;
+      prod4[ti][tj] = T(0);
+    }
   }
   bool folded = false; /* the run total in LDS holds a segment */
   const auto fold = [&]() {
@@ -405,12 +434,19 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
     for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
       for (int tj = 0; tj < TJ; ++tj) {
-#pragma unroll
-        for (int x = 0; x < 4; ++x) {
-          T* const slot = red + ((q * NTILE + ti * TJ + tj) * 4 + x) * 64 + lane;
-          *slot = (folded ? *slot : T(0)) + prod[ti][tj][x];
+        if constexpr (B4) {
+          T* const slot = red + (q * NTILE + ti * TJ + tj) * 64 + lane;
+          *slot = (folded ? *slot : T(0)) + prod4[ti][tj];
+          prod4[ti][tj] = T(0);
         }
-        prod[ti][tj] = zero;
+        else {
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            T* const slot = red + ((q * NTILE + ti * TJ + tj) * 4 + x) * 64 + lane;
+            *slot = (folded ? *slot : T(0)) + prod[ti][tj][x];
+          }
+          prod[ti][tj] = zero;
+        }
       }
     }
     folded = true;
@@ -420,7 +456,17 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
   do { \
     const int k_ = 4 * (KSTEP) + lk; \
     const bool kv_ = (k_ < K); \
-    if constexpr (PAIR) { \
+    if constexpr (B4) { \
+      _Pragma("unroll") for (int p_ = 0; p_ < TI2; ++p_) { \
+        qa[KBI][p_] = lxs::gload2(kv_ ? (adata + ((A0) + k_ * M + min(r4 + 2 * p_, M - 2))) : zeros); \
+      } \
+      if constexpr (0 != TI % 2) fa[KBI][0] = gload(kv_ ? (adata + ((A0) + k_ * M + min(r4 + TI - 1, M - 1))) : zeros); \
+      _Pragma("unroll") for (int p_ = 0; p_ < TJ2; ++p_) { \
+        qb[KBI][p_] = lxs::gload2(kv_ ? (bdata + ((B0) + k_ * N + min(c4 + 2 * p_, N - 2))) : zeros); \
+      } \
+      if constexpr (0 != TJ % 2) fb[KBI][0] = gload(kv_ ? (bdata + ((B0) + k_ * N + min(c4 + TJ - 1, N - 1))) : zeros); \
+    } \
+    else if constexpr (PAIR) { \
       _Pragma("unroll") for (int p_ = 0; p_ < PI; ++p_) { \
         pa[KBI][p_] = lxs::gload2(kv_ ? (adata + ((A0) + k_ * M + aoff[2 * p_])) : zeros); \
       } \
@@ -470,7 +516,31 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
 #pragma unroll
           for (int kb = 0; kb < KB; ++kb) {
             const int ks = s * KB + kb;
-            if (ks < KS) {
+            if constexpr (B4) {
+              if (ks < KS) {
+                T av[TI], bv[TJ];
+#pragma unroll
+                for (int p = 0; p < TI2; ++p) {
+                  av[2 * p] = (a4x[p] ? qa[kb][p][1] : qa[kb][p][0]);
+                  av[2 * p + 1] = qa[kb][p][1];
+                }
+                if constexpr (0 != TI % 2) av[TI - 1] = fa[kb][0];
+#pragma unroll
+                for (int p = 0; p < TJ2; ++p) {
+                  bv[2 * p] = (b4x[p] ? qb[kb][p][1] : qb[kb][p][0]);
+                  bv[2 * p + 1] = qb[kb][p][1];
+                }
+                if constexpr (0 != TJ % 2) bv[TJ - 1] = fb[kb][0];
+#pragma unroll
+                for (int ti = 0; ti < TI; ++ti) {
+#pragma unroll
+                  for (int tj = 0; tj < TJ; ++tj) {
+                    prod4[ti][tj] = __builtin_amdgcn_mfma_f64_4x4x4f64(av[ti], bv[tj], prod4[ti][tj], 0, 0, 0);
+                  }
+                }
+              }
+            }
+            else if (ks < KS) {
 #pragma unroll
               for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
@@ -508,7 +578,27 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    {
+    if constexpr (B4) { /* instruction t = (r, c): this lane's block row lane / 16, column lane % 4 */
+      T* const cblk = cdata + (c0 - pz);
+      const int rb = 4 * TI * (g4 & 1) + TI * (lane >> 4), cb = 4 * TJ * (g4 >> 1) + TJ * (lane & 3);
+      for (int t = q; t < NTILE; t += EP) {
+        const int r = t / TJ, c = t - r * TJ;
+        T v = red[t * 64 + lane];
+#pragma unroll
+        for (int p = 1; p < EP; ++p) v += red[(p * NTILE + t) * 64 + lane];
+        const int i = rb + r, j = cb + c;
+        if (i < M && j < N) {
+          T* const ptr = cblk + j * M + i;
+          if (flush_atomic) {
+            unsafeAtomicAdd(ptr, v);
+          }
+          else {
+            *ptr += v;
+          }
+        }
+      }
+    }
+    else {
       T* const cblk = cdata + (c0 - pz);
       for (int t = q; t < NTILE; t += EP) {
         const int ti = t / TJ, tj = t - ti * TJ;
@@ -544,10 +634,28 @@ __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(T
 #undef LXS_LOAD_KSTEP
 }
 
-#define LXS_MG(TI, TJ, NWI, KB) {TI, TJ, NWI, KB, 0, 0, 0, 1, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, 0, 0, 0>},
-#define LXS_MS(M, N, K, TI, TJ, NWI, KB) {TI, TJ, NWI, KB, M, N, K, 1, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, M, N, K>},
-#define LXS_ME(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, KB, M, N, K, EP, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, M, N, K>},
-#define LXS_MEG(TI, TJ, EP, KB) {TI, TJ, 1, KB, 0, 0, 0, EP, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, 0, 0, 0>},
+#define LXS_EP_ARGS \
+  T* __restrict__ cdata, const T* __restrict__ adata, const T* __restrict__ bdata, const int* __restrict__ stack, \
+    int stack_size, int bs, int m_rt, int n_rt, int k_rt, int kt_unused, int atomic, int ps, int pz, int tsplit
+#define LXS_EP_PASS cdata, adata, bdata, stack, stack_size, bs, m_rt, n_rt, k_rt, kt_unused, atomic, ps, pz, tsplit
+
+template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
+__global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(LXS_EP_ARGS) {
+  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, false>(LXS_EP_PASS);
+}
+/* FP64 on 4x4 blocks: C blocks up to 8 TI x 8 TJ */
+template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
+__global__ void __launch_bounds__(64 * EP, 4) smm_mfma_eb(LXS_EP_ARGS) {
+  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, true>(LXS_EP_PASS);
+}
+
+#define LXS_MG(TI, TJ, NWI, KB) {TI, TJ, NWI, KB, 0, 0, 0, 1, 0, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, 0, 0, 0>},
+#define LXS_MS(M, N, K, TI, TJ, NWI, KB) {TI, TJ, NWI, KB, M, N, K, 1, 0, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, M, N, K>},
+#define LXS_ME(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, KB, M, N, K, EP, 0, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, M, N, K>},
+#define LXS_MEG(TI, TJ, EP, KB) {TI, TJ, 1, KB, 0, 0, 0, EP, 0, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, 0, 0, 0>},
+#define LXS_EB(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, KB, M, N, K, EP, 1, (const void*)&smm_mfma_eb<real_t, TI, TJ, EP, KB, M, N, K>},
+#define LXS_EBG(TI, TJ, EP, KB) {TI, TJ, 1, KB, 0, 0, 0, EP, 1, (const void*)&smm_mfma_eb<real_t, TI, TJ, EP, KB, 0, 0, 0>},
+#define LXS_IS_f64 1 /* the 4x4-block kernels are FP64 only */
 
 const mfma_entry_t mfma_table[] = {
   /* runtime shape (m <= 16*ti, n <= 16*tj), 4 k-steps (16 of k) per stage: every tile grid up to
@@ -581,6 +689,21 @@ const mfma_entry_t mfma_table[] = {
   LXS_MEG(2, 4, 4, 4) LXS_MEG(3, 1, 4, 4) LXS_MEG(3, 2, 4, 4) LXS_MEG(4, 1, 4, 4) LXS_MEG(4, 2, 4, 4)
   LXS_MEG(5, 1, 4, 4) LXS_MEG(1, 1, 8, 4) LXS_MEG(1, 2, 8, 4) LXS_MEG(1, 3, 8, 4) LXS_MEG(1, 4, 8, 4)
   LXS_MEG(2, 1, 8, 4) LXS_MEG(2, 2, 8, 4) LXS_MEG(3, 1, 8, 4) LXS_MEG(4, 1, 8, 4)
+#if LXS_CAT(LXS_IS_, LXS_TAG)
+  /* entry-parallel on 4x4 blocks (FP64): the stack workloads whose blocks pad less to multiples of 8 than to 16 */
+  LXS_EB(23, 23, 23, 3, 3, 4, 3) LXS_EB(23, 23, 23, 3, 3, 8, 3) LXS_EB(23, 23, 23, 3, 3, 4, 6)
+  LXS_EB(23, 23, 23, 3, 3, 8, 6) LXS_EB(5, 5, 5, 1, 1, 4, 2) LXS_EB(5, 5, 5, 1, 1, 8, 2)
+  LXS_EB(13, 13, 13, 2, 2, 4, 4) LXS_EB(13, 13, 13, 2, 2, 8, 4) LXS_EB(4, 9, 22, 1, 2, 4, 6)
+  LXS_EB(4, 9, 22, 1, 2, 8, 6)
+  /* runtime shape (m <= 8*ti, n <= 8*tj), 4 k-steps per stage (2 for 16 blocks), 4 or 8 wavefronts per run */
+  LXS_EBG(1, 1, 4, 4) LXS_EBG(1, 2, 4, 4) LXS_EBG(1, 3, 4, 4) LXS_EBG(1, 4, 4, 4) LXS_EBG(2, 1, 4, 4)
+  LXS_EBG(2, 2, 4, 4) LXS_EBG(2, 3, 4, 4) LXS_EBG(2, 4, 4, 4) LXS_EBG(3, 1, 4, 4) LXS_EBG(3, 2, 4, 4)
+  LXS_EBG(3, 3, 4, 4) LXS_EBG(3, 4, 4, 4) LXS_EBG(4, 1, 4, 4) LXS_EBG(4, 2, 4, 4) LXS_EBG(4, 3, 4, 4)
+  LXS_EBG(4, 4, 4, 2) LXS_EBG(1, 1, 8, 4) LXS_EBG(1, 2, 8, 4) LXS_EBG(1, 3, 8, 4) LXS_EBG(1, 4, 8, 4)
+  LXS_EBG(2, 1, 8, 4) LXS_EBG(2, 2, 8, 4) LXS_EBG(2, 3, 8, 4) LXS_EBG(2, 4, 8, 4) LXS_EBG(3, 1, 8, 4)
+  LXS_EBG(3, 2, 8, 4) LXS_EBG(3, 3, 8, 4) LXS_EBG(3, 4, 8, 4) LXS_EBG(4, 1, 8, 4) LXS_EBG(4, 2, 8, 4)
+  LXS_EBG(4, 3, 8, 4) LXS_EBG(4, 4, 8, 2)
+#endif
 };
 
 } /* namespace */

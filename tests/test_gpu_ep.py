@@ -1,7 +1,8 @@
-"""GPU parity of the entry-parallel matrix-core stack kernel (variant 2: several wavefronts share each
-C-run, their partial blocks added in a fixed order through LDS) through the C ABI, against the oracle.
+"""GPU parity of the entry-parallel matrix-core stack kernels (variant 2: several wavefronts share each
+C-run, their partial blocks added in a fixed order through LDS; variant 3: the same on 4x4 blocks, FP64) through
+the C ABI, against the oracle.
 
-Covered: every compiled variant-2 kernel on its shape, FP64 and FP32, sorted stacks with ragged runs,
+Covered: every compiled variant-2 and variant-3 kernel on its shape, FP64 and FP32, sorted stacks with ragged runs,
 long runs cut into pieces (atomic flush of the pieces), stacks whose runs are interleaved (atomic flush),
 the read-modify-write flush on sorted stacks, and bit-reproducibility: runs that are not cut give the same
 bits on every call, for every bs, and under both flush modes.
@@ -31,10 +32,11 @@ def stream():
 
 
 def _ep_configs(tid, m, n, k):
+    """every entry-parallel kernel for the shape: 16x16 tiles (variant 2) and 4x4 blocks (variant 3, FP64)"""
     out = []
     for variant, tm, tn, nt, kt in lx.kernel_candidates(tid, m, n, k):
-        if variant == 2:
-            out.append(lx.SmmConfig(variant=2, tm=tm, tn=tn, nt=nt, kt=kt, bs=32, flush=2))
+        if variant in (2, 3):
+            out.append(lx.SmmConfig(variant=variant, tm=tm, tn=tn, nt=nt, kt=kt, bs=32, flush=2))
     return out
 
 
@@ -109,7 +111,22 @@ def test_ep_bit_reproducible(stream, shape):
             assert np.array_equal(ref.view(np.uint64), out.view(np.uint64)), (cfg, bs, flush)
 
 
-GENERIC_SHAPES = [(17, 29, 11), (31, 23, 40), (6, 21, 16), (40, 33, 9)]
+# (19, 22, 6), (9, 25, 7), (31, 23, 40) and (17, 29, 11) put the last row or column of a block at the start of a
+# 4x4-block kernel's element pair (the pair is shifted back by one element)
+GENERIC_SHAPES = [(17, 29, 11), (31, 23, 40), (6, 21, 16), (40, 33, 9), (19, 22, 6), (9, 25, 7)]
+
+
+def test_4x4_block_kernels_cover_their_shapes():
+    """variant 3 (FP64 only) serves every shape up to 32 x 32 with exactly the blocks it needs, and no FP32 shape"""
+    lx.init()
+    try:
+        for m, n in ((5, 5), (23, 23), (13, 13), (4, 9), (9, 25), (32, 32), (1, 1), (2, 17)):
+            v3 = [c for c in lx.kernel_candidates(lx.REAL_8, m, n, 7) if c[0] == 3]
+            assert v3 and all((c[1], c[2]) == ((m + 7) // 8, (n + 7) // 8) for c in v3), (m, n, v3)
+            assert not [c for c in lx.kernel_candidates(lx.REAL_4, m, n, 7) if c[0] == 3]
+        assert not [c for c in lx.kernel_candidates(lx.REAL_8, 33, 8, 7) if c[0] == 3]
+    finally:
+        lx.finalize()
 
 
 @pytest.mark.parametrize("dtype", DTYPES, ids=["f64", "f32"])
