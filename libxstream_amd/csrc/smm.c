@@ -132,20 +132,42 @@ static int lxs_smm_fit_kt(int datatype, int m, int n, int k, libsmm_acc_smm_conf
 }
 
 /**
- * Built-in choice when nothing is tuned. The matrix-core kernel covers the C block with 16x16 tiles;
- * when at least 30% of the covered area is useful (m*n >= 0.3 * 256 * ti * tj), it takes a kernel
- * compiled for exactly this shape, else the runtime-shape kernel for exactly that tile grid.
- * Smaller blocks go to the vector kernel: 256 threads, the square microtile (up to 5x5) that keeps
- * the most threads busy with at most 16 groups of threads sharing a C-run's entries (runs are tens
- * of entries long), ties to the smaller tile; 64 threads for blocks too small for that.
- * Stack entries per work-group: 32 (matrix cores) or 64 (vector kernel), from measurements on
- * MI355X over the shapes 5^3..64^3.
+ * Built-in choice when nothing is tuned. C blocks of up to 16 tiles of 16x16 (m, n <= 64) go to the
+ * entry-parallel matrix-core kernel (variant 2) for exactly that tile grid, with the most wavefronts per
+ * run compiled for it (8 up to 4 tiles, 4 up to 8, 2 up to 16; a kernel compiled for the shape first) and
+ * 32 (FP64) or 16 (FP32) stack entries per work-group: over the 419 such shapes of the autotuner's
+ * 500-triplet sweeps this reaches 0.94 (FP64) and 0.92 (FP32) of each shape's best on average.
+ * Larger blocks: the matrix-core kernel with one wavefront per run when at least 30% of the covered
+ * area is useful (m*n >= 0.3 * 256 * ti * tj), a kernel compiled for exactly this shape first, else the
+ * runtime-shape kernel for exactly that tile grid. Otherwise the vector kernel: 256 threads, the square
+ * microtile (up to 5x5) that keeps the most threads busy with at most 16 groups of threads sharing a
+ * C-run's entries (runs are tens of entries long), ties to the smaller tile; 64 threads for blocks too
+ * small for that. Stack entries per work-group there: 32 (matrix cores) or 64 (vector kernel).
  */
 static int lxs_smm_default(int datatype, int m, int n, int k, libsmm_acc_smm_config_t* config) {
   const int ti = (m + 15) / 16, tj = (n + 15) / 16;
   int q[5], i, t, count, cand[5 * 256], best_t = 0, best_nt = 256;
   double best_util = 0;
   memset(config, 0, sizeof(*config));
+  if (16 >= ti * tj) {
+    const int nt = 64 * (4 >= ti * tj ? 8 : (8 >= ti * tj ? 4 : 2));
+    count = lxs_smm_candidates(datatype, m, n, k, cand, 256); /* kernels compiled for the shape come first */
+    for (i = 0; i < count && i < 256; ++i) {
+      const int* const c = cand + 5 * i;
+      if (2 == c[0] && ti == c[1] && tj == c[2] && nt == c[3]) {
+        config->variant = 2;
+        config->tm = ti;
+        config->tn = tj;
+        config->nt = nt;
+        config->kt = c[4];
+        config->bs = (dbcsr_type_real_8 == datatype ? 32 : 16);
+        config->s = LXS_SMM_S;
+        if (EXIT_SUCCESS == libsmm_acc_config_valid((libsmm_acc_data_t)datatype, m, n, k, config)) return EXIT_SUCCESS;
+        memset(config, 0, sizeof(*config));
+        break;
+      }
+    }
+  }
   if ((double)m * n >= 0.3 * 256.0 * ti * tj) {
     int found = (0 <= lxs_smm_specialised(datatype, 1, m, n, k, q));
     count = lxs_smm_variants(datatype, cand, 256);
