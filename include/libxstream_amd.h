@@ -23,17 +23,19 @@
  *                partial blocks summed in LDS in a fixed order before the one flush of the run
  *            3 = variant 2 on 4x4 blocks (MFMA 4x4x4, four blocks per instruction; FP64 only): C blocks
  *                padded to multiples of 8 instead of 16
+ *            4 = variant 3 with the four blocks of an instruction splitting k (16 of k per instruction, one
+ *                4x4 block of C): C blocks up to 16x16 padded to multiples of 4
  *   tm, tn:  variant 0: C microtile (rows x columns) owned by one thread;
  *            variants 1, 2: number of 16x16 output tiles along m and n (padded cover of the C block);
- *            variant 3: m <= 8 tm and n <= 8 tn, the smallest such tm, tn
+ *            variant 3: m <= 8 tm and n <= 8 tn, the smallest such tm, tn; variant 4: the same with 4
  *   nt:      threads per work-group; variant 0: 64, 128, 256 or 512, the threads left after
  *            covering one C block with microtiles form further groups that share the run's
  *            stack entries; variant 1: 64 per wavefront, the wavefronts split the tile rows;
- *            variants 2, 3: 64 per wavefront, the wavefronts split the run's entries
+ *            variants 2-4: 64 per wavefront, the wavefronts split the run's entries
  *   bs:      nominal stack entries per work-group; a work-group owns the C-runs whose first entry
  *            falls in its range and finishes them, so each C block is written by one work-group
  *   kt:      variant 0: k-slice staged in LDS at once (0 = all of k when it fits);
- *            variants 1-3: k per register stage (a multiple of 4 fixed by the kernel; 0 = any)
+ *            variants 1-4: k per register stage (a multiple of 4 fixed by the kernel; 0 = any)
  *   flush:   0 = automatic, 1 = plain read-modify-write (stack grouped by C), 2 = atomic add.
  *            Configurations reached through libsmm_acc_process (tuned CSV, learned predictor, built-in
  *            default, environment) and the batch always flush with atomic adds, which is correct for

@@ -17,7 +17,7 @@ struct smm_entry_t {
 struct mfma_entry_t {
   int ti, tj, nwi, kb, m, n, k;
   int ep; /* 1: one wavefront per run (variant 1); > 1: ep wavefronts share each run (variant 2) */
-  int b4; /* 1: entry-parallel on 4x4 blocks (variant 3, FP64) */
+  int b4; /* 1: entry-parallel on 4x4 blocks (variant 3, FP64); 2: the same splitting k (variant 4) */
   const void* fn;
 };
 
@@ -87,22 +87,26 @@ static const mfma_entry_t* mfma_table(int dtype, int* count) {
 
 /* threads of an MFMA kernel: nwi wavefronts splitting the tiles, or ep wavefronts sharing each run */
 static int mfma_threads(const mfma_entry_t* e) { return 64 * (1 < e->ep ? e->ep : e->nwi); }
-static int mfma_variant(const mfma_entry_t* e) { return (0 != e->b4 ? 3 : (1 < e->ep ? 2 : 1)); }
+static int mfma_variant(const mfma_entry_t* e) { return (0 != e->b4 ? 2 + e->b4 : (1 < e->ep ? 2 : 1)); }
 /* LDS bytes of an MFMA kernel: the partial blocks of the ep wavefronts (16 x 16 tiles, or one register per
    4x4-block instruction and lane) */
 static size_t mfma_lds(const mfma_entry_t* e, int dtype) {
-  return (1 < e->ep ? (size_t)e->ep * e->ti * e->tj * (0 != e->b4 ? 64 : 256) * (3 == dtype ? 8 : 4) : 0);
+  return (1 < e->ep ? (size_t)e->ep * e->ti * e->tj * (2 == e->b4 ? 16 : (1 == e->b4 ? 64 : 256)) * (3 == dtype ? 8 : 4) : 0);
 }
 /* a runtime-shape kernel covers (m, n): 16x16 tiles up to 16 ti x 16 tj, with at least two rows and columns
    where the kernel loads element pairs (an even number of tiles in both dimensions); 4x4 blocks with exactly
-   the ti = ceil(m / 8), tj = ceil(n / 8) the shape needs (each lane loads ti consecutive rows, tj columns) */
+   the ti = ceil(m / 8), tj = ceil(n / 8) the shape needs (each lane loads ti consecutive rows, tj columns), or
+   ti = ceil(m / 4), tj = ceil(n / 4) when the blocks split k */
 static int mfma_fits(const mfma_entry_t* e, int m, int n) {
-  if (0 != e->b4) return m <= 8 * e->ti && m > 8 * (e->ti - 1) && n <= 8 * e->tj && n > 8 * (e->tj - 1);
+  if (0 != e->b4) {
+    const int r = (2 == e->b4 ? 4 : 8);
+    return m <= r * e->ti && m > r * (e->ti - 1) && n <= r * e->tj && n > r * (e->tj - 1);
+  }
   if (0 == e->ti % 2 && 0 == e->tj % 2 && (2 > m || 2 > n)) return 0;
   return m <= 16 * e->ti && n <= 16 * e->tj;
 }
 
-/* MFMA kernel for (dtype, variant 1, 2 or 3, tiles tm x tn, nt threads, kt = k per stage or 0): exact shape first */
+/* MFMA kernel for (dtype, variant 1 to 4, tiles tm x tn, nt threads, kt = k per stage or 0): exact shape first */
 static const mfma_entry_t* mfma_find(int dtype, int variant, int m, int n, int k, int tm, int tn, int nt, int kt) {
   int count = 0, i;
   const mfma_entry_t* const table = mfma_table(dtype, &count);
@@ -160,7 +164,7 @@ extern "C" int lxs_smm_launchable(const lxs_smm_args_t* args) {
   size_t lds;
   int kt;
   if (1 == args->variant) return 1;
-  if (2 == args->variant || 3 == args->variant) {
+  if (2 <= args->variant && args->variant <= 4) {
     const mfma_entry_t* const me = mfma_find(args->dtype, args->variant, args->m, args->n, args->k, args->tm, args->tn,
       args->nt, args->kt);
     return (NULL != me && mfma_lds(me, args->dtype) <= lds_limit) ? 1 : 0;
@@ -177,7 +181,7 @@ extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
   const smm_entry_t* e;
   int kt;
   if (1 == args->variant) return 0;
-  if (2 == args->variant || 3 == args->variant) {
+  if (2 <= args->variant && args->variant <= 4) {
     const mfma_entry_t* const me = mfma_find(args->dtype, args->variant, args->m, args->n, args->k, args->tm, args->tn,
       args->nt, args->kt);
     return (NULL != me ? mfma_lds(me, args->dtype) : 0);
@@ -189,7 +193,7 @@ extern "C" size_t lxs_smm_lds_bytes(const lxs_smm_args_t* args) {
 }
 
 extern "C" int lxs_smm_available(int dtype, int variant, int m, int n, int k, int tm, int tn, int nt, int kt) {
-  if (1 <= variant && variant <= 3) {
+  if (1 <= variant && variant <= 4) {
     const mfma_entry_t* const e = mfma_find(dtype, variant, m, n, k, tm, tn, nt, kt);
     return (NULL == e ? 0 : (0 != e->m ? 2 : 1));
   }
@@ -284,7 +288,7 @@ extern "C" int lxs_smm_candidates(int dtype, int m, int n, int k, int* q, int ca
 
 extern "C" int lxs_smm_specialised(int dtype, int variant, int m, int n, int k, int* q) {
   int count = 0, i;
-  if (1 <= variant && variant <= 3) {
+  if (1 <= variant && variant <= 4) {
     const mfma_entry_t* const table = mfma_table(dtype, &count);
     for (i = 0; i < count; ++i) {
       if (table[i].m == m && table[i].n == n && table[i].k == k && mfma_variant(table + i) == variant) {
@@ -322,7 +326,7 @@ extern "C" int lxs_smm_launch(const lxs_smm_args_t* a, void* hip_stream) {
   const void *pa, *pb;
   const int* stack;
   void* args[14];
-  if (1 <= a->variant && a->variant <= 3) {
+  if (1 <= a->variant && a->variant <= 4) {
     const mfma_entry_t* const e = mfma_find(a->dtype, a->variant, a->m, a->n, a->k, a->tm, a->tn, a->nt, a->kt);
     if (NULL == e) return LXS_ENOVARIANT;
     fn = e->fn;

@@ -96,7 +96,7 @@ LIBXSTREAM_API int libsmm_acc_config_valid(libsmm_acc_data_t datatype, int m, in
   const libsmm_acc_smm_config_t* config) {
   lxs_smm_args_t args;
   if (NULL == config || 0 >= m || 0 >= n || 0 >= k || 0 == lxs_typesize(datatype)) return EXIT_FAILURE;
-  if (0 > config->variant || 3 < config->variant || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt ||
+  if (0 > config->variant || 4 < config->variant || 0 >= config->tm || 0 >= config->tn || 0 >= config->nt ||
       0 > config->bs || 0 > config->kt || 0 > config->split || 0 > config->bs_short || 0 > config->split_short) {
     return EXIT_FAILURE;
   }

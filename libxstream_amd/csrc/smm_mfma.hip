@@ -43,7 +43,7 @@ typedef LXS_REAL real_t;
 struct mfma_entry_t {
   int ti, tj, nwi, kb, m, n, k; /* m = n = k = 0: runtime shape with m <= 16*ti, n <= 16*tj (b4: 8*ti, 8*tj) */
   int ep;                       /* 1: smm_mfma; > 1: smm_mfma_ep with ep wavefronts per C-run */
-  int b4;                       /* 1: smm_mfma_eb (4x4 blocks, ti x tj instructions per k-step) */
+  int b4;                       /* 1: smm_mfma_eb (4x4 blocks, ti x tj instructions per k-step), 2: smm_mfma_ek */
   const void* fn;
 };
 
@@ -275,8 +275,13 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
  * (r, c) of a k-step takes its block rows r and columns c with the rows interleaved: block row i of
  * instruction r is C row 4 TI (g % 2) + TI i + r (columns likewise), so each lane's TI rows of A (TJ columns
  * of B) are consecutive in memory and load as pairs.
+ * B4 = 2 (smm_mfma_ek): the four blocks of an instruction are one 4x4 block of C over four quarters of a 16-wide
+ * k-step (block g takes k = 16 s + 4 g + l / 16), so a C block up to 4 TI x 4 TJ takes TI x TJ instructions per
+ * 16 of k and every lane loads distinct elements (C row TI i + r of instruction r, i = l % 4). The quarters are
+ * added (lanes l, l ^ 4, l ^ 8, l ^ 12, in a fixed order) where a segment is folded into the run total, which
+ * keeps one element per lane of lanes 16 i + j.
  */
-template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC, bool B4>
+template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC, int B4>
 __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restrict__ adata,
   const T* __restrict__ bdata, const int* __restrict__ stack, int stack_size, int bs, int m_rt, int n_rt, int k_rt,
   int kt_unused, int atomic, int ps, int pz, int tsplit) {
@@ -284,13 +289,14 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
   constexpr int SEG_LEN = 16; /* entries per matrix-core accumulation before the fold into the run total */
   constexpr int NTILE = TI * TJ; /* 16x16 tiles, or (B4) 4x4-block instructions per k-step */
   const int M = (0 < MC ? MC : m_rt), N = (0 < NC ? NC : n_rt), K = (0 < KC ? KC : k_rt);
-  const int KS = (K + 3) >> 2;       /* k-steps of 4 */
+  const int KS = (2 == B4 ? (K + 15) >> 4 : (K + 3) >> 2); /* k-steps of 4 (B4 = 2: of 16, 4 per block) */
   const int NS = (KS + KB - 1) / KB; /* k-stages per entry */
   const int lane = threadIdx.x & 63;
   const int q = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int li = lane & 15, lk = lane >> 4;
   extern __shared__ double lxs_smem[];
-  T* const red = reinterpret_cast<T*>(lxs_smem); /* [EP][NTILE][4][64] partial blocks ([EP][NTILE][64] for B4) */
+  /* partial blocks: [EP][NTILE][4][64] (16x16 tiles), [EP][NTILE][64] (B4 = 1), [EP][NTILE][16] (B4 = 2) */
+  T* const red = reinterpret_cast<T*>(lxs_smem);
   (void)kt_unused;
 
   const int begin = blockIdx.x * bs;
@@ -301,7 +307,7 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
      readlanes in wave-uniform code) instead of chains of dependent loads; entries outside the window are
      loaded. Kernels of 16 tiles or more keep loading: their accumulators leave no registers for the window
      (a spilled window register reloaded under a partial EXEC mask would feed the readlanes stale lanes). */
-  constexpr bool WIN = (NTILE < 16);
+  constexpr bool WIN = (NTILE < 16) || (2 == B4 && NTILE <= 16); /* B4 = 2: one register pair per instruction */
   constexpr int NONE = INT_MIN; /* no C block: differs from every offset */
   const int wi = begin + lane, pi = begin - 64 + lane;
   const bool win = (wi < stack_size);
@@ -390,7 +396,7 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
 
   /* PAIR (even tile counts): paired tiles (lxs::pair_start), one two-element load per lane feeds two tiles,
      which halves the fragment load instructions and the cache lines they touch */
-  constexpr bool PAIR = !B4 && (0 == TI % 2) && (0 == TJ % 2);
+  constexpr bool PAIR = (0 == B4) && (0 == TI % 2) && (0 == TJ % 2);
   constexpr int PI = (PAIR ? TI / 2 : 1), PJ = (PAIR ? TJ / 2 : 1);
   typedef typename lxs::pair_of<T>::type v2_t;
   int aoff[TI], boff[TJ];
@@ -405,7 +411,8 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
      select waits for a load in flight). Rows and columns past the block only feed padding outputs. */
   constexpr int TI2 = TI / 2, TJ2 = TJ / 2;
   const int g4 = (lane >> 2) & 3;
-  const int r4 = 4 * TI * (g4 & 1) + TI * (lane & 3), c4 = 4 * TJ * (g4 >> 1) + TJ * (lane & 3);
+  const int r4 = (2 == B4 ? 0 : 4 * TI * (g4 & 1)) + TI * (lane & 3);
+  const int c4 = (2 == B4 ? 0 : 4 * TJ * (g4 >> 1)) + TJ * (lane & 3);
   bool a4x[TI2 > 0 ? TI2 : 1], b4x[TJ2 > 0 ? TJ2 : 1];
 #pragma unroll
   for (int p = 0; p < TI2; ++p) a4x[p] = (r4 + 2 * p == M - 1);
@@ -434,7 +441,16 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
     for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
       for (int tj = 0; tj < TJ; ++tj) {
-        if constexpr (B4) {
+        if constexpr (2 == B4) { /* the four k-quarters of the block, added in a fixed order, then lanes 16 i + j */
+          const T v01 = prod4[ti][tj] + __shfl_xor(prod4[ti][tj], 4);
+          const T v = v01 + __shfl_xor(v01, 8);
+          if (0 == g4) {
+            T* const slot = red + (q * NTILE + ti * TJ + tj) * 16 + ((lane >> 4) << 2) + (lane & 3);
+            *slot = (folded ? *slot : T(0)) + v;
+          }
+          prod4[ti][tj] = T(0);
+        }
+        else if constexpr (1 == B4) {
           T* const slot = red + (q * NTILE + ti * TJ + tj) * 64 + lane;
           *slot = (folded ? *slot : T(0)) + prod4[ti][tj];
           prod4[ti][tj] = T(0);
@@ -454,9 +470,9 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
 
 #define LXS_LOAD_KSTEP(KBI, A0, B0, KSTEP) \
   do { \
-    const int k_ = 4 * (KSTEP) + lk; \
+    const int k_ = (2 == B4 ? 16 * (KSTEP) + 4 * g4 : 4 * (KSTEP)) + lk; \
     const bool kv_ = (k_ < K); \
-    if constexpr (B4) { \
+    if constexpr (0 != B4) { \
       _Pragma("unroll") for (int p_ = 0; p_ < TI2; ++p_) { \
         qa[KBI][p_] = lxs::gload2(kv_ ? (adata + ((A0) + k_ * M + min(r4 + 2 * p_, M - 2))) : zeros); \
       } \
@@ -516,7 +532,7 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
 #pragma unroll
           for (int kb = 0; kb < KB; ++kb) {
             const int ks = s * KB + kb;
-            if constexpr (B4) {
+            if constexpr (0 != B4) {
               if (ks < KS) {
                 T av[TI], bv[TJ];
 #pragma unroll
@@ -578,7 +594,30 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
     __builtin_amdgcn_s_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-    if constexpr (B4) { /* instruction t = (r, c): this lane's block row lane / 16, column lane % 4 */
+    if constexpr (2 == B4) { /* four instructions t at a time, one per quarter-wavefront: row i, column j */
+      T* const cblk = cdata + (c0 - pz);
+      const int x = lane & 15, rb = TI * (x >> 2), cb = TJ * (x & 3);
+      for (int t0 = 4 * q; t0 < NTILE; t0 += 4 * EP) {
+        const int t = t0 + (lane >> 4);
+        if (t < NTILE) {
+          const int r = t / TJ, c = t - r * TJ;
+          T v = red[t * 16 + x];
+#pragma unroll
+          for (int p = 1; p < EP; ++p) v += red[(p * NTILE + t) * 16 + x];
+          const int i = rb + r, j = cb + c;
+          if (i < M && j < N) {
+            T* const ptr = cblk + j * M + i;
+            if (flush_atomic) {
+              unsafeAtomicAdd(ptr, v);
+            }
+            else {
+              *ptr += v;
+            }
+          }
+        }
+      }
+    }
+    else if constexpr (1 == B4) { /* instruction t = (r, c): this lane's block row lane / 16, column lane % 4 */
       T* const cblk = cdata + (c0 - pz);
       const int rb = 4 * TI * (g4 & 1) + TI * (lane >> 4), cb = 4 * TJ * (g4 >> 1) + TJ * (lane & 3);
       for (int t = q; t < NTILE; t += EP) {
@@ -641,12 +680,17 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
 
 template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
 __global__ void __launch_bounds__(64 * EP, (TI * TJ <= 4 ? 4 : 1)) smm_mfma_ep(LXS_EP_ARGS) {
-  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, false>(LXS_EP_PASS);
+  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, 0>(LXS_EP_PASS);
 }
 /* FP64 on 4x4 blocks: C blocks up to 8 TI x 8 TJ */
 template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
 __global__ void __launch_bounds__(64 * EP, 4) smm_mfma_eb(LXS_EP_ARGS) {
-  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, true>(LXS_EP_PASS);
+  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, 1>(LXS_EP_PASS);
+}
+/* FP64 on 4x4 blocks that split k: C blocks up to 4 TI x 4 TJ, KB k-steps of 16 per stage */
+template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC>
+__global__ void __launch_bounds__(64 * EP, 4) smm_mfma_ek(LXS_EP_ARGS) {
+  ep_body<T, TI, TJ, EP, KB, MC, NC, KC, 2>(LXS_EP_PASS);
 }
 
 #define LXS_MG(TI, TJ, NWI, KB) {TI, TJ, NWI, KB, 0, 0, 0, 1, 0, (const void*)&smm_mfma<real_t, TI, TJ, NWI, KB, 0, 0, 0>},
@@ -655,6 +699,9 @@ __global__ void __launch_bounds__(64 * EP, 4) smm_mfma_eb(LXS_EP_ARGS) {
 #define LXS_MEG(TI, TJ, EP, KB) {TI, TJ, 1, KB, 0, 0, 0, EP, 0, (const void*)&smm_mfma_ep<real_t, TI, TJ, EP, KB, 0, 0, 0>},
 #define LXS_EB(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, KB, M, N, K, EP, 1, (const void*)&smm_mfma_eb<real_t, TI, TJ, EP, KB, M, N, K>},
 #define LXS_EBG(TI, TJ, EP, KB) {TI, TJ, 1, KB, 0, 0, 0, EP, 1, (const void*)&smm_mfma_eb<real_t, TI, TJ, EP, KB, 0, 0, 0>},
+/* kb of a k-split kernel: 4 KB (its k per stage is 16 KB) */
+#define LXS_EK(M, N, K, TI, TJ, EP, KB) {TI, TJ, 1, 4 * KB, M, N, K, EP, 2, (const void*)&smm_mfma_ek<real_t, TI, TJ, EP, KB, M, N, K>},
+#define LXS_EKG(TI, TJ, EP, KB) {TI, TJ, 1, 4 * KB, 0, 0, 0, EP, 2, (const void*)&smm_mfma_ek<real_t, TI, TJ, EP, KB, 0, 0, 0>},
 #define LXS_IS_f64 1 /* the 4x4-block kernels are FP64 only */
 
 const mfma_entry_t mfma_table[] = {
@@ -695,6 +742,17 @@ const mfma_entry_t mfma_table[] = {
   LXS_EB(23, 23, 23, 3, 3, 8, 6) LXS_EB(5, 5, 5, 1, 1, 4, 2) LXS_EB(5, 5, 5, 1, 1, 8, 2)
   LXS_EB(13, 13, 13, 2, 2, 4, 4) LXS_EB(13, 13, 13, 2, 2, 8, 4) LXS_EB(4, 9, 22, 1, 2, 4, 6)
   LXS_EB(4, 9, 22, 1, 2, 8, 6)
+  /* entry-parallel on 4x4 blocks that split k in four (FP64): small blocks, one 16-k step (or two) per stage */
+  LXS_EK(5, 5, 5, 2, 2, 4, 1) LXS_EK(5, 5, 5, 2, 2, 8, 1) LXS_EK(13, 13, 13, 4, 4, 4, 1) LXS_EK(13, 13, 13, 4, 4, 8, 1)
+  LXS_EK(4, 9, 22, 1, 3, 4, 2) LXS_EK(4, 9, 22, 1, 3, 8, 2)
+  /* runtime shape (m <= 4*ti, n <= 4*tj, up to 16 x 16) */
+  LXS_EKG(1, 1, 4, 1) LXS_EKG(1, 2, 4, 1) LXS_EKG(1, 3, 4, 1) LXS_EKG(1, 4, 4, 1) LXS_EKG(2, 1, 4, 1) LXS_EKG(2, 2, 4, 1)
+  LXS_EKG(2, 3, 4, 1) LXS_EKG(2, 4, 4, 1) LXS_EKG(3, 1, 4, 1) LXS_EKG(3, 2, 4, 1) LXS_EKG(3, 3, 4, 1) LXS_EKG(3, 4, 4, 1)
+  LXS_EKG(4, 1, 4, 1) LXS_EKG(4, 2, 4, 1) LXS_EKG(4, 3, 4, 1) LXS_EKG(4, 4, 4, 1) LXS_EKG(1, 1, 8, 1) LXS_EKG(1, 2, 8, 1)
+  LXS_EKG(1, 3, 8, 1) LXS_EKG(1, 4, 8, 1) LXS_EKG(2, 1, 8, 1) LXS_EKG(2, 2, 8, 1) LXS_EKG(2, 3, 8, 1) LXS_EKG(2, 4, 8, 1)
+  LXS_EKG(3, 1, 8, 1) LXS_EKG(3, 2, 8, 1) LXS_EKG(3, 3, 8, 1) LXS_EKG(3, 4, 8, 1) LXS_EKG(4, 1, 8, 1) LXS_EKG(4, 2, 8, 1)
+  LXS_EKG(4, 3, 8, 1) LXS_EKG(4, 4, 8, 1) LXS_EKG(1, 1, 4, 2) LXS_EKG(1, 2, 4, 2) LXS_EKG(1, 3, 4, 2) LXS_EKG(1, 4, 4, 2)
+  LXS_EKG(2, 1, 4, 2) LXS_EKG(2, 2, 4, 2) LXS_EKG(3, 1, 4, 2) LXS_EKG(4, 1, 4, 2)
   /* runtime shape (m <= 8*ti, n <= 8*tj), 4 k-steps per stage (2 for 16 blocks), 4 or 8 wavefronts per run */
   LXS_EBG(1, 1, 4, 4) LXS_EBG(1, 2, 4, 4) LXS_EBG(1, 3, 4, 4) LXS_EBG(1, 4, 4, 4) LXS_EBG(2, 1, 4, 4)
   LXS_EBG(2, 2, 4, 4) LXS_EBG(2, 3, 4, 4) LXS_EBG(2, 4, 4, 4) LXS_EBG(3, 1, 4, 4) LXS_EBG(3, 2, 4, 4)

@@ -32,10 +32,11 @@ def stream():
 
 
 def _ep_configs(tid, m, n, k):
-    """every entry-parallel kernel for the shape: 16x16 tiles (variant 2) and 4x4 blocks (variant 3, FP64)"""
+    """every entry-parallel kernel for the shape: 16x16 tiles (variant 2) and 4x4 blocks (variants 3 and 4, FP64;
+    variant 4 splits each 16 of k over the four blocks of an instruction)"""
     out = []
     for variant, tm, tn, nt, kt in lx.kernel_candidates(tid, m, n, k):
-        if variant in (2, 3):
+        if variant in (2, 3, 4):
             out.append(lx.SmmConfig(variant=variant, tm=tm, tn=tn, nt=nt, kt=kt, bs=32, flush=2))
     return out
 
@@ -113,11 +114,13 @@ def test_ep_bit_reproducible(stream, shape):
 
 # (19, 22, 6), (9, 25, 7), (31, 23, 40) and (17, 29, 11) put the last row or column of a block at the start of a
 # 4x4-block kernel's element pair (the pair is shifted back by one element)
-GENERIC_SHAPES = [(17, 29, 11), (31, 23, 40), (6, 21, 16), (40, 33, 9), (19, 22, 6), (9, 25, 7)]
+GENERIC_SHAPES = [(17, 29, 11), (31, 23, 40), (6, 21, 16), (40, 33, 9), (19, 22, 6), (9, 25, 7), (7, 10, 30), (14, 3, 9),
+                  (16, 16, 5)]
 
 
 def test_4x4_block_kernels_cover_their_shapes():
-    """variant 3 (FP64 only) serves every shape up to 32 x 32 with exactly the blocks it needs, and no FP32 shape"""
+    """variant 3 (FP64 only) serves every shape up to 32 x 32 with exactly the blocks it needs, and no FP32 shape;
+    variant 4 every shape up to 16 x 16"""
     lx.init()
     try:
         for m, n in ((5, 5), (23, 23), (13, 13), (4, 9), (9, 25), (32, 32), (1, 1), (2, 17)):
@@ -125,6 +128,11 @@ def test_4x4_block_kernels_cover_their_shapes():
             assert v3 and all((c[1], c[2]) == ((m + 7) // 8, (n + 7) // 8) for c in v3), (m, n, v3)
             assert not [c for c in lx.kernel_candidates(lx.REAL_4, m, n, 7) if c[0] == 3]
         assert not [c for c in lx.kernel_candidates(lx.REAL_8, 33, 8, 7) if c[0] == 3]
+        # variant 4 (4x4 blocks splitting k): exactly ti = ceil(m / 4), tj = ceil(n / 4), up to 16 x 16
+        for m, n in ((5, 5), (7, 10), (14, 3), (16, 16), (1, 1)):
+            v4 = [c for c in lx.kernel_candidates(lx.REAL_8, m, n, 7) if c[0] == 4]
+            assert v4 and all((c[1], c[2]) == ((m + 3) // 4, (n + 3) // 4) for c in v4), (m, n, v4)
+        assert not [c for c in lx.kernel_candidates(lx.REAL_8, 17, 8, 7) if c[0] == 4]
     finally:
         lx.finalize()
 
