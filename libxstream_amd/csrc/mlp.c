@@ -23,15 +23,15 @@
 #define MLP_MAXNETS 16
 
 /* class values of the encoded parameters (one-hot) */
-static const int mlp_variant[] = {0, 1, 2};
+static const int mlp_variant[] = {0, 1, 2, 3, 4};
 static const int mlp_tm[] = {1, 2, 3, 4, 5};
 static const int mlp_tn[] = {1, 2, 3, 4, 5};
 static const int mlp_nt[] = {64, 128, 192, 256, 320, 512};
-static const int mlp_kt[] = {0, 8, 12, 16, 24};
+static const int mlp_kt[] = {0, 8, 12, 16, 24, 32};
 static const int mlp_bs[] = {16, 32, 64, 128};
 static const int mlp_flush[] = {1, 2};
 static const int* const mlp_vals[MLP_NPARAM] = {mlp_variant, mlp_tm, mlp_tn, mlp_nt, mlp_kt, mlp_bs, mlp_flush};
-static const int mlp_nvals[MLP_NPARAM] = {3, 5, 5, 6, 5, 4, 2};
+static const int mlp_nvals[MLP_NPARAM] = {5, 5, 5, 6, 6, 4, 2};
 
 struct libsmm_acc_mlp_t {
   int nnets;                 /* ensemble members, identical shape */

@@ -1,7 +1,7 @@
 """GPU checks of the learned predictor (libsmm_acc_mlp_*, HIP kernels in mlp.hip).
 
 - Gradients of the HIP backward pass match the double-precision oracle (oracle/mlp_oracle.c, pinned
-  by finite differences), for the predictor's own layout (38 inputs, 3 x 64 ReLU, 2 outputs).
+  by finite differences), for the predictor's own layout (41 inputs, 3 x 64 ReLU, 2 outputs).
 - Full-batch Adam on the GPU reduces the loss of a learnable synthetic task.
 - Predicted configurations are valid for their shapes; an installed model serves shapes without a
   tuned configuration, and a threshold it cannot reach leaves the default in place.
@@ -53,7 +53,7 @@ def test_gradient_matches_oracle(dev):
     model = lx.Mlp(1, 64, 3, seed=7)
     try:
         info = model.info()
-        assert info["width"] == [38, 64, 64, 64, 2]
+        assert info["width"] == [41, 64, 64, 64, 2]
         x = _inputs(rng, 96)
         t = np.stack([rng.uniform(0.3, 1.0, 96), rng.uniform(8, 15, 96)], axis=1).astype(np.float32)
         w = np.array(learned.OUT_WEIGHTS, dtype=np.float32)
