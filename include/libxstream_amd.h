@@ -23,8 +23,8 @@
  *                partial blocks summed in LDS in a fixed order before the one flush of the run
  *            3 = variant 2 on 4x4 blocks (MFMA 4x4x4, four blocks per instruction; FP64 only): C blocks
  *                padded to multiples of 8 instead of 16
- *            4 = variant 3 with the four blocks of an instruction splitting k (16 of k per instruction, one
- *                4x4 block of C): C blocks up to 16x16 padded to multiples of 4
+ *            4 = 4x4 blocks whose instruction splits k (FP64: four blocks of 4 k, FP32: sixteen blocks of one k,
+ *                so 16 of k per instruction for one 4x4 block of C): C blocks up to 16x16 padded to multiples of 4
  *   tm, tn:  variant 0: C microtile (rows x columns) owned by one thread;
  *            variants 1, 2: number of 16x16 output tiles along m and n (padded cover of the C block);
  *            variant 3: m <= 8 tm and n <= 8 tn, the smallest such tm, tn; variant 4: the same with 4

@@ -25,7 +25,7 @@ typedef struct lxs_smm_args_t {
   int m, n, k;
   int dtype;
   int variant;       /* 0: vector-FMA kernel (LDS staging), 1: MFMA kernel (registers only), 2: entry-parallel
-                        MFMA kernel, 3: entry-parallel on 4x4 blocks (FP64), 4: the same splitting k */
+                        MFMA kernel, 3: entry-parallel on 4x4 blocks (FP64), 4: 4x4 blocks splitting k */
   int tm, tn, nt, bs, kt;
   int atomic;        /* 1: flush C-runs with atomic adds; 0: plain read-modify-write */
   int pfmt;          /* 0: 3-int records, 1-based offsets; else 6-int DBM task records (m, n, k, a, b, c),

@@ -34,6 +34,33 @@ struct mfma<float> {
   static __device__ __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
 };
 
+/* 4x4-block MFMA (checked on gfx950 with probes). FP64 v_mfma_f64_4x4x4_4b: four blocks of 4x4x4; lane l feeds
+   k = l / 16 of block (l / 4) % 4, A row l % 4 and B column l % 4, and holds the block's result for A row l / 16
+   and B column l % 4 (one register). FP32 v_mfma_f32_4x4x1_16b: sixteen blocks of 4x4x1; lane l feeds block
+   l / 4 (its one k), A row l % 4 and B column l % 4, and holds the block's column l % 4 in its four registers
+   (register r: row r). klane: the k within 16 that lane l feeds when the blocks of one instruction take
+   consecutive k of the same C block. */
+template <typename T>
+struct mfma4;
+template <>
+struct mfma4<double> {
+  typedef double acc_t;
+  static constexpr bool FP32 = false;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int klane(int lane) { return 4 * ((lane >> 2) & 3) + (lane >> 4); }
+};
+template <>
+struct mfma4<float> {
+  typedef f4_t acc_t;
+  static constexpr bool FP32 = true;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int klane(int lane) { return lane >> 2; }
+};
+
 /* zero operands for fragment lanes past k (read-only) */
 __device__ const double zeros_f64[2] = {0.0, 0.0};
 __device__ const float zeros_f32[4] = {0.0f, 0.0f, 0.0f, 0.0f};

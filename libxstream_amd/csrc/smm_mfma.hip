@@ -423,7 +423,8 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
      which lives in this wavefront's partial block in LDS (red, slot q): total = 0 + prod first, total + prod
      after, the same additions a register total would make, without its registers */
   acc_t prod[TI][TJ];
-  T prod4[TI][TJ]; /* B4 */
+  typedef typename lxs::mfma4<T>::acc_t acc4_t;
+  acc4_t prod4[TI][TJ]; /* B4 */
   T fa[KB][TI], fb[KB][TJ];
   v2_t pa[KB][PI], pb[KB][PJ];
   const acc_t zero = {T(0), T(0), T(0), T(0)};
@@ -432,7 +433,7 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
 #pragma unroll
     for (int tj = 0; tj < TJ; ++tj) {
       prod[ti][tj] = zero;
-      prod4[ti][tj] = T(0);
+      prod4[ti][tj] = acc4_t{};
     }
   }
   bool folded = false; /* the run total in LDS holds a segment */
@@ -441,19 +442,37 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
     for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
       for (int tj = 0; tj < TJ; ++tj) {
-        if constexpr (2 == B4) { /* the four k-quarters of the block, added in a fixed order, then lanes 16 i + j */
-          const T v01 = prod4[ti][tj] + __shfl_xor(prod4[ti][tj], 4);
-          const T v = v01 + __shfl_xor(v01, 8);
-          if (0 == g4) {
-            T* const slot = red + (q * NTILE + ti * TJ + tj) * 16 + ((lane >> 4) << 2) + (lane & 3);
-            *slot = (folded ? *slot : T(0)) + v;
+        if constexpr (2 == B4) {
+          if constexpr (lxs::mfma4<T>::FP32) {
+            /* sixteen k-blocks: lanes of one column j = l % 4 hold row i in register i, added over l ^ 4 .. l ^ 32 in
+               a fixed order; lanes 0..3 keep element 4 i + j */
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+              T v = prod4[ti][tj][x];
+              v += __shfl_xor(v, 4);
+              v += __shfl_xor(v, 8);
+              v += __shfl_xor(v, 16);
+              v += __shfl_xor(v, 32);
+              if (lane < 4) {
+                T* const slot = red + (q * NTILE + ti * TJ + tj) * 16 + 4 * x + lane;
+                *slot = (folded ? *slot : T(0)) + v;
+              }
+            }
           }
-          prod4[ti][tj] = T(0);
+          else { /* the four k-quarters of the block, added in a fixed order, then lanes 16 i + j */
+            const T v01 = prod4[ti][tj] + __shfl_xor(prod4[ti][tj], 4);
+            const T v = v01 + __shfl_xor(v01, 8);
+            if (0 == g4) {
+              T* const slot = red + (q * NTILE + ti * TJ + tj) * 16 + ((lane >> 4) << 2) + (lane & 3);
+              *slot = (folded ? *slot : T(0)) + v;
+            }
+          }
+          prod4[ti][tj] = acc4_t{};
         }
         else if constexpr (1 == B4) {
           T* const slot = red + (q * NTILE + ti * TJ + tj) * 64 + lane;
           *slot = (folded ? *slot : T(0)) + prod4[ti][tj];
-          prod4[ti][tj] = T(0);
+          prod4[ti][tj] = acc4_t{};
         }
         else {
 #pragma unroll
@@ -470,7 +489,7 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
 
 #define LXS_LOAD_KSTEP(KBI, A0, B0, KSTEP) \
   do { \
-    const int k_ = (2 == B4 ? 16 * (KSTEP) + 4 * g4 : 4 * (KSTEP)) + lk; \
+    const int k_ = (2 == B4 ? 16 * (KSTEP) + lxs::mfma4<T>::klane(lane) : 4 * (KSTEP) + lk); \
     const bool kv_ = (k_ < K); \
     if constexpr (0 != B4) { \
       _Pragma("unroll") for (int p_ = 0; p_ < TI2; ++p_) { \
@@ -551,7 +570,7 @@ __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restri
                 for (int ti = 0; ti < TI; ++ti) {
 #pragma unroll
                   for (int tj = 0; tj < TJ; ++tj) {
-                    prod4[ti][tj] = __builtin_amdgcn_mfma_f64_4x4x4f64(av[ti], bv[tj], prod4[ti][tj], 0, 0, 0);
+                    prod4[ti][tj] = lxs::mfma4<T>::mma(av[ti], bv[tj], prod4[ti][tj]);
                   }
                 }
               }
@@ -742,7 +761,17 @@ const mfma_entry_t mfma_table[] = {
   LXS_EB(23, 23, 23, 3, 3, 8, 6) LXS_EB(5, 5, 5, 1, 1, 4, 2) LXS_EB(5, 5, 5, 1, 1, 8, 2)
   LXS_EB(13, 13, 13, 2, 2, 4, 4) LXS_EB(13, 13, 13, 2, 2, 8, 4) LXS_EB(4, 9, 22, 1, 2, 4, 6)
   LXS_EB(4, 9, 22, 1, 2, 8, 6)
-  /* entry-parallel on 4x4 blocks that split k in four (FP64): small blocks, one 16-k step (or two) per stage */
+  /* runtime shape (m <= 8*ti, n <= 8*tj), 4 k-steps per stage (2 for 16 blocks), 4 or 8 wavefronts per run */
+  LXS_EBG(1, 1, 4, 4) LXS_EBG(1, 2, 4, 4) LXS_EBG(1, 3, 4, 4) LXS_EBG(1, 4, 4, 4) LXS_EBG(2, 1, 4, 4)
+  LXS_EBG(2, 2, 4, 4) LXS_EBG(2, 3, 4, 4) LXS_EBG(2, 4, 4, 4) LXS_EBG(3, 1, 4, 4) LXS_EBG(3, 2, 4, 4)
+  LXS_EBG(3, 3, 4, 4) LXS_EBG(3, 4, 4, 4) LXS_EBG(4, 1, 4, 4) LXS_EBG(4, 2, 4, 4) LXS_EBG(4, 3, 4, 4)
+  LXS_EBG(4, 4, 4, 2) LXS_EBG(1, 1, 8, 4) LXS_EBG(1, 2, 8, 4) LXS_EBG(1, 3, 8, 4) LXS_EBG(1, 4, 8, 4)
+  LXS_EBG(2, 1, 8, 4) LXS_EBG(2, 2, 8, 4) LXS_EBG(2, 3, 8, 4) LXS_EBG(2, 4, 8, 4) LXS_EBG(3, 1, 8, 4)
+  LXS_EBG(3, 2, 8, 4) LXS_EBG(3, 3, 8, 4) LXS_EBG(3, 4, 8, 4) LXS_EBG(4, 1, 8, 4) LXS_EBG(4, 2, 8, 4)
+  LXS_EBG(4, 3, 8, 4) LXS_EBG(4, 4, 8, 2)
+#endif
+  /* entry-parallel on 4x4 blocks that split k (four quarters of 16 in FP64, sixteen single k in FP32): small
+     blocks, one 16-k step (or two) per stage */
   LXS_EK(5, 5, 5, 2, 2, 4, 1) LXS_EK(5, 5, 5, 2, 2, 8, 1) LXS_EK(13, 13, 13, 4, 4, 4, 1) LXS_EK(13, 13, 13, 4, 4, 8, 1)
   LXS_EK(4, 9, 22, 1, 3, 4, 2) LXS_EK(4, 9, 22, 1, 3, 8, 2)
   /* runtime shape (m <= 4*ti, n <= 4*tj, up to 16 x 16) */
@@ -753,15 +782,6 @@ const mfma_entry_t mfma_table[] = {
   LXS_EKG(3, 1, 8, 1) LXS_EKG(3, 2, 8, 1) LXS_EKG(3, 3, 8, 1) LXS_EKG(3, 4, 8, 1) LXS_EKG(4, 1, 8, 1) LXS_EKG(4, 2, 8, 1)
   LXS_EKG(4, 3, 8, 1) LXS_EKG(4, 4, 8, 1) LXS_EKG(1, 1, 4, 2) LXS_EKG(1, 2, 4, 2) LXS_EKG(1, 3, 4, 2) LXS_EKG(1, 4, 4, 2)
   LXS_EKG(2, 1, 4, 2) LXS_EKG(2, 2, 4, 2) LXS_EKG(3, 1, 4, 2) LXS_EKG(4, 1, 4, 2)
-  /* runtime shape (m <= 8*ti, n <= 8*tj), 4 k-steps per stage (2 for 16 blocks), 4 or 8 wavefronts per run */
-  LXS_EBG(1, 1, 4, 4) LXS_EBG(1, 2, 4, 4) LXS_EBG(1, 3, 4, 4) LXS_EBG(1, 4, 4, 4) LXS_EBG(2, 1, 4, 4)
-  LXS_EBG(2, 2, 4, 4) LXS_EBG(2, 3, 4, 4) LXS_EBG(2, 4, 4, 4) LXS_EBG(3, 1, 4, 4) LXS_EBG(3, 2, 4, 4)
-  LXS_EBG(3, 3, 4, 4) LXS_EBG(3, 4, 4, 4) LXS_EBG(4, 1, 4, 4) LXS_EBG(4, 2, 4, 4) LXS_EBG(4, 3, 4, 4)
-  LXS_EBG(4, 4, 4, 2) LXS_EBG(1, 1, 8, 4) LXS_EBG(1, 2, 8, 4) LXS_EBG(1, 3, 8, 4) LXS_EBG(1, 4, 8, 4)
-  LXS_EBG(2, 1, 8, 4) LXS_EBG(2, 2, 8, 4) LXS_EBG(2, 3, 8, 4) LXS_EBG(2, 4, 8, 4) LXS_EBG(3, 1, 8, 4)
-  LXS_EBG(3, 2, 8, 4) LXS_EBG(3, 3, 8, 4) LXS_EBG(3, 4, 8, 4) LXS_EBG(4, 1, 8, 4) LXS_EBG(4, 2, 8, 4)
-  LXS_EBG(4, 3, 8, 4) LXS_EBG(4, 4, 8, 2)
-#endif
 };
 
 } /* namespace */

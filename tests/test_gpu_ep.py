@@ -32,8 +32,8 @@ def stream():
 
 
 def _ep_configs(tid, m, n, k):
-    """every entry-parallel kernel for the shape: 16x16 tiles (variant 2) and 4x4 blocks (variants 3 and 4, FP64;
-    variant 4 splits each 16 of k over the four blocks of an instruction)"""
+    """every entry-parallel kernel for the shape: 16x16 tiles (variant 2) and 4x4 blocks (variant 3, FP64, and
+    variant 4, whose instructions split 16 of k over their blocks)"""
     out = []
     for variant, tm, tn, nt, kt in lx.kernel_candidates(tid, m, n, k):
         if variant in (2, 3, 4):
@@ -120,7 +120,7 @@ GENERIC_SHAPES = [(17, 29, 11), (31, 23, 40), (6, 21, 16), (40, 33, 9), (19, 22,
 
 def test_4x4_block_kernels_cover_their_shapes():
     """variant 3 (FP64 only) serves every shape up to 32 x 32 with exactly the blocks it needs, and no FP32 shape;
-    variant 4 every shape up to 16 x 16"""
+    variant 4 (FP64 and FP32) every shape up to 16 x 16"""
     lx.init()
     try:
         for m, n in ((5, 5), (23, 23), (13, 13), (4, 9), (9, 25), (32, 32), (1, 1), (2, 17)):
@@ -131,6 +131,8 @@ def test_4x4_block_kernels_cover_their_shapes():
         # variant 4 (4x4 blocks splitting k): exactly ti = ceil(m / 4), tj = ceil(n / 4), up to 16 x 16
         for m, n in ((5, 5), (7, 10), (14, 3), (16, 16), (1, 1)):
             v4 = [c for c in lx.kernel_candidates(lx.REAL_8, m, n, 7) if c[0] == 4]
+            assert v4 and all((c[1], c[2]) == ((m + 3) // 4, (n + 3) // 4) for c in v4), (m, n, v4)
+            v4 = [c for c in lx.kernel_candidates(lx.REAL_4, m, n, 7) if c[0] == 4]
             assert v4 and all((c[1], c[2]) == ((m + 3) // 4, (n + 3) // 4) for c in v4), (m, n, v4)
         assert not [c for c in lx.kernel_candidates(lx.REAL_8, 17, 8, 7) if c[0] == 4]
     finally:
