@@ -779,10 +779,12 @@ def _compact_ozaki(oz):
         if not ok:
             continue
         best = min(ok, key=lambda c: c["ms"])
-        out.append({"n": entry["n"], "scheme": best["scheme"], "moduli_or_slices": best.get("max_primes",
-                    best.get("slices")), "trim": best["trim"], "ms": best["ms"], "rocblas_ms": entry.get("rocblas_ms"),
+        # scheme 2 reports the moduli it used (at most max_primes), scheme 1 its slices
+        used = best.get("primes") or best.get("max_primes") or best.get("slices")
+        out.append({"n": entry["n"], "scheme": best["scheme"], "moduli_or_slices": used, "trim": best["trim"],
+                    "ms": best["ms"], "rocblas_ms": entry.get("rocblas_ms"),
                     "vs_rocblas": round(entry["rocblas_ms"] / best["ms"], 3) if entry.get("rocblas_ms") else None,
-                    "vs_own_fp64": best["speedup_vs_native"], "max_error": max(best["errors"].values())})
+                    "vs_own_fp64": best["speedup_vs_native"], "max_error": float("%.2g" % max(best["errors"].values()))})
     return out
 
 
@@ -870,7 +872,7 @@ def main():
             best = min(ok, key=lambda c: c["ms"])
             secondary["ozaki_stack_64"] = {"slices": best["slices"], "ms": best["ms"], "native_ms": ozs["native_ms"],
                                            "speedup": best["speedup_vs_native"],
-                                           "max_error": max(best["errors"].values())}
+                                           "max_error": float("%.2g" % max(best["errors"].values()))}
         print("bench: ozaki done", file=sys.stderr, flush=True)
     if extras and not args.no_learned:
         report["learned"] = rep = learned_section(lx)
