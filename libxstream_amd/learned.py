@@ -120,7 +120,7 @@ def dataset(sweep, keys):
     return np.array(xs, dtype=np.float32), np.array(ts, dtype=np.float32), np.array(gs)
 
 
-def train(sweep, train_keys, nnets=4, hidden=64, nhidden=3, seed=1, epochs=2000, lr=3e-3, weight_decay=1e-5):
+def train(sweep, train_keys, nnets=8, hidden=64, nhidden=3, seed=1, epochs=4000, lr=3e-3, weight_decay=1e-5):
     """an ensemble trained on every measured configuration of the training shapes: (model, mean loss)"""
     x, t, _ = dataset(sweep, train_keys)
     model = lx.Mlp(nnets, hidden, nhidden, seed=seed)
@@ -273,8 +273,8 @@ def main(argv):
     r.add_argument("sweeps", nargs="+")
     r.add_argument("--model", required=True, help="output model file")
     r.add_argument("--report", help="output JSON report")
-    r.add_argument("--epochs", type=int, default=2000)
-    r.add_argument("--nets", type=int, default=4)
+    r.add_argument("--epochs", type=int, default=4000)
+    r.add_argument("--nets", type=int, default=8)
     e = sub.add_parser("evaluate", help="evaluate a model on the held-out shapes of the sweeps")
     e.add_argument("sweeps", nargs="+")
     e.add_argument("--model", required=True)
