@@ -42,7 +42,7 @@ static void* work(void* arg) {
     }
     /* shared shapes: tuned rows and untuned ones that the learned slot (host inference) resolves */
     if (EXIT_SUCCESS != libsmm_acc_config_get(dbcsr_type_real_8, 23, 23, 23, &got, NULL) ||
-        EXIT_SUCCESS != libsmm_acc_config_get(dbcsr_type_real_4, 17 + i % 40, 9 + t, 33, &got, &src) ||
+        EXIT_SUCCESS != libsmm_acc_config_get(dbcsr_type_real_4, 17 + i % 10, 9 + t, 33, &got, &src) ||
         (libsmm_acc_config_learned == src && 0 == __atomic_add_fetch(&learned, 1, __ATOMIC_RELAXED)) ||
         EXIT_SUCCESS != libsmm_acc_config_heuristic(dbcsr_type_real_8, 30 + t, 31, 7 + i % 20, &got)) {
       __atomic_add_fetch(&failures, 1, __ATOMIC_RELAXED);
