@@ -275,11 +275,13 @@ __global__ void __launch_bounds__(64 * NWI) smm_mfma(T* __restrict__ cdata, cons
  * (r, c) of a k-step takes its block rows r and columns c with the rows interleaved: block row i of
  * instruction r is C row 4 TI (g % 2) + TI i + r (columns likewise), so each lane's TI rows of A (TJ columns
  * of B) are consecutive in memory and load as pairs.
- * B4 = 2 (smm_mfma_ek): the four blocks of an instruction are one 4x4 block of C over four quarters of a 16-wide
- * k-step (block g takes k = 16 s + 4 g + l / 16), so a C block up to 4 TI x 4 TJ takes TI x TJ instructions per
- * 16 of k and every lane loads distinct elements (C row TI i + r of instruction r, i = l % 4). The quarters are
- * added (lanes l, l ^ 4, l ^ 8, l ^ 12, in a fixed order) where a segment is folded into the run total, which
- * keeps one element per lane of lanes 16 i + j.
+ * B4 = 2 (smm_mfma_ek, FP64 and FP32): the blocks of an instruction are one 4x4 block of C over consecutive k
+ * of a 16-wide k-step: in FP64 four blocks of 4 k (block g takes k = 16 s + 4 g + l / 16), in FP32 the sixteen
+ * blocks of v_mfma_f32_4x4x1_16b one k each (k = 16 s + l / 4; lxs::mfma4). A C block up to 4 TI x 4 TJ takes
+ * TI x TJ instructions per 16 of k and every lane loads distinct elements (C row TI i + r of instruction r,
+ * i = l % 4). The k-blocks are added in a fixed order where a segment is folded into the run total (FP64: lanes
+ * l, l ^ 4, l ^ 8, l ^ 12, keeping element (i, j) in lane 16 i + j; FP32: lanes l ^ 4 .. l ^ 32, keeping row i
+ * of column j in register i of lane j).
  */
 template <typename T, int TI, int TJ, int EP, int KB, int MC, int NC, int KC, int B4>
 __device__ __forceinline__ void ep_body(T* __restrict__ cdata, const T* __restrict__ adata,
