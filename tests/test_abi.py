@@ -221,6 +221,23 @@ def test_default_config_valid(shape, tid):
     assert src in (0, 1, 3) and lx.config_valid(tid, *shape, cfg)
 
 
+@pytest.mark.parametrize("shape,tid,want", [
+    ((5, 5, 5), lx.REAL_8, (4, 2, 2, 256)),      # FP64 up to 16x16: 4x4 blocks splitting k, exact block grid
+    ((4, 9, 22), lx.REAL_8, (4, 1, 3, 256)),
+    ((16, 16, 16), lx.REAL_8, (4, 4, 4, 256)),
+    ((17, 5, 5), lx.REAL_8, (2, 2, 1, 512)),     # beyond 16 rows: 16x16 tiles, 8 wavefronts up to 4 tiles
+    ((5, 5, 5), lx.REAL_4, (2, 1, 1, 512)),      # FP32: 16x16 tiles
+    ((30, 60, 9), lx.REAL_4, (2, 2, 4, 256)),    # 8 tiles: 4 wavefronts per run
+    ((40, 40, 9), lx.REAL_4, (2, 3, 3, 128)),    # 9 tiles: 2 wavefronts per run
+    ((64, 64, 64), lx.REAL_8, (2, 4, 4, 128)),   # 16 tiles: 2 wavefronts per run
+    ((80, 80, 80), lx.REAL_8, (1, 5, 5, 320))])  # more than 16 tiles: one wavefront per run, a wavefront per tile row
+def test_default_heuristic_choices(shape, tid, want):
+    """the built-in choice for untuned shapes (libsmm_acc_config_heuristic): entry-parallel kernels up to 16 tiles"""
+    cfg = lx.default_config(tid, *shape)
+    assert (cfg.variant, cfg.tm, cfg.tn, cfg.nt) == want, (shape, cfg.variant, cfg.tm, cfg.tn, cfg.nt)
+    assert cfg.bs == (16 if tid == lx.REAL_4 and cfg.variant == 2 else 32) or cfg.variant not in (2, 4)
+
+
 def test_config_validation():
     tid = lx.REAL_8
     assert not lx.config_valid(tid, 23, 23, 23, lx.SmmConfig(variant=0, tm=3, tn=3, nt=32, bs=32))   # too few threads
